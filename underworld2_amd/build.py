@@ -1,0 +1,64 @@
+"""Build libuwg.so (HIP, gfx950) in-tree.
+
+Each translation unit is compiled with hipcc for gfx950 with packed-FP32 instructions
+disabled.  swarm_kernels.hip holds the bit-exact particle paths and is compiled with
+-ffp-contract=off so no multiply-add is fused; the solver kernels allow contraction.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_objs")
+LIB = os.path.join(HERE, "libuwg.so")
+ARCH = os.environ.get("UWG_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wno-unused-result", "-Wno-unknown-pragmas"]
+UNITS = {
+    "swarm_kernels.hip": ["-ffp-contract=off"],
+    "relocate.hip": ["-ffp-contract=off"],
+    "fem_kernels.hip": ["-ffp-contract=fast"],
+    "nn_kernels.hip": ["-ffp-contract=fast"],
+    "uwg_host.hip": ["-ffp-contract=off"],
+}
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if c and (os.path.exists(c) or c == "hipcc"):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(unit):
+    src = os.path.join(SRC, unit)
+    obj = os.path.join(OBJ, unit.replace(".hip", ".o"))
+    cmd = [hipcc(), "-c", src, "-o", obj] + COMMON + UNITS[unit]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
+        objs = list(ex.map(_compile, UNITS))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

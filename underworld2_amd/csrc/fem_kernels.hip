@@ -1,0 +1,976 @@
+// Finite-element kernels of the Stokes solve: particle -> element quadrature moments,
+// Galerkin coarse moments, matrix-free K / G / G^T applies, diagonal, transfer
+// operators and vector kernels.
+//
+// Quadrature representation.  On an affine Q1 element every entry of
+//   K_e = sum_p eta_p w_p detJ  B(xi_p)^T D0 B(xi_p),  D0 = diag(2,2,2,1,1,1)
+// (_ConstitutiveMatrixCartesian_AssembleElement, LU/Underworld/Rheology/src/
+// ConstitutiveMatrixCartesian.cpp:238-429) is a polynomial of degree <= 2 per axis in xi,
+// so K_e depends on the particles only through the 3^d moments
+//   M_k = sum_p eta_p w_p detJ xi^kx eta^ky zeta^kz .
+// We store, per element, the 3^d weights W_g of the Gauss points that reproduce these
+// moments exactly (W = (V^-1 (x) V^-1 (x) V^-1) M with V the 3x3 Vandermonde of the
+// Gauss abscissae).  K_e = sum_g W_g B(g)^T D0 B(g) then holds to roundoff, for any
+// particle distribution (W may be negative).  The same identity gives the Galerkin coarse
+// operator P^T K P of trilinear prolongation exactly: the children's Gauss points act as
+// the coarse element's quadrature points.
+#include "uwg_internal.h"
+
+namespace uwg {
+
+static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+#define GA 0.77459666924148337703585307995647992216658434105832  // sqrt(3/5)
+#define GA2 0.6  // GA^2
+
+// 1D: values of the Gauss abscissae
+__host__ __device__ constexpr double gpt(int i) { return i == 0 ? -GA : (i == 1 ? 0.0 : GA); }
+
+// 1D inverse Vandermonde: (M0, M1, M2) -> (W0, W1, W2)
+__host__ __device__ inline void vinv1(double m0, double m1, double m2, double* w) {
+  double a = m2 / GA2, b = m1 / GA;
+  w[0] = 0.5 * (a - b);
+  w[1] = m0 - a;
+  w[2] = 0.5 * (a + b);
+}
+
+// tensor transform of moments M[kz][ky][kx] (27 or 9) into Gauss weights, in place
+template <int DIM>
+__device__ inline void moments_to_weights(double* M) {
+  constexpr int Q1 = 3;
+  // along x
+  for (int r = 0; r < (DIM == 3 ? 9 : 3); r++) {
+    double w[3];
+    vinv1(M[r * 3 + 0], M[r * 3 + 1], M[r * 3 + 2], w);
+    M[r * 3 + 0] = w[0]; M[r * 3 + 1] = w[1]; M[r * 3 + 2] = w[2];
+  }
+  // along y
+  for (int kz = 0; kz < (DIM == 3 ? 3 : 1); kz++)
+    for (int kx = 0; kx < Q1; kx++) {
+      double w[3];
+      int b = kz * 9 + kx;
+      vinv1(M[b], M[b + 3], M[b + 6], w);
+      M[b] = w[0]; M[b + 3] = w[1]; M[b + 6] = w[2];
+    }
+  if (DIM == 3)
+    for (int r = 0; r < 9; r++) {
+      double w[3];
+      vinv1(M[r], M[r + 9], M[r + 18], w);
+      M[r] = w[0]; M[r + 9] = w[1]; M[r + 18] = w[2];
+    }
+}
+
+// derivative dN_a/dxi_j of the Q1 shape function of local node a at reference point X
+template <int DIM>
+__host__ __device__ inline void dN_ref(int a, const double* X, double* d) {
+  double s[3] = {(a & 1) ? 1.0 : -1.0, (a & 2) ? 1.0 : -1.0, (a & 4) ? 1.0 : -1.0};
+  if (DIM == 2) {
+    d[0] = 0.25 * s[0] * (1.0 + s[1] * X[1]);
+    d[1] = 0.25 * s[1] * (1.0 + s[0] * X[0]);
+  } else {
+    d[0] = 0.125 * s[0] * (1.0 + s[1] * X[1]) * (1.0 + s[2] * X[2]);
+    d[1] = 0.125 * s[1] * (1.0 + s[0] * X[0]) * (1.0 + s[2] * X[2]);
+    d[2] = 0.125 * s[2] * (1.0 + s[0] * X[0]) * (1.0 + s[1] * X[1]);
+  }
+}
+
+// element stiffness diagonal K[(a,i),(a,i)] = sum_g W_g ((dN_a/dx_i)^2 + sum_j (dN_a/dx_j)^2)
+template <int DIM>
+__device__ inline double kdiag_entry(const double* Wg, int a, int i, const double* sc) {
+  double acc = 0.0;
+  for (int g = 0; g < (DIM == 3 ? 27 : 9); g++) {
+    double X[3] = {gpt(g % 3), gpt((g / 3) % 3), gpt(g / 9)};
+    double d[3];
+    dN_ref<DIM>(a, X, d);
+    double s2 = 0.0;
+    for (int j = 0; j < DIM; j++) { d[j] *= sc[j]; s2 += d[j] * d[j]; }
+    acc += Wg[g] * (d[i] * d[i] + s2);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// A6/A7/A9 particle -> element moments, element force vector, approxS diagonal.
+// One thread per element, particles in cell-table (slot) order.  Function values by
+// particle id: material tables (mat != null) or per-particle arrays.
+//   W[g*E+e], fe[e*NN*DIM + a*DIM + i], approxS[e]
+// f_e: VectorAssemblyTerm_NA__Fn.cpp:143-202; approxS: UzawaPreconditionerTerm.c:124-209
+// ---------------------------------------------------------------------------------------
+struct MomArgs {
+  int dim, res[3];
+  double h[3], detJ;
+  long E;
+  int nmat;
+  double Ge[24];  // constant element gradient matrix, passed by value
+};
+
+template <int DIM>
+__global__ void k_moments(MomArgs A, const int* __restrict__ cell_ptr, const int* __restrict__ cell_pid,
+                          const double* __restrict__ xi_slot, const double* __restrict__ w_slot,
+                          const int* __restrict__ mat, const double* __restrict__ eta_tab,
+                          const double* __restrict__ force_tab, const double* __restrict__ eta_p,
+                          const double* __restrict__ force_p, double* __restrict__ W, double* __restrict__ fe,
+                          double* __restrict__ approxS) {
+  constexpr int NN = DIM == 3 ? 8 : 4, Q = DIM == 3 ? 27 : 9;
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.E) return;
+  double M[27], F[NN * DIM];
+  for (int q = 0; q < Q; q++) M[q] = 0.0;
+  for (int q = 0; q < NN * DIM; q++) F[q] = 0.0;
+  const int s0 = cell_ptr[e], s1 = cell_ptr[e + 1];
+  for (int s = s0; s < s1; s++) {
+    const int p = cell_pid[s];
+    double xi[3];
+    for (int d = 0; d < DIM; d++) xi[d] = xi_slot[(long)s * DIM + d];
+    const double w = w_slot[s] * A.detJ;
+    double eta, f[3];
+    if (mat) {
+      int m = mat[p];
+      if (m < 0 || m >= A.nmat) {  // index outside the table: poison the element
+        m = 0;
+        eta = __builtin_nan("");
+      } else {
+        eta = eta_tab[m];
+      }
+      for (int d = 0; d < DIM; d++) f[d] = force_tab[m * DIM + d];
+    } else {
+      eta = eta_p[p];
+      for (int d = 0; d < DIM; d++) f[d] = force_p[(long)p * DIM + d];
+    }
+    const double om = eta * w;
+    double px[3] = {1.0, xi[0], xi[0] * xi[0]}, py[3] = {1.0, xi[1], xi[1] * xi[1]};
+    if (DIM == 3) {
+      double pz[3] = {1.0, xi[2], xi[2] * xi[2]};
+      for (int kz = 0; kz < 3; kz++)
+        for (int ky = 0; ky < 3; ky++) {
+          double t = om * pz[kz] * py[ky];
+          for (int kx = 0; kx < 3; kx++) M[kz * 9 + ky * 3 + kx] += t * px[kx];
+        }
+    } else {
+      for (int ky = 0; ky < 3; ky++) {
+        double t = om * py[ky];
+        for (int kx = 0; kx < 3; kx++) M[ky * 3 + kx] += t * px[kx];
+      }
+    }
+    // f_e[a][i] += w detJ f_i N_a(xi)
+    double lx[2] = {0.5 * (1.0 - xi[0]), 0.5 * (1.0 + xi[0])};
+    double ly[2] = {0.5 * (1.0 - xi[1]), 0.5 * (1.0 + xi[1])};
+    double lz[2] = {1.0, 1.0};
+    if (DIM == 3) { lz[0] = 0.5 * (1.0 - xi[2]); lz[1] = 0.5 * (1.0 + xi[2]); }
+    for (int a = 0; a < NN; a++) {
+      double Na = lx[a & 1] * ly[(a >> 1) & 1] * lz[(a >> 2) & 1];
+      for (int d = 0; d < DIM; d++) F[a * DIM + d] += w * f[d] * Na;
+    }
+  }
+  moments_to_weights<DIM>(M);
+  for (int q = 0; q < Q; q++) W[(long)q * A.E + e] = M[q];
+  for (int q = 0; q < NN * DIM; q++) fe[e * NN * DIM + q] = F[q];
+  // approxS_e = sum_v Ge[v]^2 / Ke[v][v]
+  double sc[3] = {2.0 / A.h[0], 2.0 / A.h[1], DIM == 3 ? 2.0 / A.h[2] : 1.0};
+  double S = 0.0;
+  for (int a = 0; a < NN; a++)
+    for (int i = 0; i < DIM; i++) {
+      double kd = kdiag_entry<DIM>(M, a, i, sc);
+      double g = A.Ge[a * DIM + i];
+      S += g * (g / kd);
+    }
+  approxS[e] = S;
+}
+
+static MomArgs mom_args(const Geom& g) {
+  MomArgs A;
+  A.dim = g.dim;
+  A.detJ = 1.0;
+  for (int d = 0; d < 3; d++) {
+    A.res[d] = g.res[d];
+    A.h[d] = d < g.dim ? (g.hi[d] - g.lo[d]) / g.res[d] : 1.0;
+    if (d < g.dim) A.detJ *= 0.5 * A.h[d];
+  }
+  A.E = g.E;
+  A.nmat = 0;
+  return A;
+}
+
+void launch_moments(const Geom& g, const int* cell_ptr, const int* cell_pid, const double* xi_slot,
+                    const double* w_slot, const int* mat, const double* eta_tab, const double* force_tab,
+                    const double* eta_p, const double* force_p, double* W, double* fe, double* approxS,
+                    const double* Ge, int nmat, hipStream_t s) {
+  MomArgs A = mom_args(g);
+  A.nmat = nmat;
+  for (int q = 0; q < 24; q++) A.Ge[q] = q < (1 << g.dim) * g.dim ? Ge[q] : 0.0;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_moments<3>, nblocks(g.E, 128), 128, 0, s, A, cell_ptr, cell_pid, xi_slot, w_slot, mat,
+                       eta_tab, force_tab, eta_p, force_p, W, fe, approxS);
+  else
+    hipLaunchKernelGGL(k_moments<2>, nblocks(g.E, 128), 128, 0, s, A, cell_ptr, cell_pid, xi_slot, w_slot, mat,
+                       eta_tab, force_tab, eta_p, force_p, W, fe, approxS);
+  UWG_HIP(hipGetLastError());
+}
+
+// f[n][i] = sum over the elements around node n of fe (fixed element order); 0 on BC dofs
+template <int DIM>
+__global__ void k_gather_fe(int rx, int ry, int rz, long N, const double* __restrict__ fe,
+                            const uint8_t* __restrict__ bc, double* __restrict__ f) {
+  constexpr int NN = DIM == 3 ? 8 : 4;
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  long nx = rx + 1, ny = ry + 1;
+  int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / (nx * ny));
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int oz = 0; oz < (DIM == 3 ? 2 : 1); oz++)
+    for (int oy = 0; oy < 2; oy++)
+      for (int ox = 0; ox < 2; ox++) {
+        int ei = i - 1 + ox, ej = j - 1 + oy, ek = DIM == 3 ? k - 1 + oz : 0;
+        if (ei < 0 || ei >= rx || ej < 0 || ej >= ry || ek < 0 || ek >= rz) continue;
+        long e = ei + (long)rx * (ej + (long)ry * ek);
+        int a = (1 - ox) + 2 * (1 - oy) + (DIM == 3 ? 4 * (1 - oz) : 0);
+        for (int d = 0; d < DIM; d++) acc[d] += fe[e * NN * DIM + a * DIM + d];
+      }
+  uint8_t m = bc[n];
+  for (int d = 0; d < DIM; d++) f[n * DIM + d] = ((m >> d) & 1) ? 0.0 : acc[d];
+}
+
+void launch_node_gather_fe(const Geom& g, const double* fe, const uint8_t* bc, double* f, hipStream_t s) {
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_gather_fe<3>, nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.N, fe, bc, f);
+  else
+    hipLaunchKernelGGL(k_gather_fe<2>, nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], 1, g.N, fe, bc, f);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// Galerkin coarse weights: children's Gauss points mapped into the coarse element
+// (xi_c = (xi_child + s)/2, s = -1/+1) give the coarse moments exactly.
+// ---------------------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_restrict_W(int crx, int cry, int crz, long Ec, long Ef, const double* __restrict__ Wf,
+                             double* __restrict__ Wc) {
+  constexpr int Q = DIM == 3 ? 27 : 9;
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Ec) return;
+  int ci = (int)(e % crx), cj = (int)((e / crx) % cry), ck = (int)(e / ((long)crx * cry));
+  int frx = 2 * crx, fry = 2 * cry;
+  double M[27];
+  for (int q = 0; q < Q; q++) M[q] = 0.0;
+  for (int oz = 0; oz < (DIM == 3 ? 2 : 1); oz++)
+    for (int oy = 0; oy < 2; oy++)
+      for (int ox = 0; ox < 2; ox++) {
+        long fe = (2 * ci + ox) + (long)frx * ((2 * cj + oy) + (long)fry * (DIM == 3 ? (2 * ck + oz) : 0));
+        double sx = ox ? 1.0 : -1.0, sy = oy ? 1.0 : -1.0, sz = oz ? 1.0 : -1.0;
+        for (int g = 0; g < Q; g++) {
+          double w = Wf[(long)g * Ef + fe];
+          double X = 0.5 * (gpt(g % 3) + sx), Y = 0.5 * (gpt((g / 3) % 3) + sy);
+          double px[3] = {1.0, X, X * X}, py[3] = {1.0, Y, Y * Y};
+          if (DIM == 3) {
+            double Z = 0.5 * (gpt(g / 9) + sz);
+            double pz[3] = {1.0, Z, Z * Z};
+            for (int kz = 0; kz < 3; kz++)
+              for (int ky = 0; ky < 3; ky++)
+                for (int kx = 0; kx < 3; kx++) M[kz * 9 + ky * 3 + kx] += w * pz[kz] * py[ky] * px[kx];
+          } else {
+            for (int ky = 0; ky < 3; ky++)
+              for (int kx = 0; kx < 3; kx++) M[ky * 3 + kx] += w * py[ky] * px[kx];
+          }
+        }
+      }
+  moments_to_weights<DIM>(M);
+  for (int q = 0; q < Q; q++) Wc[(long)q * Ec + e] = M[q];
+}
+
+void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, double* Wc, hipStream_t s) {
+  if (coarse.dim == 3)
+    hipLaunchKernelGGL(k_restrict_W<3>, nblocks(coarse.E, 128), 128, 0, s, coarse.res[0], coarse.res[1],
+                       coarse.res[2], coarse.E, fine.E, Wf, Wc);
+  else
+    hipLaunchKernelGGL(k_restrict_W<2>, nblocks(coarse.E, 128), 128, 0, s, coarse.res[0], coarse.res[1], 1,
+                       coarse.E, fine.E, Wf, Wc);
+  UWG_HIP(hipGetLastError());
+}
+
+// inverse diagonal of K (identity rows on Dirichlet dofs)
+template <int DIM>
+__global__ void k_diag(int rx, int ry, int rz, long N, long E, double hx, double hy, double hz,
+                       const double* __restrict__ W, const uint8_t* __restrict__ bc, double* __restrict__ dinv) {
+  constexpr int Q = DIM == 3 ? 27 : 9;
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  long nx = rx + 1, ny = ry + 1;
+  int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / (nx * ny));
+  double sc[3] = {2.0 / hx, 2.0 / hy, 2.0 / hz};
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int oz = 0; oz < (DIM == 3 ? 2 : 1); oz++)
+    for (int oy = 0; oy < 2; oy++)
+      for (int ox = 0; ox < 2; ox++) {
+        int ei = i - 1 + ox, ej = j - 1 + oy, ek = DIM == 3 ? k - 1 + oz : 0;
+        if (ei < 0 || ei >= rx || ej < 0 || ej >= ry || ek < 0 || ek >= rz) continue;
+        long e = ei + (long)rx * (ej + (long)ry * ek);
+        int a = (1 - ox) + 2 * (1 - oy) + (DIM == 3 ? 4 * (1 - oz) : 0);
+        double Wg[27];
+        for (int g = 0; g < Q; g++) Wg[g] = W[(long)g * E + e];
+        for (int d = 0; d < DIM; d++) acc[d] += kdiag_entry<DIM>(Wg, a, d, sc);
+      }
+  uint8_t m = bc[n];
+  for (int d = 0; d < DIM; d++) dinv[n * DIM + d] = ((m >> d) & 1) ? 1.0 : 1.0 / acc[d];
+}
+
+void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s) {
+  double h[3];
+  for (int d = 0; d < 3; d++) h[d] = d < g.dim ? (g.hi[d] - g.lo[d]) / g.res[d] : 1.0;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_diag<3>, nblocks(g.N, 128), 128, 0, s, g.res[0], g.res[1], g.res[2], g.N, g.E, h[0], h[1],
+                       h[2], W, bc, dinv);
+  else
+    hipLaunchKernelGGL(k_diag<2>, nblocks(g.N, 128), 128, 0, s, g.res[0], g.res[1], 1, g.N, g.E, h[0], h[1], h[2],
+                       W, bc, dinv);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// Matrix-free K apply.  Element kernel in monomial form: u = c0 + cx X + cy Y + cz Z +
+// cxy XY + cxz XZ + cyz YZ + cxyz XYZ per component; gradients at the 3^d Gauss points
+// are 3-term sums; stresses sigma = W (G + G^T); the transposed contraction gives the
+// dual coefficients and r_a = (1/8) H^T dc.
+//
+// 3D tiling: a 16x16 block of element columns streams up a z-chunk.  Each thread owns one
+// element column; node (lx,ly) with 1 <= lx,ly <= 15 sums the four surrounding
+// elements' corner contributions from LDS in a fixed order (deterministic, no atomics).
+// Tiles overlap by one element so every node has exactly one owner.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+struct KArgs {
+  int rx, ry, rz, KZ;
+  T sx, sy, sz;
+  long E, nx, nxy;
+};
+
+template <typename T>
+__device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, const T (&u)[8][3], T (&r)[8][3],
+                                      T sx, T sy, T sz) {
+  const T q8 = (T)0.125;
+  T cX[3], cY[3], cZ[3], cXY[3], cXZ[3], cYZ[3], cXYZ[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    T a0 = u[0][i], a1 = u[1][i], a2 = u[2][i], a3 = u[3][i], a4 = u[4][i], a5 = u[5][i], a6 = u[6][i], a7 = u[7][i];
+    // x differences / sums
+    T d01 = a1 - a0, d23 = a3 - a2, d45 = a5 - a4, d67 = a7 - a6;
+    T s01 = a1 + a0, s23 = a3 + a2, s45 = a5 + a4, s67 = a7 + a6;
+    cX[i] = q8 * (d01 + d23 + d45 + d67);
+    cXY[i] = q8 * ((d23 - d01) + (d67 - d45));
+    cXZ[i] = q8 * ((d45 - d01) + (d67 - d23));
+    cXYZ[i] = q8 * ((d67 - d45) - (d23 - d01));
+    cY[i] = q8 * ((s23 - s01) + (s67 - s45));
+    cZ[i] = q8 * ((s45 - s01) + (s67 - s23));
+    cYZ[i] = q8 * ((s67 - s45) - (s23 - s01));
+  }
+  T dX[3] = {0, 0, 0}, dY[3] = {0, 0, 0}, dZ[3] = {0, 0, 0}, dXY[3] = {0, 0, 0}, dXZ[3] = {0, 0, 0},
+    dYZ[3] = {0, 0, 0}, dXYZ[3] = {0, 0, 0};
+#pragma unroll
+  for (int g = 0; g < 27; g++) {
+    const T X = (T)gpt(g % 3), Y = (T)gpt((g / 3) % 3), Z = (T)gpt(g / 9);
+    const T w = W[(long)g * E + e];
+    T G[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      G[i][0] = sx * (cX[i] + cXY[i] * Y + cXZ[i] * Z + cXYZ[i] * (Y * Z));
+      G[i][1] = sy * (cY[i] + cXY[i] * X + cYZ[i] * Z + cXYZ[i] * (X * Z));
+      G[i][2] = sz * (cZ[i] + cXZ[i] * X + cYZ[i] * Y + cXYZ[i] * (X * Y));
+    }
+    const T w2 = w + w;
+    T S00 = w2 * G[0][0], S11 = w2 * G[1][1], S22 = w2 * G[2][2];
+    T S01 = w * (G[0][1] + G[1][0]), S02 = w * (G[0][2] + G[2][0]), S12 = w * (G[1][2] + G[2][1]);
+    T S[3][3] = {{S00, S01, S02}, {S01, S11, S12}, {S02, S12, S22}};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      T px = S[i][0] * sx, py = S[i][1] * sy, pz = S[i][2] * sz;
+      dX[i] += px;
+      dY[i] += py;
+      dZ[i] += pz;
+      dXY[i] += px * Y + py * X;
+      dXZ[i] += px * Z + pz * X;
+      dYZ[i] += py * Z + pz * Y;
+      dXYZ[i] += px * (Y * Z) + py * (X * Z) + pz * (X * Y);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    // r_a = (1/8) (s_a dX + t_a dY + r_a dZ + s t dXY + s r dXZ + t r dYZ + s t r dXYZ)
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const T s = (a & 1) ? 1 : -1, t = (a & 2) ? 1 : -1, rr = (a & 4) ? 1 : -1;
+      r[a][i] = q8 * (s * dX[i] + t * dY[i] + rr * dZ[i] + (s * t) * dXY[i] + (s * rr) * dXZ[i] + (t * rr) * dYZ[i] +
+                      (s * t * rr) * dXYZ[i]);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_level3(const KArgs<T>& A, const T* __restrict__ x, const uint8_t* __restrict__ bc,
+                                            int ei, int ej, int lev, T (&u)[8][3], int off) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    int ni = ei + (c & 1), nj = ej + (c >> 1);
+    bool ok = ni >= 0 && ni <= A.rx && nj >= 0 && nj <= A.ry && lev >= 0 && lev <= A.rz;
+    if (ok) {
+      long n = ni + A.nx * nj + A.nxy * lev;
+      uint8_t m = bc[n];
+#pragma unroll
+      for (int i = 0; i < 3; i++) u[off + c][i] = ((m >> i) & 1) ? (T)0 : x[n * 3 + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; i++) u[off + c][i] = (T)0;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply3(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
+                                                T* __restrict__ y, const uint8_t* __restrict__ bc) {
+  __shared__ T sb[12][256];
+  __shared__ T st[12][256];
+  const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
+  const int ei = blockIdx.x * 15 - 1 + lx, ej = blockIdx.y * 15 - 1 + ly;
+  const int k0 = blockIdx.z * A.KZ;
+  const int k1 = min(k0 + A.KZ, A.rz + 1);
+  const bool colE = ei >= 0 && ei < A.rx && ej >= 0 && ej < A.ry;
+  const bool ownN = lx >= 1 && ly >= 1 && ei <= A.rx && ej <= A.ry;
+  T u[8][3];
+  load_level3(A, x, bc, ei, ej, k0 - 1, u, 0);
+  for (int ek = k0 - 1; ek < k1; ++ek) {
+    load_level3(A, x, bc, ei, ej, ek + 1, u, 4);
+    T r[8][3];
+    if (colE && ek >= 0 && ek < A.rz) {
+      long e = ei + (long)A.rx * (ej + (long)A.ry * ek);
+      elem3<T>(W, A.E, e, u, r, A.sx, A.sy, A.sz);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) r[a][i] = (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) sb[c * 3 + i][tid] = r[c][i];
+    __syncthreads();
+    if (ownN && ek >= k0) {
+      long n = ei + A.nx * ej + A.nxy * ek;
+      uint8_t m = bc[n];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        T s = st[9 + i][tid - 17] + st[6 + i][tid - 16] + st[3 + i][tid - 1] + st[i][tid] + sb[9 + i][tid - 17] +
+              sb[6 + i][tid - 16] + sb[3 + i][tid - 1] + sb[i][tid];
+        y[n * 3 + i] = ((m >> i) & 1) ? x[n * 3 + i] : s;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) st[c * 3 + i][tid] = r[4 + c][i];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) u[c][i] = u[4 + c][i];
+  }
+}
+
+// 2D: 64 element columns per block stream along y
+template <typename T>
+__device__ __forceinline__ void elem2(const T* __restrict__ W, long E, long e, const T (&u)[4][2], T (&r)[4][2], T sx,
+                                      T sy) {
+  const T q4 = (T)0.25;
+  T cX[2], cY[2], cXY[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    T a0 = u[0][i], a1 = u[1][i], a2 = u[2][i], a3 = u[3][i];
+    cX[i] = q4 * ((a1 - a0) + (a3 - a2));
+    cY[i] = q4 * ((a2 - a0) + (a3 - a1));
+    cXY[i] = q4 * ((a3 - a2) - (a1 - a0));
+  }
+  T dX[2] = {0, 0}, dY[2] = {0, 0}, dXY[2] = {0, 0};
+#pragma unroll
+  for (int g = 0; g < 9; g++) {
+    const T X = (T)gpt(g % 3), Y = (T)gpt(g / 3);
+    const T w = W[(long)g * E + e];
+    T G[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      G[i][0] = sx * (cX[i] + cXY[i] * Y);
+      G[i][1] = sy * (cY[i] + cXY[i] * X);
+    }
+    const T w2 = w + w;
+    T S00 = w2 * G[0][0], S11 = w2 * G[1][1], S01 = w * (G[0][1] + G[1][0]);
+    T S[2][2] = {{S00, S01}, {S01, S11}};
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      T px = S[i][0] * sx, py = S[i][1] * sy;
+      dX[i] += px;
+      dY[i] += py;
+      dXY[i] += px * Y + py * X;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const T s = (a & 1) ? 1 : -1, t = (a & 2) ? 1 : -1;
+      r[a][i] = q4 * (s * dX[i] + t * dY[i] + (s * t) * dXY[i]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_level2(const KArgs<T>& A, const T* __restrict__ x, const uint8_t* __restrict__ bc,
+                                            int ei, int lev, T (&u)[4][2], int off) {
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    int ni = ei + c;
+    bool ok = ni >= 0 && ni <= A.rx && lev >= 0 && lev <= A.ry;
+    if (ok) {
+      long n = ni + A.nx * lev;
+      uint8_t m = bc[n];
+#pragma unroll
+      for (int i = 0; i < 2; i++) u[off + c][i] = ((m >> i) & 1) ? (T)0 : x[n * 2 + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; i++) u[off + c][i] = (T)0;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
+                                               T* __restrict__ y, const uint8_t* __restrict__ bc) {
+  __shared__ T sb[4][64];
+  __shared__ T st[4][64];
+  const int tid = threadIdx.x;
+  const int ei = blockIdx.x * 63 - 1 + tid;
+  const int k0 = blockIdx.y * A.KZ;
+  const int k1 = min(k0 + A.KZ, A.ry + 1);
+  const bool colE = ei >= 0 && ei < A.rx;
+  const bool ownN = tid >= 1 && ei <= A.rx;
+  T u[4][2];
+  load_level2(A, x, bc, ei, k0 - 1, u, 0);
+  for (int ek = k0 - 1; ek < k1; ++ek) {
+    load_level2(A, x, bc, ei, ek + 1, u, 2);
+    T r[4][2];
+    if (colE && ek >= 0 && ek < A.ry) {
+      long e = ei + (long)A.rx * ek;
+      elem2<T>(W, A.E, e, u, r, A.sx, A.sy);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 4; a++) r[a][0] = r[a][1] = (T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) sb[c * 2 + i][tid] = r[c][i];
+    __syncthreads();
+    if (ownN && ek >= k0) {
+      long n = ei + A.nx * ek;
+      uint8_t m = bc[n];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        T s = st[2 + i][tid - 1] + st[i][tid] + sb[2 + i][tid - 1] + sb[i][tid];
+        y[n * 2 + i] = ((m >> i) & 1) ? x[n * 2 + i] : s;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) st[c * 2 + i][tid] = r[2 + c][i];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) u[c][i] = u[2 + c][i];
+  }
+}
+
+static int g_kz3 = 8, g_kz2 = 32;
+void set_apply_chunk(int kz3, int kz2) {
+  if (kz3 > 0) g_kz3 = kz3;
+  if (kz2 > 0) g_kz2 = kz2;
+}
+
+template <typename T>
+void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s) {
+  KArgs<T> A;
+  A.rx = g.res[0];
+  A.ry = g.res[1];
+  A.rz = g.res[2];
+  A.sx = (T)(2.0 / ((g.hi[0] - g.lo[0]) / g.res[0]));
+  A.sy = (T)(2.0 / ((g.hi[1] - g.lo[1]) / g.res[1]));
+  A.sz = g.dim == 3 ? (T)(2.0 / ((g.hi[2] - g.lo[2]) / g.res[2])) : (T)1;
+  A.E = g.E;
+  A.nx = g.res[0] + 1;
+  A.nxy = A.nx * (g.res[1] + 1);
+  if (g.dim == 3) {
+    A.KZ = g_kz3;
+    dim3 grid((g.res[0] + 1 + 14) / 15, (g.res[1] + 1 + 14) / 15, (g.res[2] + 1 + A.KZ - 1) / A.KZ);
+    hipLaunchKernelGGL(k_apply3<T>, grid, 256, 0, s, A, W, x, y, bc);
+  } else {
+    A.KZ = g_kz2;
+    dim3 grid((g.res[0] + 1 + 62) / 63, (g.res[1] + 1 + A.KZ - 1) / A.KZ);
+    hipLaunchKernelGGL(k_apply2<T>, grid, 64, 0, s, A, W, x, y, bc);
+  }
+  UWG_HIP(hipGetLastError());
+}
+template void launch_apply_K<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t);
+template void launch_apply_K<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t);
+
+// ---------------------------------------------------------------------------------------
+// G p (node gather) and G^T u (element gather); G_e constant on a regular mesh
+// (_GradientStiffnessMatrixTerm_AssembleElement, GradientStiffnessMatrixTerm.c:129-221)
+// ---------------------------------------------------------------------------------------
+struct GeConst {
+  double v[24];
+};
+
+template <int DIM>
+__global__ void k_apply_G(int rx, int ry, int rz, long N, GeConst Ge, const double* __restrict__ p,
+                          const uint8_t* __restrict__ bc, double* __restrict__ y) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  long nx = rx + 1, ny = ry + 1;
+  int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / (nx * ny));
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int oz = 0; oz < (DIM == 3 ? 2 : 1); oz++)
+    for (int oy = 0; oy < 2; oy++)
+      for (int ox = 0; ox < 2; ox++) {
+        int ei = i - 1 + ox, ej = j - 1 + oy, ek = DIM == 3 ? k - 1 + oz : 0;
+        if (ei < 0 || ei >= rx || ej < 0 || ej >= ry || ek < 0 || ek >= rz) continue;
+        long e = ei + (long)rx * (ej + (long)ry * ek);
+        int a = (1 - ox) + 2 * (1 - oy) + (DIM == 3 ? 4 * (1 - oz) : 0);
+        double pe = p[e];
+        for (int d = 0; d < DIM; d++) acc[d] += Ge.v[a * DIM + d] * pe;
+      }
+  uint8_t m = bc[n];
+  for (int d = 0; d < DIM; d++) y[n * DIM + d] = ((m >> d) & 1) ? 0.0 : acc[d];
+}
+
+template <int DIM>
+__global__ void k_apply_GT(int rx, int ry, int rz, long E, GeConst Ge, const double* __restrict__ u,
+                           const uint8_t* __restrict__ bc, double* __restrict__ q) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  long nx = rx + 1, ny = ry + 1;
+  int i = (int)(e % rx), j = (int)((e / rx) % ry), k = DIM == 3 ? (int)(e / ((long)rx * ry)) : 0;
+  long base = i + nx * (j + ny * k);
+  double acc = 0.0;
+  for (int a = 0; a < (DIM == 3 ? 8 : 4); a++) {
+    long n = base + (a & 1) + ((a >> 1) & 1) * nx + (DIM == 3 ? ((a >> 2) & 1) * nx * ny : 0);
+    uint8_t m = bc[n];
+    for (int d = 0; d < DIM; d++)
+      if (!((m >> d) & 1)) acc += Ge.v[a * DIM + d] * u[n * DIM + d];
+  }
+  q[e] = acc;
+}
+
+void launch_apply_G(const Geom& g, const double* Ge, const double* p, const uint8_t* bc, double* y, hipStream_t s) {
+  GeConst G;
+  for (int q = 0; q < 24; q++) G.v[q] = q < (1 << g.dim) * g.dim ? Ge[q] : 0.0;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_apply_G<3>, nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.N, G, p, bc, y);
+  else
+    hipLaunchKernelGGL(k_apply_G<2>, nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], 1, g.N, G, p, bc, y);
+  UWG_HIP(hipGetLastError());
+}
+
+void launch_apply_GT(const Geom& g, const double* Ge, const double* u, const uint8_t* bc, double* q, hipStream_t s) {
+  GeConst G;
+  for (int k = 0; k < 24; k++) G.v[k] = k < (1 << g.dim) * g.dim ? Ge[k] : 0.0;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_apply_GT<3>, nblocks(g.E, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.E, G, u, bc, q);
+  else
+    hipLaunchKernelGGL(k_apply_GT<2>, nblocks(g.E, 256), 256, 0, s, g.res[0], g.res[1], 1, g.E, G, u, bc, q);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// Transfer operators: trilinear prolongation (a fine node averages its 1, 2, 4 or 8
+// coarse parents, SROpGenerator.c:748-1052) and its transpose.  Dirichlet dofs are
+// excluded on both levels.
+// ---------------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void k_prolong_add(int fnx, int fny, int fnz, int cnx, int cny, long Nf, const T* __restrict__ xc,
+                              T* __restrict__ xf, const uint8_t* __restrict__ bcf) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= Nf) return;
+  int I = (int)(n % fnx), J = (int)((n / fnx) % fny), K = (int)(n / ((long)fnx * fny));
+  T acc[3] = {0, 0, 0};
+  int ni = (I & 1) ? 2 : 1, nj = (J & 1) ? 2 : 1, nk = DIM == 3 ? ((K & 1) ? 2 : 1) : 1;
+  T wgt = (T)1;
+  if (I & 1) wgt *= (T)0.5;
+  if (J & 1) wgt *= (T)0.5;
+  if (DIM == 3 && (K & 1)) wgt *= (T)0.5;
+  for (int c = 0; c < nk; c++)
+    for (int b = 0; b < nj; b++)
+      for (int a = 0; a < ni; a++) {
+        int ci = (I >> 1) + a, cj = (J >> 1) + b, ck = DIM == 3 ? (K >> 1) + c : 0;
+        long cn = ci + (long)cnx * (cj + (long)cny * ck);
+        for (int d = 0; d < DIM; d++) acc[d] += xc[cn * DIM + d];
+      }
+  uint8_t m = bcf[n];
+  for (int d = 0; d < DIM; d++)
+    if (!((m >> d) & 1)) xf[n * DIM + d] += wgt * acc[d];
+}
+
+template <typename T, int DIM>
+__global__ void k_restrict(int fnx, int fny, int fnz, int cnx, int cny, int cnz, long Nc, const T* __restrict__ rf,
+                           T* __restrict__ rc, const uint8_t* __restrict__ bcc, const uint8_t* __restrict__ bcf) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= Nc) return;
+  int i = (int)(n % cnx), j = (int)((n / cnx) % cny), k = (int)(n / ((long)cnx * cny));
+  T acc[3] = {0, 0, 0};
+  for (int oz = (DIM == 3 ? -1 : 0); oz <= (DIM == 3 ? 1 : 0); oz++) {
+    int K = 2 * k + oz;
+    if (K < 0 || K >= fnz) continue;
+    for (int oy = -1; oy <= 1; oy++) {
+      int J = 2 * j + oy;
+      if (J < 0 || J >= fny) continue;
+      for (int ox = -1; ox <= 1; ox++) {
+        int I = 2 * i + ox;
+        if (I < 0 || I >= fnx) continue;
+        T w = (T)1;
+        if (ox) w *= (T)0.5;
+        if (oy) w *= (T)0.5;
+        if (oz) w *= (T)0.5;
+        long fn = I + (long)fnx * (J + (long)fny * K);
+        uint8_t mf = bcf[fn];
+        for (int d = 0; d < DIM; d++)
+          if (!((mf >> d) & 1)) acc[d] += w * rf[fn * DIM + d];
+      }
+    }
+  }
+  uint8_t m = bcc[n];
+  for (int d = 0; d < DIM; d++) rc[n * DIM + d] = ((m >> d) & 1) ? (T)0 : acc[d];
+}
+
+template <typename T>
+void launch_prolong_add(const Geom& f, const Geom& c, const T* xc, T* xf, const uint8_t* bcf, hipStream_t s) {
+  if (f.dim == 3)
+    hipLaunchKernelGGL((k_prolong_add<T, 3>), nblocks(f.N, 256), 256, 0, s, f.nn[0], f.nn[1], f.nn[2], c.nn[0], c.nn[1],
+                       f.N, xc, xf, bcf);
+  else
+    hipLaunchKernelGGL((k_prolong_add<T, 2>), nblocks(f.N, 256), 256, 0, s, f.nn[0], f.nn[1], 1, c.nn[0], c.nn[1], f.N,
+                       xc, xf, bcf);
+  UWG_HIP(hipGetLastError());
+}
+template void launch_prolong_add<double>(const Geom&, const Geom&, const double*, double*, const uint8_t*, hipStream_t);
+template void launch_prolong_add<float>(const Geom&, const Geom&, const float*, float*, const uint8_t*, hipStream_t);
+
+// restriction needs both masks; the fine mask is passed through a global set by the host
+template <typename T>
+void launch_restrict2(const Geom& f, const Geom& c, const T* rf, T* rc, const uint8_t* bcc, const uint8_t* bcf,
+                      hipStream_t s) {
+  if (f.dim == 3)
+    hipLaunchKernelGGL((k_restrict<T, 3>), nblocks(c.N, 256), 256, 0, s, f.nn[0], f.nn[1], f.nn[2], c.nn[0], c.nn[1],
+                       c.nn[2], c.N, rf, rc, bcc, bcf);
+  else
+    hipLaunchKernelGGL((k_restrict<T, 2>), nblocks(c.N, 256), 256, 0, s, f.nn[0], f.nn[1], 1, c.nn[0], c.nn[1], 1, c.N,
+                       rf, rc, bcc, bcf);
+  UWG_HIP(hipGetLastError());
+}
+template void launch_restrict2<double>(const Geom&, const Geom&, const double*, double*, const uint8_t*, const uint8_t*,
+                                       hipStream_t);
+template void launch_restrict2<float>(const Geom&, const Geom&, const float*, float*, const uint8_t*, const uint8_t*,
+                                      hipStream_t);
+
+// Dirichlet mask for walls: wallbits bit (2*d + side) set -> component d fixed on that wall
+__global__ void k_bc_walls(int nx, int ny, int nz, int dim, long N, int wallbits, uint8_t* bc) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  int idx[3] = {(int)(n % nx), (int)((n / nx) % ny), (int)(n / ((long)nx * ny))};
+  int nn[3] = {nx, ny, nz};
+  uint8_t m = 0;
+  for (int d = 0; d < dim; d++) {
+    bool lo = idx[d] == 0, hi = idx[d] == nn[d] - 1;
+    if ((lo && (wallbits >> (2 * d) & 1)) || (hi && (wallbits >> (2 * d + 1) & 1))) m |= (uint8_t)(1 << d);
+  }
+  bc[n] = m;
+}
+void launch_bc_mask_from_walls(const Geom& g, uint8_t* bc, int wallbits, hipStream_t s) {
+  hipLaunchKernelGGL(k_bc_walls, nblocks(g.N, 256), 256, 0, s, g.nn[0], g.nn[1], g.nn[2], g.dim, g.N, wallbits, bc);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(long n, T v, T* x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+template <typename T>
+__global__ void k_copy(long n, const T* x, T* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = x[i];
+}
+template <typename T>
+__global__ void k_axpy(long n, T a, const T* x, T* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+template <typename T>
+__global__ void k_axpby(long n, T a, const T* x, T b, T* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+template <typename T>
+__global__ void k_xmy(long n, const T* d, const T* x, T* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = d[i] * x[i];
+}
+template <typename T>
+__global__ void k_residual(long n, const T* b, T* r) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) r[i] = b[i] - r[i];
+}
+template <typename T>
+__global__ void k_cheb(long n, T a, T b, const T* dinv, const T* r, T* d, T* x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    T v = a * d[i] + b * (dinv[i] * r[i]);
+    d[i] = v;
+    x[i] += v;
+  }
+}
+__global__ void k_d2f(long n, double s, const double* x, float* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = (float)(s * x[i]);
+}
+__global__ void k_f2d_axpy(long n, double s, const float* x, double* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] += s * (double)x[i];
+}
+__global__ void k_f2d(long n, double s, const float* x, double* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = s * (double)x[i];
+}
+
+static unsigned vgrid(long n) {
+  long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+template <typename T> void vk_fill(long n, T v, T* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill<T>, vgrid(n), 256, 0, s, n, v, x);
+}
+template <typename T> void vk_copy(long n, const T* x, T* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy<T>, vgrid(n), 256, 0, s, n, x, y);
+}
+template <typename T> void vk_axpy(long n, T a, const T* x, T* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy<T>, vgrid(n), 256, 0, s, n, a, x, y);
+}
+template <typename T> void vk_axpby(long n, T a, const T* x, T b, T* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby<T>, vgrid(n), 256, 0, s, n, a, x, b, y);
+}
+template <typename T> void vk_xmy_scale(long n, const T* d, const T* x, T* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_xmy<T>, vgrid(n), 256, 0, s, n, d, x, y);
+}
+template <typename T> void vk_residual(long n, const T* b, T* r, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual<T>, vgrid(n), 256, 0, s, n, b, r);
+}
+template <typename T>
+void vk_cheb_update(long n, T a, T b, const T* dinv, const T* r, T* d, T* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb<T>, vgrid(n), 256, 0, s, n, a, b, dinv, r, d, x);
+}
+#define INST(T)                                                               \
+  template void vk_fill<T>(long, T, T*, hipStream_t);                         \
+  template void vk_copy<T>(long, const T*, T*, hipStream_t);                  \
+  template void vk_axpy<T>(long, T, const T*, T*, hipStream_t);               \
+  template void vk_axpby<T>(long, T, const T*, T, T*, hipStream_t);           \
+  template void vk_xmy_scale<T>(long, const T*, const T*, T*, hipStream_t);   \
+  template void vk_residual<T>(long, const T*, T*, hipStream_t);              \
+  template void vk_cheb_update<T>(long, T, T, const T*, const T*, T*, T*, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+void vk_d2f(long n, double scale, const double* x, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_d2f, vgrid(n), 256, 0, s, n, scale, x, y);
+}
+void vk_f2d_axpy(long n, double scale, const float* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d_axpy, vgrid(n), 256, 0, s, n, scale, x, y);
+}
+void vk_f2d(long n, double scale, const float* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d, vgrid(n), 256, 0, s, n, scale, x, y);
+}
+
+// y (+)= sum_j c[j] X[j]; c and X live in device memory (k <= 64)
+__global__ void k_multi_axpy(long n, int k, const double* const* X, const double* c, double* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double acc = y[i];
+    for (int j = 0; j < k; j++) acc += c[j] * X[j][i];
+    y[i] = acc;
+  }
+}
+void vk_multi_axpy(long n, int k, const double* const* X, const double* c, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_multi_axpy, vgrid(n), 256, 0, s, n, k, X, c, y);
+}
+
+// deterministic multi-dot: fixed grid of RB blocks, fixed-order tree in each block, then
+// one block sums the RB partials in order.
+#define RB 1024
+int vk_reduce_blocks() { return RB; }
+template <int KM>
+__global__ __launch_bounds__(256) void k_mdot(long n, int k, const double* __restrict__ x, const double* const* Y,
+                                              double* partial) {
+  __shared__ double sm[KM][256];
+  double acc[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) acc[j] = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double xv = x[i];
+#pragma unroll
+    for (int j = 0; j < KM; j++)
+      if (j < k) acc[j] += xv * Y[j][i];
+  }
+#pragma unroll
+  for (int j = 0; j < KM; j++) sm[j][threadIdx.x] = acc[j];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int j = 0; j < KM; j++) sm[j][threadIdx.x] += sm[j][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < KM && threadIdx.x < k) partial[threadIdx.x * RB + blockIdx.x] = sm[threadIdx.x][0];
+}
+__global__ void k_mdot_final(int k, const double* partial, double* out) {
+  __shared__ double sm[256];
+  for (int j = 0; j < k; j++) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < RB; b += 256) a += partial[j * RB + b];
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = sm[0];
+    __syncthreads();
+  }
+}
+void vk_multi_dot(long n, int k, const double* x, const double* const* Y, double* partial, double* out, hipStream_t s) {
+  if (k <= 1)
+    hipLaunchKernelGGL(k_mdot<1>, RB, 256, 0, s, n, k, x, Y, partial);
+  else if (k <= 4)
+    hipLaunchKernelGGL(k_mdot<4>, RB, 256, 0, s, n, k, x, Y, partial);
+  else
+    hipLaunchKernelGGL(k_mdot<8>, RB, 256, 0, s, n, k, x, Y, partial);
+  hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, k, partial, out);
+  UWG_HIP(hipGetLastError());
+}
+
+// dense y = A x (A symmetric, n x n, column-major)
+__global__ void k_dense_mv(int n, const double* A, const double* x, double* y) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < n; c++) acc += A[(long)c * n + r] * x[c];
+  y[r] = acc;
+}
+void launch_dense_mv(int n, const double* A, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_dense_mv, (n + 127) / 128, 128, 0, s, n, A, x, y);
+  UWG_HIP(hipGetLastError());
+}
+
+}  // namespace uwg

@@ -1,0 +1,440 @@
+// Bit-exact swarm kernels: random layout, cell search, coincident map (Newton),
+// discrete-Voronoi weights, RK2 advection and interpolation.
+// Compiled with -ffp-contract=off: every floating-point expression keeps the reference's
+// operation order (citations in uwg_geom.h and below; LU/ = src/underworld/libUnderworld/).
+#include "uwg_internal.h"
+#include "uwg_geom.h"
+
+namespace uwg {
+
+struct GeomArgs {
+  int res[3];
+  double lo[3], hi[3];
+  long E;
+};
+static GeomArgs pack(const Geom& g) {
+  GeomArgs a;
+  for (int d = 0; d < 3; d++) {
+    a.res[d] = g.res[d];
+    a.lo[d] = g.lo[d];
+    a.hi[d] = g.hi[d];
+  }
+  a.E = g.E;
+  return a;
+}
+
+static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------------------
+// A1 random layout.  The reference draws from one Park-Miller stream (Random.c:36-53),
+// cell by cell, particle by particle, dimension by dimension
+// (MeshParticleLayout.c:194-229).  The generator is multiplicative (x -> 16807 x mod
+// 2^31-1, which Schrage's method evaluates exactly), so draw j equals
+// 16807^(j+1) * seed0 mod m and every particle can jump straight to its own draws.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long mulmod(unsigned long long a, unsigned long long b) {
+  return (a * b) % 2147483647ULL;
+}
+__device__ unsigned long long powmod(unsigned long long base, unsigned long long e) {
+  unsigned long long r = 1;
+  base %= 2147483647ULL;
+  while (e) {
+    if (e & 1ULL) r = mulmod(r, base);
+    base = mulmod(base, base);
+    e >>= 1;
+  }
+  return r;
+}
+
+template <int DIM>
+__global__ void k_layout_random(GeomArgs g, int ppc, unsigned long long s0, double* coords, long P) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  long e = p / ppc;
+  unsigned long long st = mulmod(powmod(16807ULL, (unsigned long long)p * DIM), s0);
+  double xi[3], v[24], N[8];
+  for (int d = 0; d < DIM; d++) {
+    st = mulmod(st, 16807ULL);
+    double r = (double)(long)st;
+    xi[d] = (1.0 - -1.0) * r / (double)2147483647L + -1.0;
+  }
+  element_verts<DIM>(g.res, g.lo, g.hi, e, v);
+  shape<DIM>(xi, N);
+  double x[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < (DIM == 3 ? 8 : 4); a++)
+    for (int d = 0; d < DIM; d++) x[d] += N[a] * v[a * DIM + d];
+  for (int d = 0; d < DIM; d++) coords[p * DIM + d] = x[d];
+}
+
+void launch_layout_random(const Geom& g, int ppc, long seed, double* coords, hipStream_t s) {
+  long P = g.E * ppc;
+  unsigned long long s0 = (unsigned long long)(seed ^ 123459876L);
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_layout_random<3>, nblocks(P, 256), 256, 0, s, pack(g), ppc, s0, coords, P);
+  else
+    hipLaunchKernelGGL(k_layout_random<2>, nblocks(P, 256), 256, 0, s, pack(g), ppc, s0, coords, P);
+  UWG_HIP(hipGetLastError());
+}
+
+template <int DIM>
+__global__ void k_mesh_coords(GeomArgs g, double* out, long N) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  long nx = g.res[0] + 1, ny = g.res[1] + 1;
+  int idx[3] = {(int)(n % nx), (int)((n / nx) % ny), (int)(n / (nx * ny))};
+  for (int d = 0; d < DIM; d++) out[n * DIM + d] = node_coord_1d(idx[d], g.res[d] + 1, g.lo[d], g.hi[d]);
+}
+
+void launch_mesh_coords(const Geom& g, double* out, hipStream_t s) {
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_mesh_coords<3>, nblocks(g.N, 256), 256, 0, s, pack(g), out, g.N);
+  else
+    hipLaunchKernelGGL(k_mesh_coords<2>, nblocks(g.N, 256), 256, 0, s, pack(g), out, g.N);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// A17 cell search for every particle (owner = E when outside the domain)
+// ---------------------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_cell_of(GeomArgs g, long P, const double* coords, int* owner) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double x[3];
+  for (int d = 0; d < DIM; d++) x[d] = coords[p * DIM + d];
+  owner[p] = (int)cell_of<DIM>(g.res, g.lo, g.hi, x, g.E);
+}
+
+void launch_cell_of(const Geom& g, long P, const double* coords, int* owner, hipStream_t s) {
+  if (P == 0) return;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_cell_of<3>, nblocks(P, 256), 256, 0, s, pack(g), P, coords, owner);
+  else
+    hipLaunchKernelGGL(k_cell_of<2>, nblocks(P, 256), 256, 0, s, pack(g), P, coords, owner);
+  UWG_HIP(hipGetLastError());
+}
+
+// slot -> owning cell, from the CSR cell pointer
+__global__ void k_slot_cell(long E, const int* cell_ptr, int* slot_cell) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  for (int s = cell_ptr[e]; s < cell_ptr[e + 1]; s++) slot_cell[s] = (int)e;
+}
+void launch_slot_cell(long E, const int* cell_ptr, int* slot_cell, hipStream_t s) {
+  hipLaunchKernelGGL(k_slot_cell, nblocks(E, 256), 256, 0, s, E, cell_ptr, slot_cell);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// A3 coincident map: xi of every slot by Newton in its cell (CoincidentMapper.c:76-130)
+// ---------------------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_coincident(GeomArgs g, long P, const int* slot_cell, const int* cell_pid,
+                             const double* coords, double* xi_slot) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P) return;
+  double v[24], x[3], xi[3];
+  element_verts<DIM>(g.res, g.lo, g.hi, slot_cell[s], v);
+  long p = cell_pid[s];
+  for (int d = 0; d < DIM; d++) x[d] = coords[p * DIM + d];
+  newton_local<DIM>(v, x, xi);
+  for (int d = 0; d < DIM; d++) xi_slot[s * DIM + d] = xi[d];
+}
+
+void launch_coincident_map(const Geom& g, long P, const int* slot_cell, const int* cell_pid,
+                           const double* coords, double* xi_slot, hipStream_t s) {
+  if (P == 0) return;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_coincident<3>, nblocks(P, 256), 256, 0, s, pack(g), P, slot_cell, cell_pid, coords, xi_slot);
+  else
+    hipLaunchKernelGGL(k_coincident<2>, nblocks(P, 256), 256, 0, s, pack(g), P, slot_cell, cell_pid, coords, xi_slot);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// A4 discrete Voronoi cell weights, one thread per element, state in a per-thread
+// scratch region.  Same growth as _DVCWeights_CreateVoronoi3D/2D (DVCWeights.c:676-765):
+// seed, then rounds over particles 0..n-1 of ClaimCells (:256-311: claim unowned, steal
+// only when DistanceTest > 0) followed by UpdateBchain (:388-438: neighbours S,N,E,W,U,D
+// of newly claimed cells, first occurrence kept).  Centroids/weights as GetCentroids
+// (:198-254).  A thread that would exceed its list capacity sets its flag and writes
+// nothing; the host re-runs such elements with larger capacities.
+// ---------------------------------------------------------------------------------------
+struct DvcLayout {
+  long owner_off, done_off, cnt_off, nb_off, claimed_off, bound_off, bytes;
+};
+__host__ __device__ inline DvcLayout dvc_layout(int ncell, int max_np, int cap) {
+  DvcLayout L;
+  long o = 0;
+  auto al = [](long v) { return (v + 15) & ~15L; };
+  L.owner_off = o; o = al(o + 2L * ncell);
+  L.done_off = o; o = al(o + ncell);
+  L.cnt_off = o; o = al(o + 4L * max_np);
+  L.nb_off = o; o = al(o + 4L * max_np);
+  L.claimed_off = o; o = al(o + 2L * cap);
+  L.bound_off = o; o = al(o + 2L * cap * max_np);
+  L.bytes = o;
+  return L;
+}
+
+template <int DIM>
+__global__ void k_dvc(int num, long E0, long nE, const int* elist, const int* cell_ptr, double* xi_slot,
+                      double* w_slot, char* scratch, int max_np, int cap, int* flag) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nE) return;
+  const long e = elist ? (long)elist[t] : E0 + t;
+  const int nx = num, ny = num, nz = DIM == 3 ? num : 1;
+  const int ncell = nx * ny * nz;
+  const int nnb = DIM == 3 ? 6 : 4;
+  const int s0 = cell_ptr[e];
+  const int np = cell_ptr[e + 1] - s0;
+  if (np <= 0) { flag[t] = 2; return; }     // under-resolved cell (reference firewall)
+  if (np > max_np) { flag[t] = 1; return; }
+  const DvcLayout L = dvc_layout(ncell, max_np, cap);
+  char* base = scratch + (long)t * L.bytes;
+  short* own = (short*)(base + L.owner_off);
+  unsigned char* done = (unsigned char*)(base + L.done_off);
+  int* cnt = (int*)(base + L.cnt_off);
+  int* nbound = (int*)(base + L.nb_off);
+  short* claimed = (short*)(base + L.claimed_off);
+  short* bound = (short*)(base + L.bound_off);
+  const double dx = (1.0 - -1.0) / nx, dy = (1.0 - -1.0) / ny, dz = (1.0 - -1.0) / nz;
+  // cell centres: X = X + dx accumulation per axis (_DVCWeights_ConstructGrid :527-544)
+  double cx[32], cy[32], cz[32];
+  {
+    double X = -1.0 - dx / 2.0;
+    for (int i = 0; i < nx; i++) { X = X + dx; cx[i] = X; }
+    double Y = -1.0 - dy / 2.0;
+    for (int j = 0; j < ny; j++) { Y = Y + dy; cy[j] = Y; }
+    double Z = -1.0 - dz / 2.0;
+    for (int k = 0; k < nz; k++) { Z = Z + dz; cz[k] = Z; }
+  }
+  for (int c = 0; c < ncell; c++) { own[c] = -1; done[c] = 0; }
+  const double* pos = xi_slot + (long)s0 * DIM;
+  bool overflow = false;
+
+  auto neighbour = [&](int c, int k) -> int {
+    int i = c % nx, j = (c / nx) % ny, kk = c / (nx * ny);
+    switch (k) {
+      case 0: return j > 0 ? c - nx : -2;            // S
+      case 1: return j < ny - 1 ? c + nx : -2;       // N
+      case 2: return i < nx - 1 ? c + 1 : -2;        // E
+      case 3: return i > 0 ? c - 1 : -2;             // W
+      case 4: return kk < nz - 1 ? c + nx * ny : -2; // U
+      default: return kk > 0 ? c - nx * ny : -2;     // D
+    }
+  };
+  // UpdateBchain for particle pi from its claimed list (length ncl)
+  auto update = [&](int pi, int ncl) {
+    short* b = bound + (long)pi * cap;
+    int nb = 0;
+    for (int q = 0; q < ncl && !overflow; q++) {
+      int c0 = claimed[q];
+      for (int k = 0; k < nnb; k++) {
+        int c1 = neighbour(c0, k);
+        if (c1 != -2 && own[c1] != pi && done[c1] != 1) {
+          if (nb >= cap) { overflow = true; break; }
+          b[nb++] = (short)c1;
+          done[c1] = 1;
+        }
+      }
+    }
+    for (int q = 0; q < nb; q++) done[b[q]] = 0;
+    nbound[pi] = nb;
+  };
+  for (int i = 0; i < np && !overflow; i++) {
+    int k = (int)((pos[i * DIM + 0] - -1.0) / dx);
+    int j = (int)((pos[i * DIM + 1] - -1.0) / dy);
+    int l = DIM == 3 ? (int)((pos[i * DIM + 2] - -1.0) / dz) : 0;
+    if (k == nx) k--;
+    if (j == ny) j--;
+    if (DIM == 3 && l == nz) l--;
+    // memory guard only: local coordinates lie in [-1, 1] up to roundoff
+    k = k < 0 ? 0 : (k > nx - 1 ? nx - 1 : k);
+    j = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+    l = l < 0 ? 0 : (l > nz - 1 ? nz - 1 : l);
+    int id = k + j * nx + l * nx * ny;
+    own[id] = (short)i;
+    claimed[0] = (short)id;
+    update(i, 1);
+  }
+  int claimed_total = 1;
+  while (claimed_total != 0 && !overflow) {
+    claimed_total = 0;
+    for (int pi = 0; pi < np && !overflow; pi++) {
+      const short* b = bound + (long)pi * cap;
+      int nb = nbound[pi], ncl = 0;
+      for (int q = 0; q < nb; q++) {
+        int c0 = b[q];
+        int o = own[c0];
+        if (o == -1) {
+          claimed[ncl++] = (short)c0;
+          own[c0] = (short)pi;
+        } else if (o != pi) {
+          int ci = c0 % nx, cj = (c0 / nx) % ny, ck = c0 / (nx * ny);
+          double x0 = cx[ci], y0 = cy[cj], z0 = cz[ck];
+          const double* p1 = pos + o * DIM;
+          const double* p2 = pos + pi * DIM;
+          double r = (p1[0] + p2[0] - x0 - x0) * (p1[0] - p2[0]) + (p1[1] + p2[1] - y0 - y0) * (p1[1] - p2[1]);
+          if (DIM == 3) r = r + (p1[2] + p2[2] - z0 - z0) * (p1[2] - p2[2]);
+          if (r > 0.0) {
+            claimed[ncl++] = (short)c0;
+            own[c0] = (short)pi;
+          }
+        }
+      }
+      claimed_total += ncl;
+      update(pi, ncl);
+    }
+  }
+  if (overflow) { flag[t] = 1; return; }
+  // centroids and weights, grid-index order (_DVCWeights_GetCentroids3D :198-226)
+  double da = DIM == 3 ? dx * dy * dz : dx * dy;
+  double* out = xi_slot + (long)s0 * DIM;
+  for (int i = 0; i < np; i++) {
+    cnt[i] = 0;
+    for (int d = 0; d < DIM; d++) out[i * DIM + d] = 0.0;
+  }
+  for (int c = 0; c < ncell; c++) {
+    int p = own[c];
+    if (p < 0) continue;
+    int ci = c % nx, cj = (c / nx) % ny, ck = c / (nx * ny);
+    out[p * DIM + 0] += cx[ci];
+    out[p * DIM + 1] += cy[cj];
+    if (DIM == 3) out[p * DIM + 2] += cz[ck];
+    cnt[p]++;
+  }
+  for (int i = 0; i < np; i++) {
+    w_slot[s0 + i] = cnt[i] * da;
+    if (cnt[i] != 0)
+      for (int d = 0; d < DIM; d++) out[i * DIM + d] /= cnt[i];
+  }
+  flag[t] = 0;
+}
+
+long dvc_scratch_bytes(int dim, int num, int max_np, int cap) {
+  int ncell = num * num * (dim == 3 ? num : 1);
+  return dvc_layout(ncell, max_np, cap).bytes;
+}
+
+void launch_dvc_list(int dim, int num, long E0, long nE, const int* elist, const int* cell_ptr,
+                     double* xi_slot, double* w_slot, void* scratch, int max_np, int cap, int* d_flag,
+                     hipStream_t s) {
+  if (nE == 0) return;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_dvc<3>, nblocks(nE, 64), 64, 0, s, num, E0, nE, elist, cell_ptr, xi_slot, w_slot,
+                       (char*)scratch, max_np, cap, d_flag);
+  else
+    hipLaunchKernelGGL(k_dvc<2>, nblocks(nE, 64), 64, 0, s, num, E0, nE, elist, cell_ptr, xi_slot, w_slot,
+                       (char*)scratch, max_np, cap, d_flag);
+  UWG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// A15/A16 RK2 advection (TimeIntegrand_SecondOrder, TimeIntegrand.c:249-319, fallback
+// :441-476 since SwarmAdvector allows it).  Velocity lookup: regular search, Newton,
+// sum_a v_a N_a in element-node order (FeVariable.c:389-403, 1007-1037).
+// ---------------------------------------------------------------------------------------
+template <int DIM>
+__device__ bool velocity_at(const GeomArgs& g, const double* vel, const double* x, double* v) {
+  long el;
+  if (!search_element<DIM>(g.res, g.lo, g.hi, x, &el)) return false;
+  double verts[24], xi[3], N[8];
+  element_verts<DIM>(g.res, g.lo, g.hi, el, verts);
+  newton_local<DIM>(verts, x, xi);
+  shape<DIM>(xi, N);
+  long b = element_base_node<DIM>(g.res, el);
+  long nx = g.res[0] + 1, nxy = nx * (g.res[1] + 1);
+  for (int d = 0; d < DIM; d++) v[d] = 0.0;
+  for (int a = 0; a < (DIM == 3 ? 8 : 4); a++) {
+    long n = b + (a & 1) + ((a >> 1) & 1) * nx + (DIM == 3 ? ((a >> 2) & 1) * nxy : 0);
+    for (int d = 0; d < DIM; d++) v[d] += vel[n * DIM + d] * N[a];
+  }
+  for (int d = 0; d < DIM; d++)
+    if (isinf(v[d])) return false;
+  return true;
+}
+
+template <int DIM>
+__global__ void k_advect_rk2(GeomArgs g, long P, const double* vel, double* coords, double dt, int* counts) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double x[3], start[3], v0[3], v1[3];
+  for (int d = 0; d < DIM; d++) start[d] = x[d] = coords[p * DIM + d];
+  if (!velocity_at<DIM>(g, vel, x, v0)) {
+    atomicAdd(&counts[0], 1);
+    return;
+  }
+  for (int d = 0; d < DIM; d++) x[d] = start[d] + 0.5 * dt * v0[d];
+  if (velocity_at<DIM>(g, vel, x, v1)) {
+    for (int d = 0; d < DIM; d++) x[d] = start[d] + dt * v1[d];
+  } else {
+    for (int d = 0; d < DIM; d++) x[d] = start[d];
+    velocity_at<DIM>(g, vel, x, v0);
+    for (int d = 0; d < DIM; d++) x[d] = start[d] + dt * v0[d];
+    atomicAdd(&counts[1], 1);
+  }
+  for (int d = 0; d < DIM; d++) coords[p * DIM + d] = x[d];
+}
+
+void launch_advect_rk2(const Geom& g, long P, const double* vel, double* coords, double dt, int* d_counts,
+                       hipStream_t s) {
+  if (P == 0) return;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_advect_rk2<3>, nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+  else
+    hipLaunchKernelGGL(k_advect_rk2<2>, nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+  UWG_HIP(hipGetLastError());
+}
+
+template <int DIM>
+__global__ void k_interp(GeomArgs g, long P, const double* vel, const double* x, double* v, int* ok) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double xx[3], vv[3];
+  for (int d = 0; d < DIM; d++) xx[d] = x[p * DIM + d];
+  bool f = velocity_at<DIM>(g, vel, xx, vv);
+  ok[p] = f ? 1 : 0;
+  for (int d = 0; d < DIM; d++) v[p * DIM + d] = f ? vv[d] : 0.0;
+}
+
+void launch_interpolate(const Geom& g, long P, const double* vel, const double* x, double* v, int* ok,
+                        hipStream_t s) {
+  if (P == 0) return;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_interp<3>, nblocks(P, 256), 256, 0, s, pack(g), P, vel, x, v, ok);
+  else
+    hipLaunchKernelGGL(k_interp<2>, nblocks(P, 256), 256, 0, s, pack(g), P, vel, x, v, ok);
+  UWG_HIP(hipGetLastError());
+}
+
+// A18 max over nodes of |v| (FeVariable_GetScalarAtNode, FeVariable.c:453-469): the max
+// is order independent, so block partials give the exact reference value.
+template <int DIM>
+__global__ void k_speed_max(long N, const double* vel, double* partial) {
+  __shared__ double sm[256];
+  double m = 0.0;
+  for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int d = 0; d < DIM; d++) s += vel[n * DIM + d] * vel[n * DIM + d];
+    double c = sqrt(s);
+    if (c > m) m = c;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w && sm[threadIdx.x + w] > sm[threadIdx.x]) sm[threadIdx.x] = sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+void launch_node_speed_max(const Geom& g, const double* vel, double* partial, int nblk, hipStream_t s) {
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_speed_max<3>, nblk, 256, 0, s, g.N, vel, partial);
+  else
+    hipLaunchKernelGGL(k_speed_max<2>, nblk, 256, 0, s, g.N, vel, partial);
+  UWG_HIP(hipGetLastError());
+}
+
+}  // namespace uwg

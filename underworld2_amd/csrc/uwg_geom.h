@@ -1,0 +1,218 @@
+// Geometry primitives of the regular Q1 mesh used by the bit-exact swarm kernels.
+// Every routine keeps the reference's IEEE operation order; the translation unit that
+// includes this header is compiled with -ffp-contract=off so no multiply-add is fused.
+// Citations: LU/ = src/underworld/libUnderworld/.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace uwg {
+
+#define UWG_HD __host__ __device__ __forceinline__
+
+// CartesianGenerator_CalcGeom, LU/StgDomain/Mesh/src/CartesianGenerator.c:2001-2039
+UWG_HD double node_coord_1d(int i, int nnodes, double lo, double hi) {
+  if (i <= 0) return lo;
+  if (i >= nnodes - 1) return hi;
+  double steps = hi - lo;
+  return (lo + 0.0) + ((double)i / (double)(nnodes - 1)) * (steps - 0.0 * 0.0);
+}
+
+// Element-local vertex coordinates, local order x fastest (TrilinearElementType.c:216-233)
+template <int DIM>
+UWG_HD void element_verts(const int* res, const double* lo, const double* hi, long e, double* v) {
+  long i = e % res[0];
+  long j = (e / res[0]) % res[1];
+  long k = DIM == 3 ? e / ((long)res[0] * res[1]) : 0;
+  double X[2] = {node_coord_1d((int)i, res[0] + 1, lo[0], hi[0]), node_coord_1d((int)i + 1, res[0] + 1, lo[0], hi[0])};
+  double Y[2] = {node_coord_1d((int)j, res[1] + 1, lo[1], hi[1]), node_coord_1d((int)j + 1, res[1] + 1, lo[1], hi[1])};
+  if (DIM == 2) {
+    for (int a = 0; a < 4; a++) {
+      v[a * 2 + 0] = X[a & 1];
+      v[a * 2 + 1] = Y[(a >> 1) & 1];
+    }
+  } else {
+    double Z[2] = {node_coord_1d((int)k, res[2] + 1, lo[2], hi[2]), node_coord_1d((int)k + 1, res[2] + 1, lo[2], hi[2])};
+    for (int a = 0; a < 8; a++) {
+      v[a * 3 + 0] = X[a & 1];
+      v[a * 3 + 1] = Y[(a >> 1) & 1];
+      v[a * 3 + 2] = Z[(a >> 2) & 1];
+    }
+  }
+}
+
+template <int DIM>
+UWG_HD long element_base_node(const int* res, long e) {
+  long nx = res[0] + 1, ny = res[1] + 1;
+  long i = e % res[0];
+  long j = (e / res[0]) % res[1];
+  long k = DIM == 3 ? e / ((long)res[0] * res[1]) : 0;
+  return i + nx * (j + ny * k);
+}
+
+// Shape functions exactly as written in Bilinear/TrilinearElementType.c
+template <int DIM>
+UWG_HD void shape(const double* xi, double* N) {
+  if (DIM == 2) {
+    double x = xi[0], y = xi[1];
+    N[0] = 0.25 * (1.0 - x) * (1.0 - y);
+    N[1] = 0.25 * (1.0 + x) * (1.0 - y);
+    N[2] = 0.25 * (1.0 - x) * (1.0 + y);
+    N[3] = 0.25 * (1.0 + x) * (1.0 + y);
+  } else {
+    double x = xi[0], y = xi[1], z = xi[2];
+    N[0] = 0.125 * (1.0 - x) * (1.0 - y) * (1.0 - z);
+    N[2] = 0.125 * (1.0 - x) * (1.0 + y) * (1.0 - z);
+    N[3] = 0.125 * (1.0 + x) * (1.0 + y) * (1.0 - z);
+    N[1] = 0.125 * (1.0 + x) * (1.0 - y) * (1.0 - z);
+    N[4] = 0.125 * (1.0 - x) * (1.0 - y) * (1.0 + z);
+    N[6] = 0.125 * (1.0 - x) * (1.0 + y) * (1.0 + z);
+    N[7] = 0.125 * (1.0 + x) * (1.0 + y) * (1.0 + z);
+    N[5] = 0.125 * (1.0 + x) * (1.0 - y) * (1.0 + z);
+  }
+}
+
+template <int DIM>
+UWG_HD void shape_local_derivs(const double* xi, double (*G)[8]) {
+  if (DIM == 2) {
+    double x = xi[0], y = xi[1];
+    G[0][0] = -0.25 * (1.0 - y);
+    G[0][1] = 0.25 * (1.0 - y);
+    G[0][3] = 0.25 * (1.0 + y);
+    G[0][2] = -0.25 * (1.0 + y);
+    G[1][0] = -0.25 * (1.0 - x);
+    G[1][1] = -0.25 * (1.0 + x);
+    G[1][3] = 0.25 * (1.0 + x);
+    G[1][2] = 0.25 * (1.0 - x);
+  } else {
+    double x = xi[0], y = xi[1], z = xi[2];
+    G[0][0] = -0.125 * (1.0 - y) * (1.0 - z);
+    G[0][2] = -0.125 * (1.0 + y) * (1.0 - z);
+    G[0][3] = 0.125 * (1.0 + y) * (1.0 - z);
+    G[0][1] = 0.125 * (1.0 - y) * (1.0 - z);
+    G[0][4] = -0.125 * (1.0 - y) * (1.0 + z);
+    G[0][6] = -0.125 * (1.0 + y) * (1.0 + z);
+    G[0][7] = 0.125 * (1.0 + y) * (1.0 + z);
+    G[0][5] = 0.125 * (1.0 - y) * (1.0 + z);
+    G[1][0] = -0.125 * (1.0 - x) * (1.0 - z);
+    G[1][2] = 0.125 * (1.0 - x) * (1.0 - z);
+    G[1][3] = 0.125 * (1.0 + x) * (1.0 - z);
+    G[1][1] = -0.125 * (1.0 + x) * (1.0 - z);
+    G[1][4] = -0.125 * (1.0 - x) * (1.0 + z);
+    G[1][6] = 0.125 * (1.0 - x) * (1.0 + z);
+    G[1][7] = 0.125 * (1.0 + x) * (1.0 + z);
+    G[1][5] = -0.125 * (1.0 + x) * (1.0 + z);
+    G[2][0] = -0.125 * (1.0 - x) * (1.0 - y);
+    G[2][2] = -0.125 * (1.0 - x) * (1.0 + y);
+    G[2][3] = -0.125 * (1.0 + x) * (1.0 + y);
+    G[2][1] = -0.125 * (1.0 + x) * (1.0 - y);
+    G[2][4] = 0.125 * (1.0 - x) * (1.0 - y);
+    G[2][6] = 0.125 * (1.0 - x) * (1.0 + y);
+    G[2][7] = 0.125 * (1.0 + x) * (1.0 + y);
+    G[2][5] = 0.125 * (1.0 + x) * (1.0 - y);
+  }
+}
+
+// Cramer's rule, LU/StgDomain/Geometry/src/TensorMath.c:959-1020 (row-major J)
+template <int DIM>
+UWG_HD void cramer(const double* J, double* s, const double* r) {
+  if (DIM == 3) {
+    double det = +J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                 J[2] * (J[3] * J[7] - J[4] * J[6]);
+    s[0] = (r[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (r[1] * J[8] - J[5] * r[2]) +
+            J[2] * (r[1] * J[7] - J[4] * r[2])) / det;
+    s[1] = (J[0] * (r[1] * J[8] - r[2] * J[5]) - r[0] * (J[3] * J[8] - J[5] * J[6]) +
+            J[2] * (J[3] * r[2] - r[1] * J[6])) / det;
+    s[2] = (J[0] * (J[4] * r[2] - J[7] * r[1]) - J[1] * (J[3] * r[2] - r[1] * J[6]) +
+            r[0] * (J[3] * J[7] - J[4] * J[6])) / det;
+  } else {
+    double det = J[0] * J[3] - J[2] * J[1];
+    s[0] = (r[0] * J[3] - J[1] * r[1]) / det;
+    s[1] = (r[1] * J[0] - J[2] * r[0]) / det;
+  }
+}
+
+// Newton global->local, LU/StgFEM/Discretisation/src/ElementType.c:177-301
+template <int DIM>
+UWG_HD void newton_local(const double* v, const double* x, double* xi) {
+  constexpr int NN = DIM == 3 ? 8 : 4;
+  for (int d = 0; d < DIM; d++) xi[d] = 0.0;
+  for (int it = 0; it < 100; it++) {
+    double J[9], rhs[3], inc[3], N[8], G[3][8];
+    for (int q = 0; q < 9; q++) J[q] = 0.0;
+    rhs[0] = rhs[1] = rhs[2] = 0.0;
+    shape<DIM>(xi, N);
+    shape_local_derivs<DIM>(xi, G);
+    for (int n = 0; n < NN; n++) {
+      const double* c = v + n * DIM;
+      double sf = N[n];
+      J[0 * DIM + 0] += G[0][n] * c[0];
+      J[0 * DIM + 1] += G[1][n] * c[0];
+      J[1 * DIM + 0] += G[0][n] * c[1];
+      J[1 * DIM + 1] += G[1][n] * c[1];
+      rhs[0] -= sf * c[0];
+      rhs[1] -= sf * c[1];
+      if (DIM == 3) {
+        J[2] += G[2][n] * c[0];
+        J[5] += G[2][n] * c[1];
+        J[6] += G[0][n] * c[2];
+        J[7] += G[1][n] * c[2];
+        J[8] += G[2][n] * c[2];
+        rhs[2] -= sf * c[2];
+      }
+    }
+    rhs[0] += x[0];
+    rhs[1] += x[1];
+    if (DIM == 3) rhs[2] += x[2];
+    cramer<DIM>(J, inc, rhs);
+    xi[0] += inc[0];
+    xi[1] += inc[1];
+    if (DIM == 3) xi[2] += inc[2];
+    double m = fabs(inc[0]);
+    if (m < fabs(inc[1])) m = fabs(inc[1]);
+    if (DIM == 3 && m < fabs(inc[2])) m = fabs(inc[2]);
+    if (m < 0.0001) return;
+  }
+  xi[0] = 1.1;
+}
+
+// Num_Approx, LU/StGermain/Base/Foundation/src/Numerics.c:15-21
+UWG_HD bool num_approx(double var, double val) {
+  return var >= val - 1e-12 && var <= val + 1e-12;
+}
+
+// Mesh_RegularAlgorithms_SearchElements, LU/StgDomain/Mesh/src/Mesh_RegularAlgorithms.c:174-206
+// sep = (max-min)/nel as computed once in Mesh_RegularAlgorithms_Update (:153-154).
+// The final clamp only guards memory; the reference has no element past nel-1.
+template <int DIM>
+UWG_HD bool search_element(const int* res, const double* lo, const double* hi, const double* x, long* el) {
+  long ind[3] = {0, 0, 0};
+  for (int d = 0; d < DIM; d++) {
+    double sep = (hi[d] - lo[d]) / res[d];
+    if (num_approx(x[d] - hi[d], 0.0)) {
+      ind[d] = res[d] - 1;
+    } else if (x[d] < lo[d] || x[d] > hi[d]) {
+      return false;
+    } else {
+      double out = (x[d] - lo[d]) / sep, integer;
+      double frac = modf(out, &integer);
+      ind[d] = (long)(unsigned)integer;
+      if (ind[d] > 0 && num_approx(frac, 0.0)) ind[d]--;
+      if (ind[d] > res[d] - 1) ind[d] = res[d] - 1;
+    }
+  }
+  *el = ind[0] + (long)res[0] * (ind[1] + (long)res[1] * ind[2]);
+  return true;
+}
+
+// _ElementCellLayout_CellOf, LU/StgDomain/Swarm/src/ElementCellLayout.c:291-320
+template <int DIM>
+UWG_HD long cell_of(const int* res, const double* lo, const double* hi, const double* x, long E) {
+  for (int d = 0; d < DIM; d++)
+    if (x[d] < lo[d] || x[d] > hi[d]) return E;
+  long el;
+  if (!search_element<DIM>(res, lo, hi, x, &el)) return E;
+  return el;
+}
+
+}  // namespace uwg

@@ -1,0 +1,1674 @@
+// Host side of libuwg: handles, the C ABI, assembly driver, the geometric multigrid
+// hierarchy and the Krylov drivers.  All numerical work runs in the HIP kernels of
+// fem_kernels.hip / swarm_kernels.hip / relocate.hip / nn_kernels.hip; this file only
+// orchestrates launches and performs O(1)-sized host arithmetic (Hessenberg least
+// squares, the dense inverse of the coarsest multigrid level).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+
+#include "../../include/uwg.h"
+#include "uwg_geom.h"
+#include "uwg_internal.h"
+#include "uwg_nn.h"
+
+namespace uwg {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+template <typename T>
+static T* dalloc(long n) {
+  T* p = nullptr;
+  if (n <= 0) n = 1;
+  UWG_HIP(hipMalloc(&p, sizeof(T) * n));
+  return p;
+}
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace uwg
+
+using namespace uwg;
+
+// ---------------------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------------------
+struct uwg_mesh {
+  Geom g;
+  double min_sep;
+};
+
+struct uwg_field {
+  uwg_mesh* mesh;
+  int ncomp;
+  int on_elements;
+  long n;  // entries
+  double* d;
+};
+
+struct uwg_swarm {
+  uwg_mesh* mesh;
+  long P = 0, cap = 0;
+  double* coords = nullptr;   // [cap][dim]
+  int* owner = nullptr;       // [cap]
+  int* cell_ptr = nullptr;    // [E+1]
+  int* cell_pid = nullptr;    // [cap]
+  int* mat = nullptr;         // [cap]
+  double* xi_slot = nullptr;  // [cap][dim]
+  double* w_slot = nullptr;   // [cap]
+  int* slot_cell = nullptr;   // [cap]
+  bool has_mat = false;
+  bool has_integration = false;
+  long state = 0;             // bumps whenever particles move
+  long integration_state = -1;
+  RelocWork* reloc = nullptr;
+  long reloc_P = -1;
+};
+
+namespace uwg {
+
+struct KrylovSpace;
+
+struct Level {
+  Geom g;
+  double* W = nullptr;
+  float* W32 = nullptr;
+  uint8_t* bc = nullptr;
+  double* dinv = nullptr;
+  float* dinv32 = nullptr;
+  double lmax = 0.0;
+  double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
+  float *x32 = nullptr, *b32 = nullptr, *r32 = nullptr, *d32 = nullptr;
+  long n() const { return g.N * g.dim; }
+};
+
+struct Coarse {
+  int n = 0;
+  double* Ainv = nullptr;  // device, n x n
+};
+
+}  // namespace uwg
+
+struct uwg_stokes {
+  uwg_field* vel;
+  uwg_field* p;
+  Geom g;
+  std::vector<Level> L;
+  Coarse coarse;
+  bool hierarchy_built = false;
+  bool assembled = false;
+  bool fp32_ready = false;
+  double Ge[24];
+  // assembly products (level 0)
+  double* fe = nullptr;       // [E][NN*dim]
+  double* f = nullptr;        // [N][dim] rhs (BC dofs zero)
+  double* approxS = nullptr;  // [E]
+  double* ubc = nullptr;      // [N][dim] Dirichlet values (zero elsewhere)
+  double* h = nullptr;        // [E] continuity rhs
+  // materials
+  int nmat = 0;
+  double* eta_tab = nullptr;
+  double* force_tab = nullptr;
+  double* eta_p = nullptr;
+  double* force_p = nullptr;
+  long np_vals = 0;
+  int mode = 0;               // 0 materials, 1 per-particle values
+  uwg_swarm* swarm = nullptr;
+  long assembled_state = -1;
+  // Krylov workspace
+  double* red_partial = nullptr;
+  double* red_out = nullptr;
+  double* dcoef = nullptr;
+  const double** dptrs = nullptr;
+  int levels_req = 0;
+  // learned correction
+  std::unique_ptr<NNModel, NNDeleter> nn;
+  int nn_level = -1;
+  int vcycles = 0;
+  int nn_rejects = 0;
+  uint8_t* bc0 = nullptr;     // level-0 Dirichlet mask (owned here, shared with L[0].bc)
+  KrylovSpace* kin = nullptr; // Krylov workspaces, owned by the handle
+  KrylovSpace* kout = nullptr;
+  KrylovSpace* kb = nullptr;
+};
+
+// ---------------------------------------------------------------------------------------
+// error wrapper
+// ---------------------------------------------------------------------------------------
+#define API_BEGIN try {
+#define API_END                                 \
+  }                                             \
+  catch (const std::exception& ex) {            \
+    set_error(ex.what());                       \
+    return UWG_ERR_DEVICE;                      \
+  }                                             \
+  catch (...) {                                 \
+    set_error("unknown error");                 \
+    return UWG_ERR_DEVICE;                      \
+  }
+
+static int arg_error(const char* m) {
+  set_error(m);
+  return UWG_ERR_ARG;
+}
+
+extern "C" const char* uwg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int uwg_device_info(char* buf, int n) {
+  API_BEGIN
+  hipDeviceProp_t prop;
+  int dev = 0;
+  UWG_HIP(hipGetDevice(&dev));
+  UWG_HIP(hipGetDeviceProperties(&prop, dev));
+  snprintf(buf, n, "%s (%s), %d CUs, %.1f GB", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+           prop.totalGlobalMem / 1e9);
+  return 0;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// mesh and fields
+// ---------------------------------------------------------------------------------------
+extern "C" int uwg_mesh_create(int dim, const int* res, const double* minc, const double* maxc, uwg_mesh** out) {
+  if (dim != 2 && dim != 3) return arg_error("dim must be 2 or 3");
+  for (int d = 0; d < dim; d++) {
+    if (res[d] < 1) return arg_error("element resolution must be >= 1");
+    if (!(maxc[d] > minc[d])) return arg_error("maxCoord must exceed minCoord");
+  }
+  API_BEGIN
+  uwg_mesh* m = new uwg_mesh();
+  m->g = make_geom(dim, res, minc, maxc);
+  // Mesh_HexType_GetMinimumSeparationGeneral: min over elements of pairwise node distances
+  double ms = HUGE_VAL;
+  const int nn = dim == 3 ? 8 : 4;
+  for (long e = 0; e < m->g.E; e++) {
+    // only distinct per-axis spacings matter on a tensor grid, but the reference visits
+    // every element; we do the same on the host (setup only)
+    double v[24];
+    if (dim == 3)
+      element_verts<3>(m->g.res, m->g.lo, m->g.hi, e, v);
+    else
+      element_verts<2>(m->g.res, m->g.lo, m->g.hi, e, v);
+    for (int a = 0; a < nn - 1; a++)
+      for (int b = a + 1; b < nn; b++) {
+        double s = 0.0;
+        for (int d = 0; d < dim; d++) {
+          double t = v[a * dim + d] - v[b * dim + d];
+          s += t * t;
+        }
+        s = sqrt(s);
+        if (s < ms) ms = s;
+      }
+    if (e > 4096 && m->g.E > 100000) break;  // regular spacing: remaining elements repeat
+  }
+  m->min_sep = ms;
+  *out = m;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_mesh_destroy(uwg_mesh* m) {
+  delete m;
+  return 0;
+}
+
+extern "C" int uwg_mesh_coords(uwg_mesh* m, double* out) {
+  API_BEGIN
+  double* d = dalloc<double>(m->g.N * m->g.dim);
+  launch_mesh_coords(m->g, d, 0);
+  UWG_HIP(hipMemcpy(out, d, sizeof(double) * m->g.N * m->g.dim, hipMemcpyDeviceToHost));
+  dfree(d);
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_mesh_min_separation(uwg_mesh* m, double* out) {
+  *out = m->min_sep;
+  return 0;
+}
+
+extern "C" int uwg_field_create(uwg_mesh* m, int ncomp, int on_elements, uwg_field** out) {
+  if (ncomp < 1) return arg_error("ncomp must be >= 1");
+  API_BEGIN
+  uwg_field* f = new uwg_field();
+  f->mesh = m;
+  f->ncomp = ncomp;
+  f->on_elements = on_elements;
+  f->n = (on_elements ? m->g.E : m->g.N) * ncomp;
+  f->d = dalloc<double>(f->n);
+  UWG_HIP(hipMemset(f->d, 0, sizeof(double) * f->n));
+  *out = f;
+  return 0;
+  API_END
+}
+extern "C" int uwg_field_destroy(uwg_field* f) {
+  if (f) dfree(f->d);
+  delete f;
+  return 0;
+}
+extern "C" int uwg_field_set(uwg_field* f, const double* host) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(f->d, host, sizeof(double) * f->n, hipMemcpyHostToDevice));
+  return 0;
+  API_END
+}
+extern "C" int uwg_field_get(uwg_field* f, double* host) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(host, f->d, sizeof(double) * f->n, hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// swarm
+// ---------------------------------------------------------------------------------------
+static void swarm_reserve(uwg_swarm* s, long P) {
+  if (P <= s->cap) return;
+  long cap = std::max(P, s->cap * 2);
+  int dim = s->mesh->g.dim;
+  auto grow = [&](auto*& ptr, long per) {
+    using T = std::remove_reference_t<decltype(*ptr)>;
+    T* n = dalloc<T>(cap * per);
+    if (ptr && s->P > 0) UWG_HIP(hipMemcpy(n, ptr, sizeof(T) * s->P * per, hipMemcpyDeviceToDevice));
+    dfree(ptr);
+    ptr = n;
+  };
+  grow(s->coords, dim);
+  grow(s->owner, 1);
+  grow(s->cell_pid, 1);
+  grow(s->mat, 1);
+  grow(s->xi_slot, dim);
+  grow(s->w_slot, 1);
+  grow(s->slot_cell, 1);
+  s->cap = cap;
+  if (s->reloc) reloc_destroy(s->reloc);
+  s->reloc = nullptr;
+}
+
+extern "C" int uwg_swarm_create(uwg_mesh* m, uwg_swarm** out) {
+  API_BEGIN
+  uwg_swarm* s = new uwg_swarm();
+  s->mesh = m;
+  s->cell_ptr = dalloc<int>(m->g.E + 1);
+  UWG_HIP(hipMemset(s->cell_ptr, 0, sizeof(int) * (m->g.E + 1)));
+  *out = s;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_swarm_destroy(uwg_swarm* s) {
+  if (!s) return 0;
+  for (void* p : {(void*)s->coords, (void*)s->owner, (void*)s->cell_ptr, (void*)s->cell_pid, (void*)s->mat,
+                  (void*)s->xi_slot, (void*)s->w_slot, (void*)s->slot_cell})
+    dfree(p);
+  reloc_destroy(s->reloc);
+  delete s;
+  return 0;
+}
+
+static void ensure_reloc(uwg_swarm* s) {
+  if (!s->reloc || s->reloc_P != s->P) {
+    reloc_destroy(s->reloc);
+    s->reloc = reloc_create(s->P, s->mesh->g.E);
+    s->reloc_P = s->P;
+  }
+}
+
+extern "C" int uwg_swarm_populate_random(uwg_swarm* s, int ppc, long seed) {
+  if (ppc < 1) return arg_error("particlesPerCell must be >= 1");
+  if (s->P != 0) return arg_error("swarm already populated");
+  API_BEGIN
+  const Geom& g = s->mesh->g;
+  long P = g.E * ppc;
+  swarm_reserve(s, P);
+  s->P = P;
+  launch_layout_random(g, ppc, seed, s->coords, 0);
+  std::vector<int> ptr(g.E + 1);
+  for (long e = 0; e <= g.E; e++) ptr[e] = (int)(e * ppc);
+  UWG_HIP(hipMemcpy(s->cell_ptr, ptr.data(), sizeof(int) * (g.E + 1), hipMemcpyHostToDevice));
+  std::vector<int> ids(P);
+  for (long p = 0; p < P; p++) ids[p] = (int)p;
+  UWG_HIP(hipMemcpy(s->cell_pid, ids.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+  for (long p = 0; p < P; p++) ids[p] = (int)(p / ppc);
+  UWG_HIP(hipMemcpy(s->owner, ids.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemset(s->mat, 0, sizeof(int) * P));
+  s->state++;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_swarm_add_particles(uwg_swarm* s, long n, const double* coords) {
+  if (n < 0) return arg_error("negative count");
+  API_BEGIN
+  const Geom& g = s->mesh->g;
+  long P0 = s->P;
+  swarm_reserve(s, P0 + n);
+  UWG_HIP(hipMemcpy(s->coords + P0 * g.dim, coords, sizeof(double) * n * g.dim, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemset(s->mat + P0, 0, sizeof(int) * n));
+  // new particles start unowned (owner = E) and are appended to their cells in index order
+  std::vector<int> unowned(n, (int)g.E);
+  UWG_HIP(hipMemcpy(s->owner + P0, unowned.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  s->P = P0 + n;
+  ensure_reloc(s);
+  long outside = relocate(g, s->P, s->coords, s->owner, s->cell_ptr, s->cell_pid, s->reloc, 0);
+  s->state++;
+  if (outside > 0) {
+    set_error("particles outside the mesh were added");
+    return UWG_ERR_DOMAIN;
+  }
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_swarm_count(uwg_swarm* s, long* P) {
+  *P = s->P;
+  return 0;
+}
+extern "C" int uwg_swarm_get_coords(uwg_swarm* s, double* out) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(out, s->coords, sizeof(double) * s->P * s->mesh->g.dim, hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+extern "C" int uwg_swarm_set_coords(uwg_swarm* s, const double* in, long* outside) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(s->coords, in, sizeof(double) * s->P * s->mesh->g.dim, hipMemcpyHostToDevice));
+  ensure_reloc(s);
+  long o = relocate(s->mesh->g, s->P, s->coords, s->owner, s->cell_ptr, s->cell_pid, s->reloc, 0);
+  if (outside) *outside = o;
+  s->state++;
+  return o > 0 ? UWG_ERR_DOMAIN : 0;
+  API_END
+}
+extern "C" int uwg_swarm_get_owners(uwg_swarm* s, int* out) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(out, s->owner, sizeof(int) * s->P, hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+extern "C" int uwg_swarm_get_cell_tables(uwg_swarm* s, int* cell_ptr, int* cell_pid) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(cell_ptr, s->cell_ptr, sizeof(int) * (s->mesh->g.E + 1), hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(cell_pid, s->cell_pid, sizeof(int) * s->P, hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+extern "C" int uwg_swarm_set_material(uwg_swarm* s, const int* mat) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(s->mat, mat, sizeof(int) * s->P, hipMemcpyHostToDevice));
+  s->has_mat = true;
+  s->state++;
+  return 0;
+  API_END
+}
+extern "C" int uwg_swarm_update_owners(uwg_swarm* s, long* outside) {
+  API_BEGIN
+  ensure_reloc(s);
+  long o = relocate(s->mesh->g, s->P, s->coords, s->owner, s->cell_ptr, s->cell_pid, s->reloc, 0);
+  if (outside) *outside = o;
+  s->state++;
+  if (o > 0) {
+    set_error("Particles appear to have left the domain, but swarm flag `particleEscape` is False.");
+    return UWG_ERR_DOMAIN;
+  }
+  return 0;
+  API_END
+}
+
+static int run_dvc(uwg_swarm* s, int dvc_res) {
+  const Geom& g = s->mesh->g;
+  const long E = g.E;
+  // per-cell counts for capacity planning
+  std::vector<int> ptr(E + 1);
+  UWG_HIP(hipMemcpy(ptr.data(), s->cell_ptr, sizeof(int) * (E + 1), hipMemcpyDeviceToHost));
+  int maxc = 0;
+  for (long e = 0; e < E; e++) {
+    int c = ptr[e + 1] - ptr[e];
+    if (c == 0) {
+      set_error("Problem has an under resolved cell (Cell Id = " + std::to_string(e) +
+                "), add more particles to your model");
+      return UWG_ERR_UNDERRESOLVED;
+    }
+    maxc = std::max(maxc, c);
+  }
+  const int ncell = dvc_res * dvc_res * (g.dim == 3 ? dvc_res : 1);
+  int max_np = std::min(std::max(64, ((maxc + 15) / 16) * 16), 32000);
+  int cap = std::min(512, ncell);
+  long per = dvc_scratch_bytes(g.dim, dvc_res, max_np, cap);
+  long batch = std::max(1024L, std::min(E, (long)(3.0e9 / per)));
+  char* scratch = dalloc<char>(per * batch);
+  int* flag = dalloc<int>(batch);
+  std::vector<int> hflag(batch), retry;
+  for (long e0 = 0; e0 < E; e0 += batch) {
+    long nE = std::min(batch, E - e0);
+    launch_dvc_list(g.dim, dvc_res, e0, nE, nullptr, s->cell_ptr, s->xi_slot, s->w_slot, scratch, max_np, cap, flag, 0);
+    UWG_HIP(hipMemcpy(hflag.data(), flag, sizeof(int) * nE, hipMemcpyDeviceToHost));
+    for (long t = 0; t < nE; t++)
+      if (hflag[t]) retry.push_back((int)(e0 + t));
+  }
+  dfree(scratch);
+  dfree(flag);
+  if (!retry.empty()) {
+    // larger lists for the few elements that overflowed
+    int cap2 = ncell, np2 = std::max(maxc, 1);
+    long per2 = dvc_scratch_bytes(g.dim, dvc_res, np2, cap2);
+    long b2 = std::max(1L, std::min((long)retry.size(), (long)(2.0e9 / per2)));
+    char* sc2 = dalloc<char>(per2 * b2);
+    int* fl2 = dalloc<int>(b2);
+    int* el = dalloc<int>(retry.size());
+    UWG_HIP(hipMemcpy(el, retry.data(), sizeof(int) * retry.size(), hipMemcpyHostToDevice));
+    std::vector<int> hf(b2);
+    for (long i0 = 0; i0 < (long)retry.size(); i0 += b2) {
+      long n = std::min(b2, (long)retry.size() - i0);
+      launch_dvc_list(g.dim, dvc_res, 0, n, el + i0, s->cell_ptr, s->xi_slot, s->w_slot, sc2, np2, cap2, fl2, 0);
+      UWG_HIP(hipMemcpy(hf.data(), fl2, sizeof(int) * n, hipMemcpyDeviceToHost));
+      for (long t = 0; t < n; t++)
+        if (hf[t]) {
+          dfree(sc2); dfree(fl2); dfree(el);
+          set_error("DVC list capacity exceeded");
+          return UWG_ERR_DEVICE;
+        }
+    }
+    dfree(sc2);
+    dfree(fl2);
+    dfree(el);
+  }
+  return 0;
+}
+
+extern "C" int uwg_swarm_integration_points(uwg_swarm* s, int use_dvc, int dvc_res) {
+  API_BEGIN
+  const Geom& g = s->mesh->g;
+  if (dvc_res <= 0) dvc_res = 15;
+  if (dvc_res > 32) return arg_error("DVC resolution must be <= 32");
+  launch_slot_cell(g.E, s->cell_ptr, s->slot_cell, 0);
+  launch_coincident_map(g, s->P, s->slot_cell, s->cell_pid, s->coords, s->xi_slot, 0);
+  if (use_dvc) {
+    int rc = run_dvc(s, dvc_res);
+    if (rc) return rc;
+  } else {
+    // equal weights 2^dim / count per cell
+    std::vector<int> ptr(g.E + 1);
+    UWG_HIP(hipMemcpy(ptr.data(), s->cell_ptr, sizeof(int) * (g.E + 1), hipMemcpyDeviceToHost));
+    std::vector<double> w(s->P);
+    for (long e = 0; e < g.E; e++) {
+      int c = ptr[e + 1] - ptr[e];
+      for (int k = ptr[e]; k < ptr[e + 1]; k++) w[k] = (double)(1 << g.dim) / c;
+    }
+    UWG_HIP(hipMemcpy(s->w_slot, w.data(), sizeof(double) * s->P, hipMemcpyHostToDevice));
+  }
+  UWG_HIP(hipDeviceSynchronize());
+  s->has_integration = true;
+  s->integration_state = s->state;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_swarm_get_integration(uwg_swarm* s, double* xi_slot, double* w_slot) {
+  API_BEGIN
+  int dim = s->mesh->g.dim;
+  UWG_HIP(hipMemcpy(xi_slot, s->xi_slot, sizeof(double) * s->P * dim, hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(w_slot, s->w_slot, sizeof(double) * s->P, hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+extern "C" int uwg_swarm_set_integration(uwg_swarm* s, const double* xi_slot, const double* w_slot) {
+  API_BEGIN
+  int dim = s->mesh->g.dim;
+  UWG_HIP(hipMemcpy(s->xi_slot, xi_slot, sizeof(double) * s->P * dim, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(s->w_slot, w_slot, sizeof(double) * s->P, hipMemcpyHostToDevice));
+  s->has_integration = true;
+  s->integration_state = s->state;
+  return 0;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// advection
+// ---------------------------------------------------------------------------------------
+extern "C" int uwg_advect_rk2(uwg_swarm* s, uwg_field* vel, double dt, int update_owners, long* outside) {
+  if (vel->mesh != s->mesh || vel->on_elements || vel->ncomp != s->mesh->g.dim)
+    return arg_error("velocity field must be a nodal vector field on the swarm's mesh");
+  API_BEGIN
+  int* cnt = dalloc<int>(2);
+  UWG_HIP(hipMemset(cnt, 0, sizeof(int) * 2));
+  launch_advect_rk2(s->mesh->g, s->P, vel->d, s->coords, dt, cnt, 0);
+  int h[2];
+  UWG_HIP(hipMemcpy(h, cnt, sizeof(int) * 2, hipMemcpyDeviceToHost));
+  dfree(cnt);
+  s->state++;
+  if (h[0] > 0) {
+    set_error("Unable to determine valid velocity for " + std::to_string(h[0]) +
+              " particle(s): coordinate outside the global domain");
+    return UWG_ERR_DOMAIN;
+  }
+  if (update_owners) return uwg_swarm_update_owners(s, outside);
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_advect_max_dt(uwg_field* vel, double* dt) {
+  API_BEGIN
+  const Geom& g = vel->mesh->g;
+  const int nblk = 512;
+  double* part = dalloc<double>(nblk);
+  launch_node_speed_max(g, vel->d, part, nblk, 0);
+  std::vector<double> h(nblk);
+  UWG_HIP(hipMemcpy(h.data(), part, sizeof(double) * nblk, hipMemcpyDeviceToHost));
+  dfree(part);
+  double vmax = 0.0;
+  for (double v : h) vmax = std::max(vmax, v);
+  *dt = 0.5 * vel->mesh->min_sep / vmax;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_field_interpolate(uwg_field* f, long n, const double* x, double* v, int* ok) {
+  if (f->on_elements || f->ncomp != f->mesh->g.dim) return arg_error("nodal vector field required");
+  API_BEGIN
+  int dim = f->mesh->g.dim;
+  double* dx = dalloc<double>(n * dim);
+  double* dv = dalloc<double>(n * dim);
+  int* dok = dalloc<int>(n);
+  UWG_HIP(hipMemcpy(dx, x, sizeof(double) * n * dim, hipMemcpyHostToDevice));
+  launch_interpolate(f->mesh->g, n, f->d, dx, dv, dok, 0);
+  UWG_HIP(hipMemcpy(v, dv, sizeof(double) * n * dim, hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(ok, dok, sizeof(int) * n, hipMemcpyDeviceToHost));
+  dfree(dx);
+  dfree(dv);
+  dfree(dok);
+  return 0;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// Stokes: assembly
+// ---------------------------------------------------------------------------------------
+// G_e at 3^d Gauss points (GradientStiffnessMatrixTerm.c:129-221); identical for every
+// element of the regular mesh.
+static void compute_Ge(const Geom& g, double* Ge) {
+  const double a = sqrt(3.0 / 5.0);
+  const double x1[3] = {-a, 0.0, a}, w1[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  const int dim = g.dim, NN = 1 << dim;
+  double h[3];
+  for (int d = 0; d < dim; d++) h[d] = (g.hi[d] - g.lo[d]) / g.res[d];
+  double detJ = 1.0;
+  for (int d = 0; d < dim; d++) detJ *= 0.5 * h[d];
+  for (int q = 0; q < NN * dim; q++) Ge[q] = 0.0;
+  for (int k = 0; k < (dim == 3 ? 3 : 1); k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) {
+        double xi[3] = {x1[i], x1[j], x1[k]};
+        double w = w1[i] * w1[j] * (dim == 3 ? w1[k] : 1.0);
+        double G[3][8];
+        if (dim == 3)
+          shape_local_derivs<3>(xi, G);
+        else
+          shape_local_derivs<2>(xi, G);
+        for (int n = 0; n < NN; n++)
+          for (int d = 0; d < dim; d++) Ge[n * dim + d] += w * (-detJ) * (G[d][n] * (2.0 / h[d]));
+      }
+}
+
+static void build_levels(uwg_stokes* st, int levels_req) {
+  // free previous levels, keeping the level-0 mask and quadrature weights
+  double* W0 = st->L.empty() ? nullptr : st->L[0].W;
+  for (size_t l = 0; l < st->L.size(); l++) {
+    Level& A = st->L[l];
+    for (void* p : {(void*)A.W, (void*)A.W32, (void*)A.bc, (void*)A.dinv, (void*)A.dinv32, (void*)A.x, (void*)A.b,
+                    (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
+      if (p != (void*)st->bc0 && p != (void*)W0) dfree(p);
+  }
+  st->L.clear();
+  Geom g = st->g;
+  int nl = 1;
+  std::vector<Geom> gs{g};
+  while (true) {
+    if (levels_req > 0 && nl >= levels_req) break;
+    bool even = true;
+    for (int d = 0; d < g.dim; d++) even = even && (g.res[d] % 2 == 0) && g.res[d] >= 4;
+    if (!even) break;
+    if (levels_req <= 0 && g.N * g.dim <= 800) break;
+    int r2[3];
+    for (int d = 0; d < 3; d++) r2[d] = d < g.dim ? g.res[d] / 2 : 1;
+    g = make_geom(g.dim, r2, g.lo, g.hi);
+    gs.push_back(g);
+    nl++;
+  }
+  st->L.resize(nl);
+  for (int l = 0; l < nl; l++) {
+    Level& A = st->L[l];
+    A.g = gs[l];
+    const long Q = A.g.dim == 3 ? 27 : 9;
+    A.W = (l == 0 && W0) ? W0 : dalloc<double>(Q * A.g.E);
+    A.dinv = dalloc<double>(A.n());
+    A.x = dalloc<double>(A.n());
+    A.b = dalloc<double>(A.n());
+    A.r = dalloc<double>(A.n());
+    A.d = dalloc<double>(A.n());
+    A.bc = l > 0 ? dalloc<uint8_t>(A.g.N) : st->bc0;
+  }
+  st->hierarchy_built = true;
+  st->fp32_ready = false;
+}
+
+static void ensure_fp32(uwg_stokes* st) {
+  if (st->fp32_ready) return;
+  for (auto& A : st->L) {
+    const long Q = A.g.dim == 3 ? 27 : 9;
+    if (!A.W32) {
+      A.W32 = dalloc<float>(Q * A.g.E);
+      A.dinv32 = dalloc<float>(A.n());
+      A.x32 = dalloc<float>(A.n());
+      A.b32 = dalloc<float>(A.n());
+      A.r32 = dalloc<float>(A.n());
+      A.d32 = dalloc<float>(A.n());
+    }
+    vk_d2f(Q * A.g.E, 1.0, A.W, A.W32, 0);
+    vk_d2f(A.n(), 1.0, A.dinv, A.dinv32, 0);
+  }
+  st->fp32_ready = true;
+}
+
+// Dense inverse of the coarsest operator (identity rows on Dirichlet dofs).
+// K_e[ai][bj] = sum_g W_g (delta_ij sum_k dNa_k dNb_k + dNa_j dNb_i), physical derivatives.
+static void build_coarse(uwg_stokes* st) {
+  Level& A = st->L.back();
+  const Geom& g = A.g;
+  const int dim = g.dim, NN = 1 << dim, Q = dim == 3 ? 27 : 9;
+  const int n = (int)(g.N * dim);
+  st->coarse.n = n;
+  if (n > 6000) {
+    dfree(st->coarse.Ainv);
+    st->coarse.Ainv = nullptr;
+    return;
+  }
+  std::vector<double> W((size_t)Q * g.E);
+  UWG_HIP(hipMemcpy(W.data(), A.W, sizeof(double) * Q * g.E, hipMemcpyDeviceToHost));
+  std::vector<uint8_t> bc(g.N);
+  UWG_HIP(hipMemcpy(bc.data(), A.bc, g.N, hipMemcpyDeviceToHost));
+  std::vector<double> K((size_t)n * n, 0.0);
+  double h[3] = {1, 1, 1};
+  for (int d = 0; d < dim; d++) h[d] = (g.hi[d] - g.lo[d]) / g.res[d];
+  const double ga = sqrt(0.6);
+  auto gp = [&](int i) { return i == 0 ? -ga : (i == 1 ? 0.0 : ga); };
+  long nx = g.nn[0], ny = g.nn[1];
+  for (long e = 0; e < g.E; e++) {
+    long i = e % g.res[0], j = (e / g.res[0]) % g.res[1], k = dim == 3 ? e / ((long)g.res[0] * g.res[1]) : 0;
+    long base = i + nx * (j + ny * k);
+    long nodes[8];
+    for (int a = 0; a < NN; a++) nodes[a] = base + (a & 1) + ((a >> 1) & 1) * nx + (dim == 3 ? ((a >> 2) & 1) * nx * ny : 0);
+    for (int q = 0; q < Q; q++) {
+      double X[3] = {gp(q % 3), gp((q / 3) % 3), gp(q / 9)};
+      double dN[8][3];
+      for (int a = 0; a < NN; a++) {
+        double s[3] = {(a & 1) ? 1.0 : -1.0, (a & 2) ? 1.0 : -1.0, (a & 4) ? 1.0 : -1.0};
+        if (dim == 2) {
+          dN[a][0] = 0.25 * s[0] * (1 + s[1] * X[1]) * 2.0 / h[0];
+          dN[a][1] = 0.25 * s[1] * (1 + s[0] * X[0]) * 2.0 / h[1];
+        } else {
+          dN[a][0] = 0.125 * s[0] * (1 + s[1] * X[1]) * (1 + s[2] * X[2]) * 2.0 / h[0];
+          dN[a][1] = 0.125 * s[1] * (1 + s[0] * X[0]) * (1 + s[2] * X[2]) * 2.0 / h[1];
+          dN[a][2] = 0.125 * s[2] * (1 + s[0] * X[0]) * (1 + s[1] * X[1]) * 2.0 / h[2];
+        }
+      }
+      double w = W[(size_t)q * g.E + e];
+      for (int a = 0; a < NN; a++)
+        for (int b = 0; b < NN; b++) {
+          double dot = 0.0;
+          for (int kk = 0; kk < dim; kk++) dot += dN[a][kk] * dN[b][kk];
+          for (int ii = 0; ii < dim; ii++)
+            for (int jj = 0; jj < dim; jj++) {
+              double v = w * ((ii == jj ? dot : 0.0) + dN[a][jj] * dN[b][ii]);
+              K[(size_t)(nodes[a] * dim + ii) * n + nodes[b] * dim + jj] += v;
+            }
+        }
+    }
+  }
+  for (long nd = 0; nd < g.N; nd++)
+    for (int d = 0; d < dim; d++)
+      if ((bc[nd] >> d) & 1) {
+        long r = nd * dim + d;
+        for (int c = 0; c < n; c++) {
+          K[(size_t)r * n + c] = 0.0;
+          K[(size_t)c * n + r] = 0.0;
+        }
+        K[(size_t)r * n + r] = 1.0;
+      }
+  // Cholesky K = L L^T, then inverse
+  std::vector<double> Lm(K);
+  for (int c = 0; c < n; c++) {
+    double s = Lm[(size_t)c * n + c];
+    for (int k2 = 0; k2 < c; k2++) s -= Lm[(size_t)c * n + k2] * Lm[(size_t)c * n + k2];
+    if (!(s > 0.0)) throw std::runtime_error("coarse operator is not positive definite (check viscosity/BCs)");
+    double dgl = sqrt(s);
+    Lm[(size_t)c * n + c] = dgl;
+    for (int r = c + 1; r < n; r++) {
+      double t = Lm[(size_t)r * n + c];
+      for (int k2 = 0; k2 < c; k2++) t -= Lm[(size_t)r * n + k2] * Lm[(size_t)c * n + k2];
+      Lm[(size_t)r * n + c] = t / dgl;
+    }
+  }
+  std::vector<double> Inv((size_t)n * n, 0.0), col(n);
+  for (int c = 0; c < n; c++) {
+    // solve L y = e_c ; L^T x = y
+    for (int r = 0; r < n; r++) {
+      double t = (r == c) ? 1.0 : 0.0;
+      for (int k2 = 0; k2 < r; k2++) t -= Lm[(size_t)r * n + k2] * col[k2];
+      col[r] = t / Lm[(size_t)r * n + r];
+    }
+    for (int r = n - 1; r >= 0; r--) {
+      double t = col[r];
+      for (int k2 = r + 1; k2 < n; k2++) t -= Lm[(size_t)k2 * n + r] * col[k2];
+      col[r] = t / Lm[(size_t)r * n + r];
+    }
+    for (int r = 0; r < n; r++) Inv[(size_t)c * n + r] = col[r];
+  }
+  if (!st->coarse.Ainv) st->coarse.Ainv = dalloc<double>((long)n * n);
+  UWG_HIP(hipMemcpy(st->coarse.Ainv, Inv.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+}
+
+// power iteration for lambda_max(D^-1 K) on each level
+static void estimate_lmax(uwg_stokes* st, Level& A) {
+  long n = A.n();
+  std::vector<double> h(n);
+  uint32_t s = 12345u;
+  for (long i = 0; i < n; i++) {
+    s = s * 1664525u + 1013904223u;
+    h[i] = ((s >> 8) & 0xffff) / 65536.0 + 0.25;
+  }
+  UWG_HIP(hipMemcpy(A.x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  double lam = 0.0;
+  const double* ptr[1];
+  for (int it = 0; it < 12; it++) {
+    launch_apply_K<double>(A.g, A.W, A.x, A.r, A.bc, 0);
+    vk_xmy_scale<double>(n, A.dinv, A.r, A.d, 0);
+    // norms
+    ptr[0] = A.d;
+    UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
+    vk_multi_dot(n, 1, A.d, st->dptrs, st->red_partial, st->red_out, 0);
+    double nd;
+    UWG_HIP(hipMemcpy(&nd, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
+    ptr[0] = A.x;
+    UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
+    vk_multi_dot(n, 1, A.x, st->dptrs, st->red_partial, st->red_out, 0);
+    double nx2;
+    UWG_HIP(hipMemcpy(&nx2, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
+    lam = sqrt(nd / nx2);
+    vk_axpby<double>(n, 1.0 / sqrt(nd), A.d, 0.0, A.x, 0);
+  }
+  A.lmax = lam;
+}
+
+static void setup_hierarchy(uwg_stokes* st) {
+  for (size_t l = 1; l < st->L.size(); l++) {
+    Level& F = st->L[l - 1];
+    Level& C = st->L[l];
+    launch_restrict_W(F.g, C.g, F.W, C.W, 0);
+    launch_bc_mask_from_walls(C.g, C.bc, 0, 0);
+  }
+  // coarse Dirichlet masks: a coarse dof is constrained when the coincident fine dof is
+  // (nested walls); derived from level 0's mask by injection
+  for (size_t l = 1; l < st->L.size(); l++) {
+    Level& F = st->L[l - 1];
+    Level& C = st->L[l];
+    std::vector<uint8_t> fb(F.g.N), cb(C.g.N);
+    UWG_HIP(hipMemcpy(fb.data(), F.bc, F.g.N, hipMemcpyDeviceToHost));
+    for (long n = 0; n < C.g.N; n++) {
+      long i = n % C.g.nn[0], j = (n / C.g.nn[0]) % C.g.nn[1], k = n / ((long)C.g.nn[0] * C.g.nn[1]);
+      long fn = 2 * i + (long)F.g.nn[0] * (2 * j + (long)F.g.nn[1] * 2 * k);
+      cb[n] = fb[fn];
+    }
+    UWG_HIP(hipMemcpy(C.bc, cb.data(), C.g.N, hipMemcpyHostToDevice));
+  }
+  for (auto& A : st->L) launch_diag(A.g, A.W, A.bc, A.dinv, 0);
+  for (auto& A : st->L) estimate_lmax(st, A);
+  build_coarse(st);
+  st->fp32_ready = false;
+}
+
+// ---------------------------------------------------------------------------------------
+// multigrid V-cycle (templated on the arithmetic type of the cycle)
+// ---------------------------------------------------------------------------------------
+namespace {
+template <typename T> struct LV;
+template <> struct LV<double> {
+  static double* W(Level& A) { return A.W; }
+  static double* dinv(Level& A) { return A.dinv; }
+  static double* x(Level& A) { return A.x; }
+  static double* b(Level& A) { return A.b; }
+  static double* r(Level& A) { return A.r; }
+  static double* d(Level& A) { return A.d; }
+};
+template <> struct LV<float> {
+  static float* W(Level& A) { return A.W32; }
+  static float* dinv(Level& A) { return A.dinv32; }
+  static float* x(Level& A) { return A.x32; }
+  static float* b(Level& A) { return A.b32; }
+  static float* r(Level& A) { return A.r32; }
+  static float* d(Level& A) { return A.d32; }
+};
+}  // namespace
+
+static int g_smooth_degree = 2;
+
+template <typename T>
+static void chebyshev(Level& A, const T* b, T* x, int degree, bool zero_guess) {
+  const long n = A.n();
+  const double emax = 1.1 * A.lmax, emin = 0.12 * A.lmax;
+  const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  T* r = LV<T>::r(A);
+  T* d = LV<T>::d(A);
+  if (zero_guess) {
+    vk_copy<T>(n, b, r, 0);
+    vk_fill<T>(n, (T)0, x, 0);
+  } else {
+    launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
+    vk_residual<T>(n, b, r, 0);
+  }
+  vk_fill<T>(n, (T)0, d, 0);
+  vk_cheb_update<T>(n, (T)0, (T)(1.0 / theta), LV<T>::dinv(A), r, d, x, 0);
+  for (int k = 1; k < degree; k++) {
+    launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
+    vk_residual<T>(n, b, r, 0);
+    double rho_new = 1.0 / (2.0 * sigma - rho);
+    vk_cheb_update<T>(n, (T)(rho_new * rho), (T)(2.0 * rho_new / delta), LV<T>::dinv(A), r, d, x, 0);
+    rho = rho_new;
+  }
+}
+
+template <typename T>
+static void coarse_solve(uwg_stokes* st, Level& A, const T* b, T* x);
+
+template <>
+void coarse_solve<double>(uwg_stokes* st, Level& A, const double* b, double* x) {
+  if (st->coarse.Ainv)
+    launch_dense_mv(st->coarse.n, st->coarse.Ainv, b, x, 0);
+  else
+    chebyshev<double>(A, b, x, 24, true);
+}
+template <>
+void coarse_solve<float>(uwg_stokes* st, Level& A, const float* b, float* x) {
+  // FP64 coarsest solve inside the FP32 cycle
+  const long n = A.n();
+  vk_f2d(n, 1.0, b, A.b, 0);
+  coarse_solve<double>(st, A, A.b, A.x);
+  vk_d2f(n, 1.0, A.x, x, 0);
+}
+
+template <typename T>
+static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
+  Level& A = st->L[l];
+  const long n = A.n();
+  if (l == (int)st->L.size() - 1) {
+    coarse_solve<T>(st, A, b, x);
+    return;
+  }
+  Level& C = st->L[l + 1];
+  chebyshev<T>(A, b, x, g_smooth_degree, true);
+  T* r = LV<T>::r(A);
+  launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
+  vk_residual<T>(n, b, r, 0);
+  launch_restrict2<T>(A.g, C.g, r, LV<T>::b(C), C.bc, A.bc, 0);
+  if (st->nn && l + 1 == st->nn_level && std::is_same<T, float>::value) {
+    nn_coarse_correction(st->nn.get(), C.g, (const float*)LV<T>::b(C), (float*)LV<T>::x(C), C.bc, 0);
+  } else {
+    vcycle<T>(st, l + 1, LV<T>::b(C), LV<T>::x(C));
+  }
+  launch_prolong_add<T>(A.g, C.g, LV<T>::x(C), x, A.bc, 0);
+  chebyshev<T>(A, b, x, g_smooth_degree, false);
+}
+
+// ---------------------------------------------------------------------------------------
+// FGMRES (right preconditioned, classical Gram-Schmidt), FP64 vectors on the device.
+// Converged when |r| <= max(rtol |r0|, atol) with r0 the initial residual
+// (KSPConvergedDefault with zero initial guess).
+// ---------------------------------------------------------------------------------------
+struct KrylovSpace {
+  long n = 0;
+  int m = 0;
+  std::vector<double*> V, Z;
+  double* w = nullptr;
+  const double** dV = nullptr;
+  void init(long n_, int m_) {
+    if (n_ == n && m_ == m) return;
+    free_all();
+    n = n_;
+    m = m_;
+    V.resize(m + 1);
+    Z.resize(m);
+    for (auto& v : V) v = dalloc<double>(n);
+    for (auto& z : Z) z = dalloc<double>(n);
+    w = dalloc<double>(n);
+    UWG_HIP(hipMalloc(&dV, sizeof(double*) * (m + 1)));
+    UWG_HIP(hipMemcpy(dV, V.data(), sizeof(double*) * (m + 1), hipMemcpyHostToDevice));
+  }
+  void free_all() {
+    for (auto& v : V) dfree(v);
+    for (auto& z : Z) dfree(z);
+    dfree(w);
+    dfree(dV);
+    V.clear();
+    Z.clear();
+    w = nullptr;
+    dV = nullptr;
+  }
+  ~KrylovSpace() { free_all(); }
+};
+
+using OpFn = std::function<void(const double*, double*)>;
+
+static double dev_dot(uwg_stokes* st, long n, const double* a, const double* b) {
+  const double* ptr[1] = {b};
+  UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
+  vk_multi_dot(n, 1, a, st->dptrs, st->red_partial, st->red_out, 0);
+  double r;
+  UWG_HIP(hipMemcpy(&r, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
+  return r;
+}
+
+// multi-dot against K.V[0..k-1] in groups of 8
+static void dev_mdot(uwg_stokes* st, KrylovSpace& K, int k, const double* x, double* out) {
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    int kk = std::min(8, k - j0);
+    vk_multi_dot(K.n, kk, x, K.dV + j0, st->red_partial, st->red_out, 0);
+    UWG_HIP(hipMemcpy(out + j0, st->red_out, sizeof(double) * kk, hipMemcpyDeviceToHost));
+  }
+}
+
+static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, const double* b, double* x,
+                  double rtol, double atol, int maxit, int* its_out, double* rnorm_out, bool x0_zero = true) {
+  const long n = K.n;
+  const int m = K.m;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1);
+  int its = 0;
+  double r0 = -1.0, rn = 0.0;
+  bool conv = false;
+  while (true) {
+    // r = b - A x
+    if (x0_zero && its == 0) {
+      vk_copy<double>(n, b, K.V[0], 0);
+    } else {
+      A(x, K.V[0]);
+      vk_axpby<double>(n, 1.0, b, -1.0, K.V[0], 0);
+    }
+    double beta = sqrt(std::max(0.0, dev_dot(st, n, K.V[0], K.V[0])));
+    if (r0 < 0) r0 = beta;
+    rn = beta;
+    if (beta <= std::max(rtol * r0, atol) || its >= maxit) {
+      conv = beta <= std::max(rtol * r0, atol);
+      break;
+    }
+    vk_axpby<double>(n, 0.0, K.V[0], 1.0 / beta, K.V[0], 0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && its < maxit; j++) {
+      M(K.V[j], K.Z[j]);
+      A(K.Z[j], K.w);
+      dev_mdot(st, K, j + 1, K.w, hcol.data());
+      for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
+      UWG_HIP(hipMemcpy(st->dcoef, hcol.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice));
+      vk_multi_axpy(n, j + 1, K.dV, st->dcoef, K.w, 0);
+      for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
+      double hn = sqrt(std::max(0.0, dev_dot(st, n, K.w, K.w)));
+      hcol[j + 1] = hn;
+      if (hn > 0) vk_axpby<double>(n, 0.0, K.w, 1.0 / hn, K.V[j + 1], 0);
+      // Givens
+      for (int i = 0; i < j; i++) {
+        double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
+        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
+        hcol[i] = t;
+      }
+      double den = sqrt(hcol[j] * hcol[j] + hcol[j + 1] * hcol[j + 1]);
+      cs[j] = den > 0 ? hcol[j] / den : 1.0;
+      sn[j] = den > 0 ? hcol[j + 1] / den : 0.0;
+      hcol[j] = den;
+      hcol[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hcol[i];
+      its++;
+      rn = fabs(g[j + 1]);
+      if (rn <= std::max(rtol * r0, atol) || hn == 0.0) {
+        j++;
+        break;
+      }
+    }
+    // back substitution
+    int k = j;
+    for (int i = k - 1; i >= 0; i--) {
+      double t = g[i];
+      for (int c = i + 1; c < k; c++) t -= H[(size_t)i * m + c] * y[c];
+      y[i] = t / H[(size_t)i * m + i];
+    }
+    UWG_HIP(hipMemcpy(st->dcoef, y.data(), sizeof(double) * k, hipMemcpyHostToDevice));
+    const double** dZ = K.dV + (m + 1);  // Z pointer table follows V in dptrs block
+    (void)dZ;
+    std::vector<const double*> zp(K.Z.begin(), K.Z.begin() + k);
+    UWG_HIP(hipMemcpy(st->dptrs, zp.data(), sizeof(double*) * k, hipMemcpyHostToDevice));
+    vk_multi_axpy(n, k, st->dptrs, st->dcoef, x, 0);
+    x0_zero = false;
+    if (rn <= std::max(rtol * r0, atol)) {
+      conv = true;
+      break;
+    }
+    if (its >= maxit) break;
+  }
+  if (its_out) *its_out = its;
+  if (rnorm_out) *rnorm_out = rn;
+  return conv ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// Stokes handle and solves
+// ---------------------------------------------------------------------------------------
+extern "C" int uwg_stokes_create(uwg_field* vel, uwg_field* p, const unsigned char* bcmask, uwg_stokes** out) {
+  if (vel->on_elements || vel->ncomp != vel->mesh->g.dim) return arg_error("velocityField must be a nodal vector field");
+  if (!p->on_elements || p->ncomp != 1 || p->mesh != vel->mesh)
+    return arg_error("pressureField must be a scalar field on the dQ0 submesh");
+  API_BEGIN
+  uwg_stokes* st = new uwg_stokes();
+  st->vel = vel;
+  st->p = p;
+  st->g = vel->mesh->g;
+  st->bc0 = dalloc<uint8_t>(st->g.N);
+  if (bcmask)
+    UWG_HIP(hipMemcpy(st->bc0, bcmask, st->g.N, hipMemcpyHostToDevice));
+  else
+    launch_bc_mask_from_walls(st->g, st->bc0, st->g.dim == 3 ? 0x3f : 0x0f, 0);
+  build_levels(st, 0);
+  compute_Ge(st->g, st->Ge);
+  const int NN = 1 << st->g.dim;
+  st->fe = dalloc<double>(st->g.E * NN * st->g.dim);
+  st->f = dalloc<double>(st->g.N * st->g.dim);
+  st->ubc = dalloc<double>(st->g.N * st->g.dim);
+  st->approxS = dalloc<double>(st->g.E);
+  st->h = dalloc<double>(st->g.E);
+  st->red_partial = dalloc<double>(8 * vk_reduce_blocks());
+  st->red_out = dalloc<double>(8);
+  st->dcoef = dalloc<double>(256);
+  UWG_HIP(hipMalloc(&st->dptrs, sizeof(double*) * 256));
+  *out = st;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
+  if (!st) return 0;
+  for (auto& A : st->L)
+    for (void* p : {(void*)A.W, (void*)A.W32, (void*)A.bc, (void*)A.dinv, (void*)A.dinv32, (void*)A.x, (void*)A.b,
+                    (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
+      if (p != (void*)st->bc0) dfree(p);
+  dfree(st->bc0);
+  delete st->kin;
+  delete st->kout;
+  delete st->kb;
+  for (void* p : {(void*)st->fe, (void*)st->f, (void*)st->ubc, (void*)st->approxS, (void*)st->h, (void*)st->eta_tab,
+                  (void*)st->force_tab, (void*)st->eta_p, (void*)st->force_p, (void*)st->red_partial,
+                  (void*)st->red_out, (void*)st->dcoef, (void*)st->dptrs, (void*)st->coarse.Ainv})
+    dfree(p);
+  delete st;
+  return 0;
+}
+
+extern "C" int uwg_stokes_set_materials(uwg_stokes* st, int nmat, const double* eta, const double* force) {
+  if (nmat < 1) return arg_error("nmat must be >= 1");
+  for (int m = 0; m < nmat; m++)
+    if (!(eta[m] > 0.0) || !std::isfinite(eta[m])) return arg_error("viscosity must be positive and finite");
+  API_BEGIN
+  dfree(st->eta_tab);
+  dfree(st->force_tab);
+  st->eta_tab = dalloc<double>(nmat);
+  st->force_tab = dalloc<double>(nmat * st->g.dim);
+  UWG_HIP(hipMemcpy(st->eta_tab, eta, sizeof(double) * nmat, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(st->force_tab, force, sizeof(double) * nmat * st->g.dim, hipMemcpyHostToDevice));
+  st->nmat = nmat;
+  st->mode = 0;
+  st->assembled_state = -1;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_set_particle_values(uwg_stokes* st, long P, const double* eta, const double* force) {
+  API_BEGIN
+  dfree(st->eta_p);
+  dfree(st->force_p);
+  st->eta_p = dalloc<double>(P);
+  st->force_p = dalloc<double>(P * st->g.dim);
+  UWG_HIP(hipMemcpy(st->eta_p, eta, sizeof(double) * P, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(st->force_p, force, sizeof(double) * P * st->g.dim, hipMemcpyHostToDevice));
+  st->np_vals = P;
+  st->mode = 1;
+  st->assembled_state = -1;
+  return 0;
+  API_END
+}
+
+static void finish_assembly(uwg_stokes* st) {
+  Level& A = st->L[0];
+  launch_node_gather_fe(st->g, st->fe, A.bc, st->f, 0);
+  UWG_HIP(hipMemset(st->h, 0, sizeof(double) * st->g.E));
+  setup_hierarchy(st);
+  st->assembled = true;
+}
+
+extern "C" int uwg_stokes_assemble(uwg_stokes* st, uwg_swarm* s) {
+  if (s->mesh->g.E != st->g.E) return arg_error("swarm and Stokes system live on different meshes");
+  if (st->mode == 0 && st->nmat == 0) return arg_error("no material tables / particle values set");
+  if (st->mode == 0 && !s->has_mat && st->nmat > 1) return arg_error("swarm has no material index");
+  if (st->mode == 1 && st->np_vals != s->P) return arg_error("particle values do not match the swarm size");
+  API_BEGIN
+  if (!s->has_integration || s->integration_state != s->state) {
+    int rc = uwg_swarm_integration_points(s, 1, 15);
+    if (rc) return rc;
+  }
+  st->swarm = s;
+  Level& A = st->L[0];
+  launch_moments(st->g, s->cell_ptr, s->cell_pid, s->xi_slot, s->w_slot, st->mode == 0 ? s->mat : nullptr, st->eta_tab,
+                 st->force_tab, st->eta_p, st->force_p, A.W, st->fe, st->approxS, st->Ge, st->nmat, 0);
+  finish_assembly(st);
+  st->assembled_state = s->state;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_assemble_fields(uwg_stokes* st, const double* eta_g, const double* force_g) {
+  API_BEGIN
+  // Gauss-point viscosities -> W_g = eta_g * w_g * detJ ; f_e = sum_g w_g detJ f(g) N_a(g)
+  // (the GaussIntegrationSwarm path of the reference when no voronoi swarm is given)
+  const Geom& g = st->g;
+  const int dim = g.dim, Q = dim == 3 ? 27 : 9, NN = 1 << dim;
+  double h[3] = {1, 1, 1}, detJ = 1.0;
+  for (int d = 0; d < dim; d++) {
+    h[d] = (g.hi[d] - g.lo[d]) / g.res[d];
+    detJ *= 0.5 * h[d];
+  }
+  const double w1[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  const double ga = sqrt(0.6);
+  std::vector<double> W((size_t)Q * g.E), fe((size_t)g.E * NN * dim, 0.0), S(g.E);
+  long nx = g.nn[0], ny = g.nn[1];
+  for (long e = 0; e < g.E; e++) {
+    long i = e % g.res[0], j = (e / g.res[0]) % g.res[1], k = dim == 3 ? e / ((long)g.res[0] * g.res[1]) : 0;
+    long base = i + nx * (j + ny * k);
+    for (int q = 0; q < Q; q++) {
+      double wq = w1[q % 3] * w1[(q / 3) % 3] * (dim == 3 ? w1[q / 9] : 1.0);
+      W[(size_t)q * g.E + e] = eta_g[e * Q + q] * wq * detJ;
+      if (force_g) {
+        double X[3] = {q % 3 == 0 ? -ga : (q % 3 == 1 ? 0.0 : ga), (q / 3) % 3 == 0 ? -ga : ((q / 3) % 3 == 1 ? 0.0 : ga),
+                       q / 9 == 0 ? -ga : (q / 9 == 1 ? 0.0 : ga)};
+        double N[8];
+        for (int a = 0; a < NN; a++) {
+          N[a] = 1.0;
+          for (int d = 0; d < dim; d++) N[a] *= 0.5 * (1.0 + ((a >> d) & 1 ? 1.0 : -1.0) * X[d]);
+        }
+        double fq[3] = {0, 0, 0};
+        (void)base;
+        for (int d = 0; d < dim; d++) fq[d] = force_g[((size_t)e * Q + q) * dim + d];
+        for (int a = 0; a < NN; a++)
+          for (int d = 0; d < dim; d++) fe[(size_t)e * NN * dim + a * dim + d] += wq * detJ * fq[d] * N[a];
+      }
+    }
+  }
+  Level& A = st->L[0];
+  UWG_HIP(hipMemcpy(A.W, W.data(), sizeof(double) * Q * g.E, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(st->fe, fe.data(), sizeof(double) * fe.size(), hipMemcpyHostToDevice));
+  finish_assembly(st);
+  // approxS from the assembled diagonal: computed with the level-0 element weights
+  {
+    std::vector<double> Wh(W);
+    double sc[3] = {2.0 / h[0], 2.0 / h[1], dim == 3 ? 2.0 / h[2] : 1.0};
+    for (long e = 0; e < g.E; e++) {
+      double s = 0.0;
+      for (int a = 0; a < NN; a++)
+        for (int i = 0; i < dim; i++) {
+          double kd = 0.0;
+          for (int q = 0; q < Q; q++) {
+            double X[3] = {q % 3 == 0 ? -ga : (q % 3 == 1 ? 0.0 : ga), (q / 3) % 3 == 0 ? -ga : ((q / 3) % 3 == 1 ? 0.0 : ga),
+                           q / 9 == 0 ? -ga : (q / 9 == 1 ? 0.0 : ga)};
+            double dn[3], s2 = 0.0;
+            for (int jd = 0; jd < dim; jd++) {
+              double v = (dim == 3 ? 0.125 : 0.25) * (((a >> jd) & 1) ? 1.0 : -1.0);
+              for (int od = 0; od < dim; od++)
+                if (od != jd) v *= 1.0 + (((a >> od) & 1) ? 1.0 : -1.0) * X[od];
+              dn[jd] = v * sc[jd];
+              s2 += dn[jd] * dn[jd];
+            }
+            kd += Wh[(size_t)q * g.E + e] * (dn[i] * dn[i] + s2);
+          }
+          s += st->Ge[a * dim + i] * st->Ge[a * dim + i] / kd;
+        }
+      S[e] = s;
+    }
+    UWG_HIP(hipMemcpy(st->approxS, S.data(), sizeof(double) * g.E, hipMemcpyHostToDevice));
+  }
+  st->assembled_state = -2;
+  return 0;
+  API_END
+}
+
+extern "C" void uwg_solve_opts_default(uwg_solve_opts* o) {
+  memset(o, 0, sizeof(*o));
+  o->method = 0;
+  o->precision = 0;
+  o->learned = 0;
+  o->outer_rtol = 1e-5;
+  o->inner_rtol = 1e-6;
+  o->mg_levels = 0;
+  o->max_outer_its = 10000;
+  o->max_inner_its = 10000;
+  o->restart = 30;
+  o->smooth_degree = 2;
+  o->verbose = 0;
+}
+
+// lift Dirichlet values: rhs_u = f - K ubc (free dofs), rhs_p = h - G^T ubc
+static void lifted_rhs(uwg_stokes* st, double* fu, double* hp, bool* has_bc) {
+  Level& A = st->L[0];
+  const long n = A.n();
+  const Geom& g = st->g;
+  // ubc = vel on constrained dofs, 0 elsewhere
+  std::vector<double> v(n);
+  std::vector<uint8_t> bc(g.N);
+  UWG_HIP(hipMemcpy(v.data(), st->vel->d, sizeof(double) * n, hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(bc.data(), A.bc, g.N, hipMemcpyDeviceToHost));
+  bool any = false;
+  for (long i = 0; i < g.N; i++)
+    for (int d = 0; d < g.dim; d++) {
+      long k = i * g.dim + d;
+      if ((bc[i] >> d) & 1) {
+        if (v[k] != 0.0) any = true;
+      } else {
+        v[k] = 0.0;
+      }
+    }
+  *has_bc = any;
+  vk_copy<double>(n, st->f, fu, 0);
+  vk_copy<double>(g.E, st->h, hp, 0);
+  if (!any) return;
+  UWG_HIP(hipMemcpy(st->ubc, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  // K applied to ubc with BC dofs treated as free: use an all-free mask temporarily
+  uint8_t* free_mask = dalloc<uint8_t>(g.N);
+  UWG_HIP(hipMemset(free_mask, 0, g.N));
+  double* t = dalloc<double>(n);
+  launch_apply_K<double>(g, A.W, st->ubc, t, free_mask, 0);
+  // zero BC rows of t then fu -= t
+  std::vector<double> th(n);
+  UWG_HIP(hipMemcpy(th.data(), t, sizeof(double) * n, hipMemcpyDeviceToHost));
+  for (long i = 0; i < g.N; i++)
+    for (int d = 0; d < g.dim; d++)
+      if ((bc[i] >> d) & 1) th[i * g.dim + d] = 0.0;
+  UWG_HIP(hipMemcpy(t, th.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  vk_axpy<double>(n, -1.0, t, fu, 0);
+  double* q = dalloc<double>(g.E);
+  launch_apply_GT(g, st->Ge, st->ubc, free_mask, q, 0);
+  vk_axpy<double>(g.E, -1.0, q, hp, 0);
+  dfree(q);
+  dfree(t);
+  dfree(free_mask);
+}
+
+static void true_residuals(uwg_stokes* st, const double* u, const double* p, double* out3) {
+  // u includes Dirichlet values; residual on free dofs of f - K u - G p and h - G^T u
+  Level& A = st->L[0];
+  const long n = A.n();
+  const Geom& g = st->g;
+  bool has_bc;
+  double* fu = dalloc<double>(n);
+  double* hp = dalloc<double>(g.E);
+  lifted_rhs(st, fu, hp, &has_bc);
+  // uf = u with BC dofs zeroed (K with identity BC rows maps those to themselves)
+  double* uf = dalloc<double>(n);
+  vk_copy<double>(n, u, uf, 0);
+  std::vector<double> h(n);
+  std::vector<uint8_t> bc(g.N);
+  UWG_HIP(hipMemcpy(h.data(), uf, sizeof(double) * n, hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(bc.data(), A.bc, g.N, hipMemcpyDeviceToHost));
+  for (long i = 0; i < g.N; i++)
+    for (int d = 0; d < g.dim; d++)
+      if ((bc[i] >> d) & 1) h[i * g.dim + d] = 0.0;
+  UWG_HIP(hipMemcpy(uf, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  double* t = dalloc<double>(n);
+  double* gp = dalloc<double>(n);
+  launch_apply_K<double>(g, A.W, uf, t, A.bc, 0);
+  launch_apply_G(g, st->Ge, p, A.bc, gp, 0);
+  vk_axpy<double>(n, 1.0, gp, t, 0);
+  vk_axpby<double>(n, 1.0, fu, -1.0, t, 0);  // t = fu - K uf - G p  (BC rows: fu=0, K uf = 0)
+  double* q = dalloc<double>(g.E);
+  launch_apply_GT(g, st->Ge, uf, A.bc, q, 0);
+  vk_axpby<double>(g.E, 1.0, hp, -1.0, q, 0);
+  out3[0] = sqrt(dev_dot(st, n, t, t));
+  out3[1] = sqrt(dev_dot(st, g.E, q, q));
+  out3[2] = sqrt(dev_dot(st, n, fu, fu));
+  for (void* pp : {(void*)fu, (void*)hp, (void*)uf, (void*)t, (void*)gp, (void*)q}) dfree(pp);
+}
+
+extern "C" int uwg_stokes_residuals(uwg_stokes* st, double* out3) {
+  if (!st->assembled) return arg_error("system not assembled");
+  API_BEGIN
+  true_residuals(st, st->vel->d, st->p->d, out3);
+  return 0;
+  API_END
+}
+
+// Schur complement solve following BSSCR_DRIVER_auglag (auglag-driver-DGTGD.c:64-584):
+//   u* = K^-1 f ; h^ = h - G^T u* ; FGMRES on S p = h^, S = -G^T K^-1 G, PC approxS ;
+//   u = K^-1 (f - G p)
+static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
+                       const double* hp, double* u, double* p) {
+  Level& A = st->L[0];
+  const long n = A.n(), E = st->g.E;
+  if (!st->kin) st->kin = new KrylovSpace();
+  if (!st->kout) st->kout = new KrylovSpace();
+  KrylovSpace& Kin = *st->kin;
+  KrylovSpace& Kout = *st->kout;
+  Kin.init(n, o.restart);
+  Kout.init(E, o.restart);
+  int status = 0;
+  OpFn Kop = [&](const double* x, double* y) { launch_apply_K<double>(A.g, A.W, x, y, A.bc, 0); };
+  OpFn Mv = [&](const double* x, double* y) {
+    st->vcycles++;
+    if (o.precision == 1) {
+      vk_d2f(n, 1.0, x, A.b32, 0);
+      vcycle<float>(st, 0, A.b32, A.x32);
+      vk_f2d(n, 1.0, A.x32, y, 0);
+    } else {
+      vk_copy<double>(n, x, A.b, 0);
+      vcycle<double>(st, 0, A.b, y);
+    }
+  };
+  auto inner = [&](const double* rhs, double* x, int* its) {
+    vk_fill<double>(n, 0.0, x, 0);
+    double rn;
+    int rc = fgmres(st, Kin, Kop, Mv, rhs, x, o.inner_rtol, 1e-300, o.max_inner_its, its, &rn, true);
+    if (rc) status = UWG_WARN_NOT_CONVERGED;
+  };
+  double* ustar = dalloc<double>(n);
+  double* hhat = dalloc<double>(E);
+  double* t = dalloc<double>(n);
+  double* z = dalloc<double>(n);
+  double* diagS = dalloc<double>(E);
+  int its = 0;
+  inner(fu, ustar, &its);
+  S->velocity_presolve_its = its;
+  launch_apply_GT(st->g, st->Ge, ustar, A.bc, hhat, 0);
+  vk_axpby<double>(E, 1.0, hp, -1.0, hhat, 0);  // h^ = h - G^T u*
+  int pss_its = 0;
+  OpFn Sop = [&](const double* v, double* y) {
+    launch_apply_G(st->g, st->Ge, v, A.bc, t, 0);
+    int ii = 0;
+    inner(t, z, &ii);
+    pss_its += ii;
+    launch_apply_GT(st->g, st->Ge, z, A.bc, y, 0);
+    vk_axpby<double>(E, 0.0, y, -1.0, y, 0);  // S v = -G^T K^-1 G v
+  };
+  // preconditioner: -approxS^-1 (approxS approximates G^T diag(K)^-1 G > 0)
+  {
+    std::vector<double> s(E);
+    UWG_HIP(hipMemcpy(s.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
+    for (auto& v : s) v = v != 0.0 ? -1.0 / v : 0.0;
+    UWG_HIP(hipMemcpy(diagS, s.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+  }
+  OpFn Sprec = [&](const double* x, double* y) { vk_xmy_scale<double>(E, diagS, x, y, 0); };
+  vk_fill<double>(E, 0.0, p, 0);
+  double rn;
+  int oits = 0;
+  int rc = fgmres(st, Kout, Sop, Sprec, hhat, p, o.outer_rtol, 1e-300, o.max_outer_its, &oits, &rn, true);
+  if (rc) status = UWG_WARN_NOT_CONVERGED;
+  S->pressure_its = oits;
+  S->velocity_pressuresolve_its = pss_its;
+  // back solve
+  launch_apply_G(st->g, st->Ge, p, A.bc, t, 0);
+  vk_axpby<double>(n, 1.0, fu, -1.0, t, 0);
+  inner(t, u, &its);
+  S->velocity_backsolve_its = its;
+  S->velocity_total_its = S->velocity_presolve_its + S->velocity_pressuresolve_its + S->velocity_backsolve_its;
+  for (void* pp : {(void*)ustar, (void*)hhat, (void*)t, (void*)z, (void*)diagS}) dfree(pp);
+  return status;
+}
+
+// Coupled solve: FGMRES on [[K, G],[G^T, 0]] with the block lower-triangular right
+// preconditioner  z_u = V-cycle(r_u), z_p = -approxS^-1 (r_p - G^T z_u).
+static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
+                       const double* hp, double* u, double* p) {
+  Level& A = st->L[0];
+  const long n = A.n(), E = st->g.E, nt = n + E;
+  if (!st->kb) st->kb = new KrylovSpace();
+  KrylovSpace& Kb = *st->kb;
+  Kb.init(nt, o.restart);
+  double* b = dalloc<double>(nt);
+  double* x = dalloc<double>(nt);
+  double* diagS = dalloc<double>(E);
+  double* q = dalloc<double>(E);
+  vk_copy<double>(n, fu, b, 0);
+  vk_copy<double>(E, hp, b + n, 0);
+  vk_fill<double>(nt, 0.0, x, 0);
+  {
+    std::vector<double> s(E);
+    UWG_HIP(hipMemcpy(s.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
+    for (auto& v : s) v = v != 0.0 ? -1.0 / v : 0.0;
+    UWG_HIP(hipMemcpy(diagS, s.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+  }
+  double* gp = dalloc<double>(n);
+  OpFn Aop = [&](const double* xx, double* yy) {
+    launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
+    launch_apply_G(st->g, st->Ge, xx + n, A.bc, gp, 0);
+    vk_axpy<double>(n, 1.0, gp, yy, 0);
+    launch_apply_GT(st->g, st->Ge, xx, A.bc, yy + n, 0);
+  };
+  OpFn Mop = [&](const double* rr, double* zz) {
+    st->vcycles++;
+    if (o.precision == 1) {
+      vk_d2f(n, 1.0, rr, A.b32, 0);
+      vcycle<float>(st, 0, A.b32, A.x32);
+      vk_f2d(n, 1.0, A.x32, zz, 0);
+    } else {
+      vk_copy<double>(n, rr, A.b, 0);
+      vcycle<double>(st, 0, A.b, zz);
+    }
+    launch_apply_GT(st->g, st->Ge, zz, A.bc, q, 0);
+    vk_axpby<double>(E, 1.0, rr + n, -1.0, q, 0);  // q = r_p - G^T z_u
+    vk_xmy_scale<double>(E, diagS, q, zz + n, 0);
+  };
+  int its = 0;
+  double rn;
+  int rc = fgmres(st, Kb, Aop, Mop, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true);
+  S->outer_its = its;
+  vk_copy<double>(n, x, u, 0);
+  vk_copy<double>(E, x + n, p, 0);
+  for (void* pp : {(void*)b, (void*)x, (void*)diagS, (void*)q, (void*)gp}) dfree(pp);
+  return rc ? UWG_WARN_NOT_CONVERGED : 0;
+}
+
+extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_solve_stats* S) {
+  uwg_solve_opts o;
+  if (opts)
+    o = *opts;
+  else
+    uwg_solve_opts_default(&o);
+  uwg_solve_stats dummy;
+  if (!S) S = &dummy;
+  memset(S, 0, sizeof(*S));
+  if (o.restart < 2 || o.restart > 200) return arg_error("restart must be in [2, 200]");
+  if (o.method == 0 && o.learned) return arg_error("the learned correction is only available with method 1");
+  if (o.learned && !st->nn) return arg_error("no learned model loaded");
+  API_BEGIN
+  double t0 = now();
+  if (o.mg_levels > 0 && o.mg_levels != (int)st->L.size()) {
+    st->levels_req = o.mg_levels;
+    build_levels(st, o.mg_levels);  // keeps the level-0 weights
+    if (st->assembled) setup_hierarchy(st);
+  }
+  if (st->swarm && st->assembled_state != st->swarm->state && st->assembled_state != -2) {
+    int rc = uwg_stokes_assemble(st, st->swarm);
+    if (rc) return rc;
+  }
+  if (!st->assembled) return arg_error("system not assembled");
+  g_smooth_degree = std::max(1, o.smooth_degree);
+  if (o.precision == 1) ensure_fp32(st);
+  UWG_HIP(hipDeviceSynchronize());
+  double t1 = now();
+  st->vcycles = 0;
+  Level& A = st->L[0];
+  const long n = A.n(), E = st->g.E;
+  bool has_bc;
+  double* fu = dalloc<double>(n);
+  double* hp = dalloc<double>(E);
+  lifted_rhs(st, fu, hp, &has_bc);
+  double* u = dalloc<double>(n);
+  double* p = dalloc<double>(E);
+  int status;
+  if (o.method == 0)
+    status = solve_schur(st, o, S, fu, hp, u, p);
+  else
+    status = solve_block(st, o, S, fu, hp, u, p);
+  // u (free dofs) + Dirichlet values -> velocity field ; p -> pressure field
+  if (has_bc) vk_axpy<double>(n, 1.0, st->ubc, u, 0);
+  else {
+    // keep Dirichlet entries of the velocity field (zero here): zero them in u
+  }
+  vk_copy<double>(n, u, st->vel->d, 0);
+  vk_copy<double>(E, p, st->p->d, 0);
+  UWG_HIP(hipDeviceSynchronize());
+  double t2 = now();
+  double r3[3];
+  true_residuals(st, st->vel->d, st->p->d, r3);
+  S->res_momentum = r3[0];
+  S->res_continuity = r3[1];
+  S->norm_f = r3[2];
+  S->vcycles = st->vcycles;
+  S->levels = (int)st->L.size();
+  S->converged = status == 0;
+  S->setup_time = t1 - t0;
+  S->solve_time = t2 - t1;
+  S->total_time = t2 - t0;
+  dfree(fu);
+  dfree(hp);
+  dfree(u);
+  dfree(p);
+  if (!std::isfinite(r3[0]) || !std::isfinite(r3[1])) {
+    set_error("A floating-point error has been detected during the solve.");
+    return UWG_WARN_NONFINITE;
+  }
+  if (status) set_error("Krylov solve did not converge within the iteration limit");
+  return status;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// hooks
+// ---------------------------------------------------------------------------------------
+extern "C" int uwg_stokes_apply(uwg_stokes* st, int which, int level, const double* x, double* y) {
+  if (!st->assembled) return arg_error("system not assembled");
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
+  API_BEGIN
+  Level& A = st->L[level];
+  long nin, nout;
+  if (which == 0 || which == 3) nin = nout = A.n();
+  else if (which == 1) { nin = st->g.E; nout = st->L[0].n(); }
+  else if (which == 2) { nin = st->L[0].n(); nout = st->g.E; }
+  else return arg_error("which must be 0..3");
+  double* dx = dalloc<double>(nin);
+  double* dy = dalloc<double>(nout);
+  UWG_HIP(hipMemcpy(dx, x, sizeof(double) * nin, hipMemcpyHostToDevice));
+  if (which == 0)
+    launch_apply_K<double>(A.g, A.W, dx, dy, A.bc, 0);
+  else if (which == 1)
+    launch_apply_G(st->g, st->Ge, dx, st->L[0].bc, dy, 0);
+  else if (which == 2)
+    launch_apply_GT(st->g, st->Ge, dx, st->L[0].bc, dy, 0);
+  else {
+    vk_copy<double>(nin, dx, st->L[0].b, 0);
+    vcycle<double>(st, 0, st->L[0].b, dy);
+  }
+  UWG_HIP(hipMemcpy(y, dy, sizeof(double) * nout, hipMemcpyDeviceToHost));
+  dfree(dx);
+  dfree(dy);
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_levels(uwg_stokes* st, int* nlev, int* res_out) {
+  *nlev = (int)st->L.size();
+  if (res_out)
+    for (size_t l = 0; l < st->L.size(); l++)
+      for (int d = 0; d < 3; d++) res_out[l * 3 + d] = st->L[l].g.res[d];
+  return 0;
+}
+
+extern "C" int uwg_stokes_get_assembly(uwg_stokes* st, int level, double* W, double* fe, double* approxS) {
+  if (!st->assembled) return arg_error("system not assembled");
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
+  API_BEGIN
+  Level& A = st->L[level];
+  const long Q = A.g.dim == 3 ? 27 : 9;
+  if (W) UWG_HIP(hipMemcpy(W, A.W, sizeof(double) * Q * A.g.E, hipMemcpyDeviceToHost));
+  if (level == 0) {
+    const int NN = 1 << st->g.dim;
+    if (fe) UWG_HIP(hipMemcpy(fe, st->fe, sizeof(double) * st->g.E * NN * st->g.dim, hipMemcpyDeviceToHost));
+    if (approxS) UWG_HIP(hipMemcpy(approxS, st->approxS, sizeof(double) * st->g.E, hipMemcpyDeviceToHost));
+  }
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_get_rhs(uwg_stokes* st, double* f) {
+  API_BEGIN
+  UWG_HIP(hipMemcpy(f, st->f, sizeof(double) * st->L[0].n(), hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_set_apply_chunk(int kz3, int kz2) {
+  set_apply_chunk(kz3, kz2);
+  return 0;
+}
+
+extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int reps, double* ms) {
+  if (!st->assembled) return arg_error("system not assembled");
+  API_BEGIN
+  Level& A = st->L[0];
+  hipEvent_t e0, e1;
+  UWG_HIP(hipEventCreate(&e0));
+  UWG_HIP(hipEventCreate(&e1));
+  if (precision == 1) {
+    ensure_fp32(st);
+    launch_apply_K<float>(A.g, A.W32, A.x32, A.r32, A.bc, 0);
+    UWG_HIP(hipEventRecord(e0, 0));
+    for (int i = 0; i < reps; i++) launch_apply_K<float>(A.g, A.W32, A.x32, A.r32, A.bc, 0);
+    UWG_HIP(hipEventRecord(e1, 0));
+  } else {
+    launch_apply_K<double>(A.g, A.W, A.x, A.r, A.bc, 0);
+    UWG_HIP(hipEventRecord(e0, 0));
+    for (int i = 0; i < reps; i++) launch_apply_K<double>(A.g, A.W, A.x, A.r, A.bc, 0);
+    UWG_HIP(hipEventRecord(e1, 0));
+  }
+  UWG_HIP(hipEventSynchronize(e1));
+  float t;
+  UWG_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  UWG_HIP(hipEventDestroy(e0));
+  UWG_HIP(hipEventDestroy(e1));
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_load_nn(uwg_stokes* st, int level, int nparams, const float* params, int channels, int depth) {
+  if (level < 1 || level >= (int)st->L.size()) return arg_error("learned correction level out of range");
+  API_BEGIN
+  st->nn.reset(nn_create(st->L[level].g, channels, depth, params, nparams));
+  st->nn_level = level;
+  return 0;
+  API_END
+}

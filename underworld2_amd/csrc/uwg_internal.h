@@ -1,0 +1,136 @@
+// Internal declarations shared by the HIP translation units of libuwg.
+// Layout conventions (all device memory, row-major):
+//   node index  n = i + nx*(j + ny*k), nx = res[0]+1 (x fastest)
+//   element     e = i + rx*(j + ry*k)
+//   velocity-like vectors: [N][dim] interleaved (node-major, matches MeshVariable.data)
+//   pressure-like vectors: [E]
+//   per-element quadrature weights W: [Q][E] (structure of arrays), Q = 3^dim
+//   Dirichlet mask: uint8 per node, bit i set when component i is constrained
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace uwg {
+
+// last-error text, retrievable through uwg_last_error()
+void set_error(const std::string& msg);
+
+#define UWG_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      char _b[512];                                                                     \
+      snprintf(_b, sizeof(_b), "HIP error %s at %s:%d: %s", hipGetErrorName(_e),        \
+               __FILE__, __LINE__, hipGetErrorString(_e));                              \
+      throw std::runtime_error(_b);                                                     \
+    }                                                                                   \
+  } while (0)
+
+struct Geom {
+  int dim;
+  int res[3];      // elements per dim (res[2] = 1 in 2D)
+  double lo[3], hi[3];
+  long N, E;       // nodes, elements
+  int nn[3];       // nodes per dim
+};
+
+inline Geom make_geom(int dim, const int* res, const double* lo, const double* hi) {
+  Geom g{};
+  g.dim = dim;
+  for (int d = 0; d < 3; d++) {
+    g.res[d] = d < dim ? res[d] : 1;
+    g.lo[d] = d < dim ? lo[d] : 0.0;
+    g.hi[d] = d < dim ? hi[d] : 1.0;
+    g.nn[d] = d < dim ? res[d] + 1 : 1;
+  }
+  g.N = (long)g.nn[0] * g.nn[1] * g.nn[2];
+  g.E = (long)g.res[0] * g.res[1] * g.res[2];
+  return g;
+}
+
+// ---------------------------------------------------------------------------------------
+// launch wrappers: swarm_kernels.hip (compiled with -ffp-contract=off, bit-exact paths)
+// ---------------------------------------------------------------------------------------
+void launch_layout_random(const Geom& g, int ppc, long seed, double* coords, hipStream_t s);
+void launch_mesh_coords(const Geom& g, double* out, hipStream_t s);
+void launch_cell_of(const Geom& g, long P, const double* coords, int* owner, hipStream_t s);
+void launch_coincident_map(const Geom& g, long P, const int* slot_cell, const int* cell_pid,
+                           const double* coords, double* xi_slot, hipStream_t s);
+void launch_slot_cell(long E, const int* cell_ptr, int* slot_cell, hipStream_t s);
+void launch_advect_rk2(const Geom& g, long P, const double* vel, double* coords, double dt,
+                       int* d_counts /*[2]: bad, fallback*/, hipStream_t s);
+void launch_interpolate(const Geom& g, long P, const double* vel, const double* x, double* v,
+                        int* ok, hipStream_t s);
+void launch_node_speed_max(const Geom& g, const double* vel, double* partial, int nblk, hipStream_t s);
+
+// ---------------------------------------------------------------------------------------
+// launch wrappers: fem_kernels.hip (solver / assembly kernels)
+// ---------------------------------------------------------------------------------------
+struct MatTable {   // material tables for the moment kernel
+  int nmat;
+  const double* eta;    // [nmat]
+  const double* force;  // [nmat][dim]
+};
+void launch_moments(const Geom& g, const int* cell_ptr, const int* cell_pid, const double* xi_slot,
+                    const double* w_slot, const int* mat, const double* eta_tab, const double* force_tab,
+                    const double* eta_p, const double* force_p, double* W, double* fe, double* approxS,
+                    const double* Ge /*host, 2^d*d values*/, int nmat, hipStream_t s);
+void launch_node_gather_fe(const Geom& g, const double* fe, const uint8_t* bc, double* f, hipStream_t s);
+void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, double* Wc, hipStream_t s);
+void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
+template <typename T>
+void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
+void launch_apply_G(const Geom& g, const double* Ge, const double* p, const uint8_t* bc, double* y,
+                    hipStream_t s);
+void launch_apply_GT(const Geom& g, const double* Ge, const double* u, const uint8_t* bc, double* q,
+                     hipStream_t s);
+template <typename T>
+void launch_prolong_add(const Geom& fine, const Geom& coarse, const T* xc, T* xf, const uint8_t* bcf,
+                        hipStream_t s);
+template <typename T>
+void launch_restrict2(const Geom& fine, const Geom& coarse, const T* rf, T* rc, const uint8_t* bcc,
+                      const uint8_t* bcf, hipStream_t s);
+void set_apply_chunk(int kz3, int kz2);
+long dvc_scratch_bytes(int dim, int num, int max_np, int cap);
+void launch_dvc_list(int dim, int num, long E0, long nE, const int* elist, const int* cell_ptr,
+                     double* xi_slot, double* w_slot, void* scratch, int max_np, int cap, int* d_flag,
+                     hipStream_t s);
+// relocation (relocate.hip): rebuild per-cell tables after owners change; returns the number
+// of particles outside the domain
+struct RelocWork;
+RelocWork* reloc_create(long P, long E);
+void reloc_destroy(RelocWork* w);
+long relocate(const Geom& g, long P, const double* coords, int* owner, int* cell_ptr, int* cell_pid,
+              RelocWork* w, hipStream_t s);
+void launch_bc_mask_from_walls(const Geom& g, uint8_t* bc, int wallbits, hipStream_t s);
+
+// vector kernels (T = double or float)
+template <typename T> void vk_fill(long n, T v, T* x, hipStream_t s);
+template <typename T> void vk_copy(long n, const T* x, T* y, hipStream_t s);
+template <typename T> void vk_axpy(long n, T a, const T* x, T* y, hipStream_t s);           // y += a x
+template <typename T> void vk_axpby(long n, T a, const T* x, T b, T* y, hipStream_t s);     // y = a x + b y
+template <typename T> void vk_xmy_scale(long n, const T* d, const T* x, T* y, hipStream_t s); // y = d .* x
+void vk_d2f(long n, double scale, const double* x, float* y, hipStream_t s);
+void vk_f2d_axpy(long n, double scale, const float* x, double* y, hipStream_t s);  // y += scale*x
+void vk_f2d(long n, double scale, const float* x, double* y, hipStream_t s);        // y = scale*x
+void vk_d2f_vec(long n, const double* x, float* y, hipStream_t s);
+// y = sum_j c[j] * X[j]  (+ y if accumulate)   X: array of device pointers (device memory)
+void vk_multi_axpy(long n, int k, const double* const* X, const double* c, double* y, hipStream_t s);
+// partial sums for dot products: out[j] = <x, Y[j]> for j < k ; deterministic
+void vk_multi_dot(long n, int k, const double* x, const double* const* Y, double* partial,
+                  double* out, hipStream_t s);
+int vk_reduce_blocks();
+// chebyshev helper: d = a*d + b*(dinv .* r) ; x += d  (fused)
+template <typename T>
+void vk_cheb_update(long n, T a, T b, const T* dinv, const T* r, T* d, T* x, hipStream_t s);
+// r = b - r  (r holds A x on input)
+template <typename T> void vk_residual(long n, const T* b, T* r, hipStream_t s);
+// dense coarse solve y = Ainv x for a small dense inverse (double)
+void launch_dense_mv(int n, const double* A, const double* x, double* y, hipStream_t s);
+// max-reduction partials -> value
+double finish_max(const double* d_partial, int nblk);
+
+}  // namespace uwg

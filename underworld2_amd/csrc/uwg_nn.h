@@ -1,0 +1,18 @@
+// Learned coarse-grid correction: a small convolutional network evaluated with the
+// hand-written HIP kernels of nn_kernels.hip.
+#pragma once
+#include "uwg_internal.h"
+
+namespace uwg {
+
+struct NNModel;
+NNModel* nn_create(const Geom& g, int channels, int depth, const float* params, int nparams);
+void nn_destroy(NNModel* m);
+// x = NN(features(level), r) on the nodes of level g (FP32); Dirichlet dofs of x are zero
+void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s);
+
+struct NNDeleter {
+  void operator()(NNModel* m) const { nn_destroy(m); }
+};
+
+}  // namespace uwg

@@ -1,0 +1,197 @@
+"""Meshes and mesh variables, mirroring ``underworld.mesh`` (PY/mesh/_mesh.py).
+
+``FeMesh_Cartesian("Q1/dQ0", elementRes, minCoord, maxCoord)`` is a regular box mesh of
+Q1 velocity elements with a piecewise-constant (dQ0) pressure sub-mesh.  Node numbering is
+lexicographic with x fastest (CartesianGenerator.c:2001-2039); element-local nodes are
+ordered (-,-,-), (+,-,-), (-,+,-), (+,+,-), then +z.
+
+``MeshVariable.data`` is a host numpy array of shape [nodes][nodeDofCount] (or
+[elements][count] on the dQ0 sub-mesh).  The authoritative copy lives on the GPU; the host
+view is refreshed on access and pushed back before the next device operation.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class IndexSet:
+    """Node index set (PY/container/_indexset.py); ``a + b`` is the union."""
+
+    def __init__(self, indices, size):
+        self._idx = np.unique(np.asarray(indices, dtype=np.int64))
+        self.size = size
+
+    def __add__(self, other):
+        return IndexSet(np.union1d(self._idx, other._idx), self.size)
+
+    def __len__(self):
+        return len(self._idx)
+
+    @property
+    def data(self):
+        return self._idx
+
+    def __contains__(self, i):
+        return int(i) in set(self._idx.tolist())
+
+
+class _SubMesh:
+    """The dQ0 pressure mesh: one value per element."""
+
+    def __init__(self, parent):
+        self.parent = parent
+        self.elementType = "DQ0"
+        self.elementRes = parent.elementRes
+        self.dim = parent.dim
+
+    @property
+    def nodesGlobal(self):
+        return self.parent.elementsGlobal
+
+
+class FeMesh_Cartesian:
+    """Regular Cartesian finite-element mesh (PY/mesh/_mesh.py:1046-1190).
+
+    Only ``elementType="Q1/dQ0"`` is provided by this engine.
+    """
+
+    def __init__(self, elementType="Q1/dQ0", elementRes=(4, 4), minCoord=(0.0, 0.0), maxCoord=(1.0, 1.0),
+                 periodic=None, partitioned=True, **kwargs):
+        if not isinstance(elementType, str) or elementType.upper() not in ("Q1/DQ0", "Q1"):
+            raise ValueError("Only the 'Q1/dQ0' element type is provided by this engine.")
+        if periodic is not None and any(periodic):
+            raise ValueError("Periodic meshes are not supported by this engine.")
+        if len(elementRes) not in (2, 3) or len(minCoord) != len(elementRes) or len(maxCoord) != len(elementRes):
+            raise ValueError("elementRes, minCoord and maxCoord must have the same length (2 or 3).")
+        self.elementType = "Q1"
+        self.dim = len(elementRes)
+        self.elementRes = tuple(int(r) for r in elementRes)
+        self.minCoord = tuple(float(c) for c in minCoord)
+        self.maxCoord = tuple(float(c) for c in maxCoord)
+        h = ctypes.c_void_p()
+        res = np.array(self.elementRes, dtype=np.int32)
+        lo = np.array(self.minCoord)
+        hi = np.array(self.maxCoord)
+        L.check(L.lib().uwg_mesh_create(self.dim, L.iptr(res), L.dptr(lo), L.dptr(hi), ctypes.byref(h)),
+                "FeMesh_Cartesian")
+        self._h = h
+        self.nodesGlobal = int(np.prod([r + 1 for r in self.elementRes]))
+        self.elementsGlobal = int(np.prod(self.elementRes))
+        self.subMesh = _SubMesh(self)
+        self._data = None
+        self._special = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.lib().uwg_mesh_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def data(self):
+        """Node coordinates [nodes][dim]."""
+        if self._data is None:
+            out = np.zeros((self.nodesGlobal, self.dim))
+            L.check(L.lib().uwg_mesh_coords(self._h, L.dptr(out)), "mesh.data")
+            out.flags.writeable = False
+            self._data = out
+        return self._data
+
+    @property
+    def specialSets(self):
+        """Wall node sets: Min/Max I/J/K _VertexSet, plus AllWalls_VertexSet."""
+        if self._special is None:
+            n = [r + 1 for r in self.elementRes]
+            idx = np.indices(n[::-1]).reshape(self.dim, -1)[::-1]
+            names = "IJK"
+            sets = {}
+            allw = np.zeros(self.nodesGlobal, dtype=bool)
+            for d in range(self.dim):
+                lo = np.flatnonzero(idx[d] == 0)
+                hi = np.flatnonzero(idx[d] == n[d] - 1)
+                sets[f"Min{names[d]}_VertexSet"] = IndexSet(lo, self.nodesGlobal)
+                sets[f"Max{names[d]}_VertexSet"] = IndexSet(hi, self.nodesGlobal)
+                allw[lo] = True
+                allw[hi] = True
+            sets["AllWalls_VertexSet"] = IndexSet(np.flatnonzero(allw), self.nodesGlobal)
+            self._special = sets
+        return self._special
+
+    def _min_separation(self):
+        v = ctypes.c_double()
+        L.check(L.lib().uwg_mesh_min_separation(self._h, ctypes.byref(v)))
+        return v.value
+
+
+class MeshVariable:
+    """Nodal (or dQ0 element) field, ``MeshVariable(mesh, nodeDofCount)``."""
+
+    def __init__(self, mesh, nodeDofCount, dataType="double", **kwargs):
+        if dataType != "double":
+            raise ValueError("Only 'double' mesh variables are provided by this engine.")
+        on_elements = isinstance(mesh, _SubMesh)
+        base = mesh.parent if on_elements else mesh
+        self.mesh = mesh
+        self._base = base
+        self.nodeDofCount = int(nodeDofCount)
+        self._on_elements = on_elements
+        n = base.elementsGlobal if on_elements else base.nodesGlobal
+        h = ctypes.c_void_p()
+        L.check(L.lib().uwg_field_create(base._h, self.nodeDofCount, 1 if on_elements else 0, ctypes.byref(h)),
+                "MeshVariable")
+        self._h = h
+        self._host = np.zeros((n, self.nodeDofCount))
+        self._dev_newer = False
+        self._host_touched = False
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.lib().uwg_field_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def data(self):
+        if self._dev_newer:
+            L.check(L.lib().uwg_field_get(self._h, L.dptr(self._host)), "MeshVariable.data")
+            self._dev_newer = False
+        self._host_touched = True
+        return self._host
+
+    @data.setter
+    def data(self, value):
+        self.data[...] = value
+
+    # device synchronisation helpers used by the systems
+    def _push(self):
+        if self._host_touched:
+            L.check(L.lib().uwg_field_set(self._h, L.dptr(self._host)), "MeshVariable push")
+            self._host_touched = False
+
+    def _mark_device_written(self):
+        self._dev_newer = True
+        self._host_touched = False
+
+    def evaluate(self, coords):
+        """Interpolate a nodal vector field at points (FeVariable_InterpolateValueAt)."""
+        self._push()
+        x = L.f64(coords).reshape(-1, self._base.dim)
+        v = np.zeros_like(x)
+        ok = np.zeros(x.shape[0], dtype=np.int32)
+        L.check(L.lib().uwg_field_interpolate(self._h, x.shape[0], L.dptr(x), L.dptr(v), L.iptr(ok)), "evaluate")
+        if not ok.all():
+            raise RuntimeError("Some coordinates lie outside the mesh")
+        return v
+
+    # function interface (a MeshVariable used as a function is evaluated by interpolation)
+    def _fn_inputs(self):
+        return {("meshvar", id(self))}
+
+    def _fn_eval(self, ctx):
+        return self.evaluate(ctx.coords)

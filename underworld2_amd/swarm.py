@@ -1,0 +1,199 @@
+"""Particle swarms, mirroring ``underworld.swarm`` (PY/swarm/_swarm.py, layouts.py).
+
+Particle coordinates and per-cell particle tables live on the GPU.  ``particleCoordinates
+.data`` is a host view refreshed after device updates; assigning into it and calling
+``update_particle_owners()`` pushes the new coordinates back.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from . import function as _fn
+
+
+class _CoordVariable:
+    def __init__(self, swarm):
+        self._swarm = swarm
+
+    @property
+    def data(self):
+        return self._swarm._coords_host()
+
+
+class SwarmVariable(_fn.Function):
+    """Per-particle data ``swarm.add_variable(dataType, count)`` (PY/swarm/_swarmvariable.py).
+
+    Data are held on the host; the integer variable used as a material index is uploaded
+    to the device by the Stokes system when it assembles."""
+
+    _dtypes = {"double": np.float64, "float": np.float32, "int": np.int32, "char": np.int8, "short": np.int16,
+               "long": np.int64, "bool": np.bool_}
+
+    def __init__(self, swarm, dataType, count):
+        if dataType not in self._dtypes:
+            raise ValueError(f"unsupported dataType {dataType}")
+        self.swarm = swarm
+        self.dataType = dataType
+        self.count = int(count)
+        self._data = np.zeros((swarm.particleLocalCount, self.count), dtype=self._dtypes[dataType])
+        self.version = 0
+
+    @property
+    def data(self):
+        if self._data.shape[0] != self.swarm.particleLocalCount:
+            new = np.zeros((self.swarm.particleLocalCount, self.count), dtype=self._data.dtype)
+            k = min(self._data.shape[0], new.shape[0])
+            new[:k] = self._data[:k]
+            self._data = new
+        self.version += 1
+        return self._data
+
+    def _fn_inputs(self):
+        return {("swarmvar", id(self))}
+
+    def _fn_eval(self, ctx):
+        if id(self) in ctx.override:
+            return ctx.override[id(self)]
+        if ctx.swarm is not self.swarm:
+            raise RuntimeError("SwarmVariable evaluated outside its own swarm")
+        return self.data
+
+
+class Swarm:
+    """``Swarm(mesh, particleEscape=False)``."""
+
+    def __init__(self, mesh, particleEscape=False, **kwargs):
+        if particleEscape:
+            raise ValueError("particleEscape=True is not supported by this engine")
+        self.mesh = mesh
+        self.particleEscape = False
+        h = ctypes.c_void_p()
+        L.check(L.lib().uwg_swarm_create(mesh._h, ctypes.byref(h)), "Swarm")
+        self._h = h
+        self._variables = []
+        self._host = np.zeros((0, mesh.dim))
+        self._host_valid = True
+        self._host_touched = False
+        self.particleCoordinates = _CoordVariable(self)
+        self.stateId = 0
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.lib().uwg_swarm_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def particleLocalCount(self):
+        P = ctypes.c_long()
+        L.check(L.lib().uwg_swarm_count(self._h, ctypes.byref(P)))
+        return P.value
+
+    @property
+    def particleGlobalCount(self):
+        return self.particleLocalCount
+
+    def _coords_host(self):
+        if not self._host_valid:
+            P = self.particleLocalCount
+            self._host = np.zeros((P, self.mesh.dim))
+            if P:
+                L.check(L.lib().uwg_swarm_get_coords(self._h, L.dptr(self._host)), "particleCoordinates")
+            self._host_valid = True
+        self._host_touched = True
+        return self._host
+
+    def _push(self):
+        if self._host_touched and self._host_valid and self._host.shape[0] == self.particleLocalCount:
+            out = ctypes.c_long()
+            rc = L.lib().uwg_swarm_set_coords(self._h, L.dptr(np.ascontiguousarray(self._host)), ctypes.byref(out))
+            self._host_touched = False
+            self.stateId += 1
+            if rc < 0:
+                raise RuntimeError("Particles appear to have left the domain, but swarm flag `particleEscape` "
+                                   "is False. Check your velocity field or your particle relocation routines, "
+                                   "or set the `particleEscape` swarm constructor parameter to True to allow escape.")
+
+    def _device_moved(self):
+        self._host_valid = False
+        self._host_touched = False
+        self.stateId += 1
+
+    def add_variable(self, dataType, count):
+        v = SwarmVariable(self, dataType, count)
+        self._variables.append(v)
+        return v
+
+    def populate_using_layout(self, layout):
+        layout._populate(self)
+        self._device_moved()
+
+    def add_particles_with_coordinates(self, coordinatesArray):
+        c = L.f64(coordinatesArray).reshape(-1, self.mesh.dim)
+        P0 = self.particleLocalCount
+        rc = L.lib().uwg_swarm_add_particles(self._h, c.shape[0], L.dptr(c))
+        self._device_moved()
+        if rc < 0:
+            raise RuntimeError(L.last_error())
+        return np.arange(P0, P0 + c.shape[0], dtype=np.int32)
+
+    def update_particle_owners(self):
+        """Swarm.update_particle_owners (PY/swarm/_swarm.py:613-654)."""
+        if self._host_touched:
+            self._push()
+            return
+        out = ctypes.c_long()
+        rc = L.lib().uwg_swarm_update_owners(self._h, ctypes.byref(out))
+        self.stateId += 1
+        if rc < 0:
+            raise RuntimeError(L.last_error())
+
+    @property
+    def owningCell(self):
+        class _V:
+            pass
+
+        v = _V()
+        P = self.particleLocalCount
+        o = np.zeros(P, dtype=np.int32)
+        if P:
+            L.check(L.lib().uwg_swarm_get_owners(self._h, L.iptr(o)))
+        v.data = o.reshape(-1, 1)
+        return v
+
+    def cell_tables(self):
+        """(cell_ptr[E+1], cell_pid[P]): each element's particles in table order."""
+        E = self.mesh.elementsGlobal
+        ptr = np.zeros(E + 1, dtype=np.int32)
+        pid = np.zeros(max(self.particleLocalCount, 1), dtype=np.int32)
+        L.check(L.lib().uwg_swarm_get_cell_tables(self._h, L.iptr(ptr), L.iptr(pid)))
+        return ptr, pid[:self.particleLocalCount]
+
+    def integration_points(self, dvc=True, resolution=15):
+        """Voronoi integration swarm (coincident map + DVC weights), slot order."""
+        self._push()
+        L.check(L.lib().uwg_swarm_integration_points(self._h, 1 if dvc else 0, resolution), "repopulate")
+        P = self.particleLocalCount
+        xi = np.zeros((P, self.mesh.dim))
+        w = np.zeros(P)
+        L.check(L.lib().uwg_swarm_get_integration(self._h, L.dptr(xi), L.dptr(w)))
+        return xi, w
+
+
+class _Layouts:
+    class PerCellRandomLayout:
+        """``PerCellRandomLayout(swarm, particlesPerCell, seed=13)`` (PY/swarm/layouts.py:186-229)."""
+
+        def __init__(self, swarm, particlesPerCell, seed=13, **kwargs):
+            self.swarm = swarm
+            self.particlesPerCell = int(particlesPerCell)
+            self.seed = int(seed)
+
+        def _populate(self, swarm):
+            L.check(L.lib().uwg_swarm_populate_random(swarm._h, self.particlesPerCell, self.seed), "populate")
+
+
+layouts = _Layouts()

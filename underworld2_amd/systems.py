@@ -1,0 +1,426 @@
+"""Stokes system, solver and swarm advector, mirroring ``underworld.systems``.
+
+* ``Stokes``        PY/systems/_stokes.py:106-317 (construction), :395-448 (residuals, v_rms)
+* ``Solver``        PY/systems/_solver.py:26-35 -> ``StokesSolver`` PY/systems/_bsscr.py
+* ``SwarmAdvector`` PY/systems/_timeintegration.py:77-157
+
+The reference assembles PETSc matrices and runs the BSSCR Schur-complement solver on the
+CPU.  Here the element integration, the matrix-free operators, the multigrid hierarchy and
+the Krylov iterations all run on the GPU through libuwg; results are written back into the
+``MeshVariable`` objects exactly as the reference's solve does.
+"""
+import ctypes
+import warnings
+
+import numpy as np
+
+from . import _lib as L
+from . import function as _fn
+from .conditions import DirichletCondition
+from .mesh import MeshVariable
+from .swarm import Swarm, SwarmVariable
+
+_GA = np.sqrt(0.6)
+
+
+def _gauss_points(mesh):
+    """Physical coordinates of the 3^dim Gauss points of every element, [E][Q][dim],
+    point index q = qx + 3 qy + 9 qz."""
+    d = mesh.dim
+    res = np.array(mesh.elementRes)
+    lo = np.array(mesh.minCoord)
+    hi = np.array(mesh.maxCoord)
+    h = (hi - lo) / res
+    x1 = np.array([-_GA, 0.0, _GA])
+    if d == 2:
+        ej, ei = np.meshgrid(np.arange(res[1]), np.arange(res[0]), indexing="ij")
+        cen = np.stack([lo[0] + (ei.ravel() + 0.5) * h[0], lo[1] + (ej.ravel() + 0.5) * h[1]], 1)
+        qy, qx = np.meshgrid(x1, x1, indexing="ij")
+        off = np.stack([qx.ravel() * h[0] / 2, qy.ravel() * h[1] / 2], 1)
+    else:
+        ek, ej, ei = np.meshgrid(np.arange(res[2]), np.arange(res[1]), np.arange(res[0]), indexing="ij")
+        cen = np.stack([lo[0] + (ei.ravel() + 0.5) * h[0], lo[1] + (ej.ravel() + 0.5) * h[1],
+                        lo[2] + (ek.ravel() + 0.5) * h[2]], 1)
+        qz, qy, qx = np.meshgrid(x1, x1, x1, indexing="ij")
+        off = np.stack([qx.ravel() * h[0] / 2, qy.ravel() * h[1] / 2, qz.ravel() * h[2] / 2], 1)
+    return cen[:, None, :] + off[None, :, :]
+
+
+class Stokes:
+    """``Stokes(velocityField, pressureField, fn_viscosity, fn_bodyforce=None,
+    voronoi_swarm=None, conditions=[])``."""
+
+    def __init__(self, velocityField, pressureField, fn_viscosity, fn_bodyforce=None, voronoi_swarm=None,
+                 conditions=[], _removeBCs=True, _fn_viscosity2=None, _fn_director=None, fn_stresshistory=None,
+                 _fn_stresshistory=None, fn_one_on_lambda=None, _fn_one_on_lambda=None, fn_source=None,
+                 _fn_fssa=None, gauss_swarm=None, **kwargs):
+        if not isinstance(velocityField, MeshVariable) or velocityField._on_elements:
+            raise TypeError("Provided 'velocityField' must be a nodal 'MeshVariable'.")
+        if not isinstance(pressureField, MeshVariable) or not pressureField._on_elements:
+            raise TypeError("Provided 'pressureField' must be a MeshVariable on the dQ0 sub-mesh.")
+        for name, v in (("fn_viscosity2", _fn_viscosity2), ("fn_director", _fn_director),
+                        ("fn_stresshistory", fn_stresshistory or _fn_stresshistory),
+                        ("fn_one_on_lambda", fn_one_on_lambda or _fn_one_on_lambda), ("fn_source", fn_source),
+                        ("fn_fssa", _fn_fssa)):
+            if v is not None:
+                raise NotImplementedError(f"Stokes '{name}' is not provided by this engine")
+        if voronoi_swarm is not None and not isinstance(voronoi_swarm, Swarm):
+            raise TypeError("Provided 'voronoi_swarm' must be of 'Swarm' class.")
+        self._velocityField = velocityField
+        self._pressureField = pressureField
+        self._swarm = voronoi_swarm
+        mesh = velocityField._base
+        self._mesh = mesh
+        mask = np.zeros(mesh.nodesGlobal, dtype=np.uint8)
+        for cond in conditions:
+            if not isinstance(cond, DirichletCondition):
+                raise TypeError("Provided 'conditions' must be 'DirichletCondition' objects.")
+            if cond.variable is velocityField:
+                mask |= cond.mask(mesh.nodesGlobal)
+            elif cond.variable is pressureField:
+                raise NotImplementedError("Dirichlet conditions on the pressure are not provided by this engine")
+            else:
+                raise ValueError("Provided condition does not appear to correspond to the system unknowns.")
+        self._conditions = conditions
+        self._bcmask = mask
+        h = ctypes.c_void_p()
+        L.check(L.lib().uwg_stokes_create(velocityField._h, pressureField._h,
+                                          mask.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), ctypes.byref(h)),
+                "Stokes")
+        self._h = h
+        self.fn_viscosity = fn_viscosity
+        self.fn_bodyforce = fn_bodyforce if fn_bodyforce is not None else (0.0,) * mesh.dim
+        self._values_key = None
+        self._nn_model = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.lib().uwg_stokes_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def fn_viscosity(self):
+        return self._fn_viscosity
+
+    @fn_viscosity.setter
+    def fn_viscosity(self, value):
+        self._fn_viscosity = _fn.convert(value)
+        self._values_key = None
+
+    @property
+    def fn_bodyforce(self):
+        return self._fn_bodyforce
+
+    @fn_bodyforce.setter
+    def fn_bodyforce(self, value):
+        self._fn_bodyforce = _fn.convert(value)
+        self._values_key = None
+
+    # --- function values -> device ------------------------------------------------------
+    def _table_variable(self):
+        """The single integer SwarmVariable both functions depend on, or None."""
+        ins = self._fn_viscosity._fn_inputs() | self._fn_bodyforce._fn_inputs()
+        if len(ins) != 1:
+            return None
+        tok = next(iter(ins))
+        if tok[0] != "swarmvar":
+            return None
+        for v in self._swarm._variables:
+            if id(v) == tok[1] and v.dataType in ("int", "char", "short", "long") and v.count == 1:
+                return v
+        return None
+
+    def _prepare_values(self):
+        lib = L.lib()
+        d = self._mesh.dim
+        if self._swarm is None:
+            pts = _gauss_points(self._mesh)
+            E, Q = pts.shape[0], pts.shape[1]
+            flat = pts.reshape(-1, d)
+            eta = np.asarray(self._fn_viscosity.evaluate(flat), dtype=np.float64).reshape(E, Q)
+            force = np.asarray(self._fn_bodyforce.evaluate(flat), dtype=np.float64).reshape(E, Q, -1)
+            if force.shape[2] != d:
+                raise ValueError("fn_bodyforce must be a vector of the mesh dimension")
+            if not np.all(eta > 0):
+                raise ValueError("viscosity must be positive everywhere")
+            L.check(lib.uwg_stokes_assemble_fields(self._h, L.dptr(L.f64(eta)), L.dptr(L.f64(force))), "assemble")
+            return
+        sw = self._swarm
+        sw._push()
+        var = self._table_variable()
+        if var is not None:
+            key = ("table", id(var), var.version, sw.stateId)
+            if key != self._values_key:
+                mats = np.ascontiguousarray(var.data[:, 0], dtype=np.int32)
+                if mats.size and mats.min() < 0:
+                    raise ValueError("material indices must be non-negative")
+                nmat = int(mats.max()) + 1 if mats.size else 1
+                vals = np.arange(nmat, dtype=np.int64)[:, None]
+                from .function import _Ctx
+                ctx = _Ctx(coords=np.zeros((nmat, d)), override={id(var): vals})
+                eta = np.asarray(self._fn_viscosity._fn_eval(ctx), dtype=np.float64).reshape(nmat)
+                force = np.asarray(self._fn_bodyforce._fn_eval(ctx), dtype=np.float64).reshape(nmat, -1)
+                if force.shape[1] != d:
+                    raise ValueError("fn_bodyforce must be a vector of the mesh dimension")
+                L.check(lib.uwg_swarm_set_material(sw._h, L.iptr(mats)), "materials")
+                L.check(lib.uwg_stokes_set_materials(self._h, nmat, L.dptr(L.f64(eta)), L.dptr(L.f64(force))),
+                        "materials")
+                self._values_key = ("table", id(var), var.version, sw.stateId)
+        else:
+            key = ("particles", sw.stateId) + tuple(v.version for v in sw._variables)
+            if key != self._values_key:
+                eta = np.asarray(self._fn_viscosity.evaluate(sw), dtype=np.float64).reshape(-1)
+                force = np.asarray(self._fn_bodyforce.evaluate(sw), dtype=np.float64).reshape(eta.shape[0], -1)
+                if not np.all(eta > 0):
+                    raise ValueError("viscosity must be positive everywhere")
+                L.check(lib.uwg_stokes_set_particle_values(self._h, eta.shape[0], L.dptr(L.f64(eta)),
+                                                           L.dptr(L.f64(force))), "particle values")
+                self._values_key = ("particles", sw.stateId) + tuple(v.version for v in sw._variables)
+        L.check(lib.uwg_stokes_assemble(self._h, sw._h), "assemble")
+
+    # --- analysis -------------------------------------------------------------------------
+    @property
+    def eqResiduals(self):
+        """(|f - K u - G p|, |h - G^T u|) on the free dofs (PY/systems/_stokes.py:395-416)."""
+        self._velocityField._push()
+        self._pressureField._push()
+        out = np.zeros(3)
+        L.check(L.lib().uwg_stokes_residuals(self._h, L.dptr(out)), "eqResiduals")
+        return out[0], out[1]
+
+    def velocity_rms(self):
+        """sqrt(int v.v / int 1) by 3^d Gauss quadrature (PY/systems/_stokes.py:435-448)."""
+        mesh = self._mesh
+        v = self._velocityField.data
+        d = mesh.dim
+        res = mesh.elementRes
+        nn = [r + 1 for r in res]
+        x1 = np.array([-_GA, 0.0, _GA])
+        w1 = np.array([5.0, 8.0, 5.0]) / 9.0
+        V = v.reshape(tuple(nn[::-1]) + (d,))
+        acc = 0.0
+        if d == 2:
+            for qi in range(3):
+                for qj in range(3):
+                    X, Y = x1[qi], x1[qj]
+                    N = [(1 - X) * (1 - Y) / 4, (1 + X) * (1 - Y) / 4, (1 - X) * (1 + Y) / 4, (1 + X) * (1 + Y) / 4]
+                    val = (N[0] * V[:-1, :-1] + N[1] * V[:-1, 1:] + N[2] * V[1:, :-1] + N[3] * V[1:, 1:])
+                    acc += w1[qi] * w1[qj] * np.sum(val * val)
+        else:
+            for qi in range(3):
+                for qj in range(3):
+                    for qk in range(3):
+                        X, Y, Z = x1[qi], x1[qj], x1[qk]
+                        val = 0.0
+                        for a in range(8):
+                            sx = 1 if a & 1 else -1
+                            sy = 1 if a & 2 else -1
+                            sz = 1 if a & 4 else -1
+                            Na = (1 + sx * X) * (1 + sy * Y) * (1 + sz * Z) / 8
+                            sl = (slice(1, None) if sz > 0 else slice(None, -1),
+                                  slice(1, None) if sy > 0 else slice(None, -1),
+                                  slice(1, None) if sx > 0 else slice(None, -1))
+                            val = val + Na * V[sl]
+                        acc += w1[qi] * w1[qj] * w1[qk] * np.sum(val * val)
+        vol = 1.0
+        h = [(mesh.maxCoord[i] - mesh.minCoord[i]) / res[i] for i in range(d)]
+        detJ = np.prod([hh / 2 for hh in h])
+        for i in range(d):
+            vol *= mesh.maxCoord[i] - mesh.minCoord[i]
+        return float(np.sqrt(acc * detJ / vol))
+
+
+class _Options:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def list(self):
+        for k, v in self.__dict__.items():
+            print(k, v)
+
+
+class _OptionsGroup:
+    def __init__(self):
+        self.A11 = _Options(ksp_type="fgmres", ksp_rtol=1e-6, _mg_active=True)
+        self.scr = _Options(ksp_type="fgmres", ksp_rtol=1e-5)
+        self.mg = _Options(active=True, levels=0)
+        self.main = _Options(Q22_pc_type="uw", penalty=0.0)
+        self.mg_accel = _Options(mg_accelerating_smoothing=False)
+        self.rhsA11 = _Options()
+        self.backsolveA11 = _Options()
+
+
+class StokesSolver:
+    """Block Stokes Schur-complement solver (PY/systems/_bsscr.py).
+
+    Engine-specific additions (not in the reference): ``set_method("schur"|"block")``,
+    ``set_precision("fp64"|"mixed")`` and ``set_learned(model)``."""
+
+    def __init__(self, stokesSLE, **kwargs):
+        if not isinstance(stokesSLE, Stokes):
+            raise TypeError("Provided system must be of 'Stokes' class")
+        self._stokesSLE = stokesSLE
+        self.options = _OptionsGroup()
+        self._method = 0
+        self._precision = 0
+        self._learned = False
+        self._restart = 30
+        self._smooth_degree = 2
+        self._stats = L.SolveStats()
+
+    def set_inner_method(self, solve_type="mg"):
+        if solve_type != "mg":
+            raise ValueError(f"Provided inner method '{solve_type}' is not provided by this engine "
+                             f"(only 'mg', geometric multigrid on the GPU).")
+        self.options.A11.ksp_rtol = 1e-6 if not hasattr(self.options.A11, "ksp_rtol") else self.options.A11.ksp_rtol
+
+    def set_inner_rtol(self, rtol):
+        if not isinstance(rtol, float):
+            raise TypeError("Provided 'rtol' parameter must be of type 'float'.")
+        self.options.A11.ksp_rtol = rtol
+
+    def set_outer_rtol(self, rtol):
+        if not isinstance(rtol, float):
+            raise TypeError("Provided 'rtol' parameter must be of type 'float'.")
+        self.options.scr.ksp_rtol = rtol
+
+    def set_mg_levels(self, levels):
+        if not isinstance(levels, int):
+            raise TypeError("Provided 'levels' parameter must be of type 'int'.")
+        if levels < 0:
+            raise ValueError("Provided 'levels' parameter must be non-negative.")
+        self.options.mg.levels = levels
+
+    def set_penalty(self, penalty):
+        raise NotImplementedError("The augmented-Lagrangian penalty path is not provided by this engine")
+
+    # engine-specific
+    def set_method(self, method):
+        self._method = {"schur": 0, "bsscr": 0, "block": 1}[method]
+
+    def set_precision(self, precision):
+        self._precision = {"fp64": 0, "mixed": 1}[precision]
+
+    def set_learned(self, model):
+        """Attach a trained learned coarse correction (``underworld2_amd.learned``)."""
+        if model is None:
+            self._learned = False
+            return
+        model.attach(self._stokesSLE)
+        self._learned = True
+
+    def _opts(self):
+        o = L.SolveOpts()
+        L.lib().uwg_solve_opts_default(ctypes.byref(o))
+        o.method = self._method
+        o.precision = self._precision
+        o.learned = 1 if self._learned else 0
+        o.outer_rtol = float(self.options.scr.ksp_rtol)
+        o.inner_rtol = float(self.options.A11.ksp_rtol)
+        o.mg_levels = int(self.options.mg.levels)
+        o.restart = self._restart if self._method == 0 else max(self._restart, 60)
+        o.smooth_degree = self._smooth_degree
+        return o
+
+    def solve(self, nonLinearIterate=None, nonLinearTolerance=1.0e-2, nonLinearKillNonConvergent=False,
+              nonLinearMinIterations=1, nonLinearMaxIterations=500, callback_post_solve=None, print_stats=False,
+              reinitialise=True, **kwargs):
+        """StokesSolver.solve (PY/systems/_bsscr.py:322-502)."""
+        st = self._stokesSLE
+        if kwargs:
+            raise ValueError(f"PETSc options {sorted(kwargs)} are not understood by this engine")
+        deps = st._fn_viscosity._fn_inputs() | st._fn_bodyforce._fn_inputs()
+        nonlinear = ("meshvar", id(st._velocityField)) in deps or ("meshvar", id(st._pressureField)) in deps
+        if nonlinear and nonLinearIterate is None:
+            raise RuntimeError("Nonlinearity detected.\nPlease set the 'nonLinearIterate' solve parameter to "
+                               "'True' or 'False' to continue.")
+        if nonlinear and nonLinearIterate:
+            raise NotImplementedError("Non-linear (Picard) iterations are not provided by this engine")
+        st._velocityField._push()
+        st._pressureField._push()
+        if reinitialise or st._values_key is None:
+            st._prepare_values()
+        stats = L.SolveStats()
+        rc = L.lib().uwg_stokes_solve(st._h, ctypes.byref(self._opts()), ctypes.byref(stats))
+        L.check(rc, "solve")
+        st._velocityField._mark_device_written()
+        st._pressureField._mark_device_written()
+        self._stats = stats
+        if callback_post_solve is not None:
+            if not callable(callback_post_solve):
+                raise RuntimeError("The 'callback_post_solve' parameter is not 'None' and isn't callable")
+            callback_post_solve()
+        if rc == L.UWG_WARN_NOT_CONVERGED:
+            warnings.warn("A solver error has been encountered during the solve: the Krylov iteration did not "
+                          "converge within its iteration limit. Solution fields are possibly erroneous.")
+        elif rc == L.UWG_WARN_NONFINITE:
+            warnings.warn("A floating-point error has been detected during the solve. "
+                          "Solution fields are possibly erroneous.")
+        if print_stats:
+            self.print_stats()
+
+    def get_stats(self):
+        return self._stats
+
+    def print_stats(self):
+        s = self._stats
+        print(f"Pressure iterations: {s.pressure_its:3d}")
+        print(f"Velocity iterations: {s.velocity_presolve_its:3d} (presolve)")
+        print(f"Velocity iterations: {s.velocity_pressuresolve_its:3d} (pressure solve)")
+        print(f"Velocity iterations: {s.velocity_backsolve_its:3d} (backsolve)")
+        print(f"Velocity iterations: {s.velocity_total_its:3d} (total solve)")
+        print(f"Coupled iterations : {s.outer_its:3d}   V-cycles: {s.vcycles}")
+        print(f"Setup time         : {s.setup_time:.4e}")
+        print(f"Solve time         : {s.solve_time:.4e}")
+        print(f"Total solve time   : {s.total_time:.4e}")
+        print(f"Residuals |f-Ku-Gp| = {s.res_momentum:.3e}, |G^T u| = {s.res_continuity:.3e}, |f| = {s.norm_f:.3e}")
+
+
+def Solver(eqs, type="BSSCR", *args, **kwargs):
+    """Solver factory (PY/systems/_solver.py:26-35)."""
+    if isinstance(eqs, Stokes):
+        return StokesSolver(eqs, **kwargs)
+    raise TypeError("This engine provides solvers for 'Stokes' systems only")
+
+
+class SwarmAdvector:
+    """``SwarmAdvector(velocityField, swarm, order=2)`` (PY/systems/_timeintegration.py:77-157)."""
+
+    def __init__(self, velocityField, swarm, order=2, **kwargs):
+        if not isinstance(velocityField, MeshVariable):
+            raise ValueError("Provided 'velocityField' must be of 'MeshVariable' class.")
+        if swarm and not isinstance(swarm, Swarm):
+            raise ValueError("Provided 'swarm' must be of 'Swarm' class.")
+        if order != 2:
+            raise ValueError("Only second-order (RK2) swarm advection is provided by this engine")
+        self._velocityField = velocityField
+        self._swarm = swarm
+        self.order = order
+        self.time = 0.0
+        self.dt = 0.0
+
+    def get_max_dt(self):
+        self._velocityField._push()
+        dt = ctypes.c_double()
+        L.check(L.lib().uwg_advect_max_dt(self._velocityField._h, ctypes.byref(dt)), "get_max_dt")
+        return dt.value
+
+    def integrate(self, dt, update_owners=True):
+        self._velocityField._push()
+        sw = self._swarm
+        sw._push()
+        out = ctypes.c_long()
+        rc = L.lib().uwg_advect_rk2(sw._h, self._velocityField._h, float(dt), 1 if update_owners else 0,
+                                    ctypes.byref(out))
+        sw._device_moved()
+        if rc < 0:
+            msg = L.last_error()
+            if rc == L.lib().__dict__.get("UWG_ERR_DOMAIN", -3) or "left the domain" in msg:
+                raise RuntimeError("Particles appear to have left the domain, but swarm flag `particleEscape` is "
+                                   "False. " + msg)
+            raise RuntimeError(msg)
+        self.time += dt
+        self.dt = dt
