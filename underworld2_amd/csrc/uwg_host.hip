@@ -935,7 +935,7 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
 // Converged when |r| <= max(rtol |r0|, atol) with r0 the initial residual
 // (KSPConvergedDefault with zero initial guess).
 // ---------------------------------------------------------------------------------------
-struct KrylovSpace {
+struct uwg::KrylovSpace {
   long n = 0;
   int m = 0;
   std::vector<double*> V, Z;
