@@ -1,0 +1,207 @@
+"""Benchmark: Stokes solves/sec on the 3D 128^3 Q1/dQ0 sinker (eta contrast 1e3,
+30 particles per cell, free-slip box), relative residual 1e-6, on one MI355X.
+
+One step = one end-to-end ``solver.solve()`` on a fixed swarm: particle-quadrature element
+integration (A6-A9: moments, approxS, right-hand side), multigrid hierarchy setup, and the
+Krylov solve to outer rtol 1e-6 -- the same work the reference's ``StokesSolver.solve`` does
+with ``reinitialise=True`` (the voronoi swarm is unchanged, so DVC is not re-run).
+
+The headline ``value`` is the learned+mixed configuration when a trained model is present
+(``--mode learned``), otherwise the mixed-precision coupled solver.  Every mode's own
+timing, iteration counts and FP64 residuals are reported in ``modes``.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 128] [--ppc 30]
+                       [--mode mixed|fp64|learned|schur] [--cpu-baseline/--no-cpu-baseline]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_problem(res, ppc, seed=0):
+    import underworld2_amd as uw
+    fn = uw.function
+    dim = len(res)
+    mesh = uw.mesh.FeMesh_Cartesian("Q1/dQ0", res, (0.0,) * dim, (1.0,) * dim)
+    vel = uw.mesh.MeshVariable(mesh, dim)
+    p = uw.mesh.MeshVariable(mesh.subMesh, 1)
+    swarm = uw.swarm.Swarm(mesh)
+    t0 = time.time()
+    swarm.populate_using_layout(uw.swarm.layouts.PerCellRandomLayout(swarm, particlesPerCell=ppc, seed=seed))
+    mat = swarm.add_variable("int", 1)
+    centre = (0.5, 0.5, 0.6) if dim == 3 else (0.5, 0.6)
+    c = fn.input() - centre
+    mat.data[:] = fn.branching.conditional([(fn.math.dot(c, c) < 0.1 ** 2, 1), (True, 0)]).evaluate(swarm)
+    log(f"populate + materials: {time.time() - t0:.2f}s ({swarm.particleLocalCount} particles)")
+    eta = fn.branching.map(fn_key=mat, mapping={0: 1.0, 1: 1.0e3})
+    rho = fn.branching.map(fn_key=mat, mapping={0: 0.0, 1: 1.0})
+    g = (0.0, 0.0, -1.0) if dim == 3 else (0.0, -1.0)
+    names = "IJK"
+    walls = tuple(mesh.specialSets[f"Min{names[d]}_VertexSet"] + mesh.specialSets[f"Max{names[d]}_VertexSet"]
+                  for d in range(dim))
+    bc = uw.conditions.DirichletCondition(vel, walls)
+    stokes = uw.systems.Stokes(vel, p, fn_viscosity=eta, fn_bodyforce=rho * g, voronoi_swarm=swarm, conditions=[bc])
+    t0 = time.time()
+    swarm.integration_points(dvc=True)
+    log(f"coincident map + DVC weights: {time.time() - t0:.2f}s")
+    return dict(uw=uw, mesh=mesh, vel=vel, p=p, swarm=swarm, stokes=stokes)
+
+
+def make_solver(P, mode, rtol):
+    s = P["uw"].systems.Solver(P["stokes"])
+    if mode == "schur":
+        s.set_method("schur")
+        s.set_precision("fp64")
+        s.set_inner_rtol(rtol * 0.1)
+        s.set_outer_rtol(rtol)
+    else:
+        s.set_method("block")
+        s.set_precision("fp64" if mode == "fp64" else "mixed")
+        s.set_outer_rtol(rtol)
+        if mode == "learned":
+            from underworld2_amd import learned
+            model = learned.load_default(P["stokes"])
+            if model is None:
+                return None
+            s.set_learned(model)
+    return s
+
+
+def time_mode(P, mode, steps, warmup, rtol):
+    import underworld2_amd._lib as L
+    s = make_solver(P, mode, rtol)
+    if s is None:
+        return None
+    for _ in range(warmup):
+        s.solve()
+    L.lib()  # noqa
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        s.solve()
+        stats.append(s.get_stats().as_dict())
+    t = time.perf_counter() - t0
+    st = stats[-1]
+    rel = float(np.hypot(st["res_momentum"], st["res_continuity"]) / st["norm_f"])
+    return dict(mode=mode, seconds_per_solve=t / steps, solves_per_sec=steps / t, outer_its=st["outer_its"],
+                pressure_its=st["pressure_its"], velocity_its=st["velocity_total_its"], vcycles=st["vcycles"],
+                levels=st["levels"], setup_s=st["setup_time"], solve_s=st["solve_time"], rel_residual_fp64=rel,
+                converged=bool(st["converged"]))
+
+
+def roofline(P, precision, reps=50):
+    """Average duration of the K apply (the dominant kernel) measured with HIP events on the
+    stream it runs on; algorithmic bytes = E*27*s (weights) + 2*3*N*s (read x, write y)."""
+    import underworld2_amd._lib as L
+    st = P["stokes"]
+    ms = ctypes.c_double()
+    L.check(L.lib().uwg_stokes_time_apply(st._h, precision, reps, ctypes.byref(ms)), "time_apply")
+    mesh = P["mesh"]
+    E, N, d = mesh.elementsGlobal, mesh.nodesGlobal, mesh.dim
+    s = 4 if precision == 1 else 8
+    Q = 3 ** d
+    bytes_ = E * Q * s + 2 * d * N * s
+    ach = bytes_ / (ms.value * 1e-3)
+    return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+            "traffic": None, "kernel": f"k_apply3<{'float' if precision else 'double'}>",
+            "kernel_ms": ms.value, "algorithmic_bytes": bytes_}
+
+
+def cpu_baseline(res_sample=(20, 20, 20), ppc=30):
+    """The oracle (scalar C restatement + scipy sparse direct solve) on a bounded sample of the
+    same workload; reported per solve and scaled to 128^3 by element count."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from sinker import sphere_material
+    t0 = time.perf_counter()
+    out = O.full_pipeline(3, res_sample, (0.0,) * 3, (1.0,) * 3, ppc, 0,
+                          lambda x: sphere_material(x, (0.5, 0.5, 0.6), 0.1), (1.0, 1.0e3), (0.0, 1.0),
+                          (0.0, 0.0, -1.0))
+    t_all = time.perf_counter() - t0
+    # time assembly + solve only (the per-solve work), excluding layout and DVC
+    t1 = time.perf_counter()
+    Ke, fe, Ge, S = O.assemble_elements(out["mesh"], out["ptr"], out["xi"], out["w"], out["eta"], out["force"])
+    sysm = O.System(out["mesh"], out["mesh"].freeslip_mask(), Ke, fe, Ge, S)
+    sysm.solve()
+    t_solve = time.perf_counter() - t1
+    E_sample = int(np.prod(res_sample))
+    scale = (128 ** 3) / E_sample
+    return {"value": 1.0 / (t_solve * scale), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"oracle assembly + sparse direct solve of the {res_sample[0]}^3 sinker "
+                      f"({E_sample} elements, {ppc} ppc): {t_solve:.2f}s per solve, scaled x{scale:.0f} by "
+                      f"element count to 128^3 (optimistic for the CPU: direct solve cost grows faster); "
+                      f"full oracle pipeline incl. layout+DVC took {t_all:.2f}s",
+            "seconds_per_solve_sample": t_solve}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=30)
+    ap.add_argument("--rtol", type=float, default=1e-6)
+    ap.add_argument("--mode", default="auto")
+    ap.add_argument("--modes", default="fp64,mixed,learned")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import underworld2_amd as uw
+    log(uw.device_info())
+    res = (args.res,) * 3
+    P = build_problem(res, args.ppc)
+    modes = {}
+    for m in args.modes.split(","):
+        m = m.strip()
+        if not m:
+            continue
+        log(f"timing mode {m} ...")
+        r = time_mode(P, m, args.steps, args.warmup, args.rtol)
+        if r is not None:
+            modes[m] = r
+            log(json.dumps(r))
+    head = args.mode
+    if head == "auto":
+        head = "learned" if "learned" in modes else "mixed"
+    H = modes[head]
+    roof = roofline(P, 0 if head == "fp64" else 1)
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # baseline is reported, never required
+            cpu = {"error": str(e)}
+    line = {
+        "metric": "Stokes solves/sec at 128^3 Q1/dQ0, eta-contrast 1e3, rtol 1e-6",
+        "value": H["solves_per_sec"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": H["seconds_per_solve"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 Krylov / f32 multigrid" if head != "fp64" else "f64",
+        "data": "synthetic: seeded PerCellRandomLayout sinker (sphere r=0.1 at (0.5,0.5,0.6), eta 1e3, rho 1)",
+        "config": {"workload": f"3D sinker {args.res}^3 Q1/dQ0, {args.ppc} ppc, free-slip, outer rtol {args.rtol}, "
+                               f"end-to-end solve() on a fixed swarm, mode={head}"},
+        "roofline": roof, "cpu_baseline": cpu, "modes": modes,
+    }
+    if "fp64" in modes:
+        line["speedup_vs_plain_fp64"] = H["solves_per_sec"] / modes["fp64"]["solves_per_sec"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

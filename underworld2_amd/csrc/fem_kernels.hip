@@ -815,6 +815,10 @@ __global__ void k_axpby(long n, T a, const T* x, T b, T* y) {
     y[i] = a * x[i] + b * y[i];
 }
 template <typename T>
+__global__ void k_scale(long n, T a, const T* x, T* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = a * x[i];
+}
+template <typename T>
 __global__ void k_xmy(long n, const T* d, const T* x, T* y) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = d[i] * x[i];
 }
@@ -862,6 +866,9 @@ template <typename T> void vk_axpy(long n, T a, const T* x, T* y, hipStream_t s)
 template <typename T> void vk_axpby(long n, T a, const T* x, T b, T* y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpby<T>, vgrid(n), 256, 0, s, n, a, x, b, y);
 }
+template <typename T> void vk_scale(long n, T a, const T* x, T* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale<T>, vgrid(n), 256, 0, s, n, a, x, y);
+}
 template <typename T> void vk_xmy_scale(long n, const T* d, const T* x, T* y, hipStream_t s) {
   hipLaunchKernelGGL(k_xmy<T>, vgrid(n), 256, 0, s, n, d, x, y);
 }
@@ -877,6 +884,7 @@ void vk_cheb_update(long n, T a, T b, const T* dinv, const T* r, T* d, T* x, hip
   template void vk_copy<T>(long, const T*, T*, hipStream_t);                  \
   template void vk_axpy<T>(long, T, const T*, T*, hipStream_t);               \
   template void vk_axpby<T>(long, T, const T*, T, T*, hipStream_t);           \
+  template void vk_scale<T>(long, T, const T*, T*, hipStream_t);              \
   template void vk_xmy_scale<T>(long, const T*, const T*, T*, hipStream_t);   \
   template void vk_residual<T>(long, const T*, T*, hipStream_t);              \
   template void vk_cheb_update<T>(long, T, T, const T*, const T*, T*, T*, hipStream_t);

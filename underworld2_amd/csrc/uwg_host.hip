@@ -804,7 +804,7 @@ static void estimate_lmax(uwg_stokes* st, Level& A) {
     double nx2;
     UWG_HIP(hipMemcpy(&nx2, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
     lam = sqrt(nd / nx2);
-    vk_axpby<double>(n, 1.0 / sqrt(nd), A.d, 0.0, A.x, 0);
+    vk_scale<double>(n, 1.0 / sqrt(nd), A.d, A.x, 0);
   }
   A.lmax = lam;
 }
@@ -860,6 +860,7 @@ template <> struct LV<float> {
 }  // namespace
 
 static int g_smooth_degree = 2;
+static int g_verbose = 0;
 
 template <typename T>
 static void chebyshev(Level& A, const T* b, T* x, int degree, bool zero_guess) {
@@ -1010,7 +1011,7 @@ static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, 
       conv = beta <= std::max(rtol * r0, atol);
       break;
     }
-    vk_axpby<double>(n, 0.0, K.V[0], 1.0 / beta, K.V[0], 0);
+    vk_scale<double>(n, 1.0 / beta, K.V[0], K.V[0], 0);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
     int j = 0;
@@ -1024,7 +1025,7 @@ static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, 
       for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
       double hn = sqrt(std::max(0.0, dev_dot(st, n, K.w, K.w)));
       hcol[j + 1] = hn;
-      if (hn > 0) vk_axpby<double>(n, 0.0, K.w, 1.0 / hn, K.V[j + 1], 0);
+      if (hn > 0) vk_scale<double>(n, 1.0 / hn, K.w, K.V[j + 1], 0);
       // Givens
       for (int i = 0; i < j; i++) {
         double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
@@ -1041,6 +1042,7 @@ static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, 
       for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hcol[i];
       its++;
       rn = fabs(g[j + 1]);
+      if (g_verbose) fprintf(stderr, "  fgmres n=%ld it %d |r| %.3e rel %.3e\n", n, its, rn, rn / r0);
       if (rn <= std::max(rtol * r0, atol) || hn == 0.0) {
         j++;
         break;
@@ -1411,7 +1413,7 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
     inner(t, z, &ii);
     pss_its += ii;
     launch_apply_GT(st->g, st->Ge, z, A.bc, y, 0);
-    vk_axpby<double>(E, 0.0, y, -1.0, y, 0);  // S v = -G^T K^-1 G v
+    vk_scale<double>(E, -1.0, y, y, 0);  // S v = -G^T K^-1 G v
   };
   // preconditioner: -approxS^-1 (approxS approximates G^T diag(K)^-1 G > 0)
   {
@@ -1516,6 +1518,7 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   }
   if (!st->assembled) return arg_error("system not assembled");
   g_smooth_degree = std::max(1, o.smooth_degree);
+  g_verbose = o.verbose || getenv("UWG_VERBOSE") != nullptr;
   if (o.precision == 1) ensure_fp32(st);
   UWG_HIP(hipDeviceSynchronize());
   double t1 = now();

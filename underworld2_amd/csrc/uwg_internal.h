@@ -112,6 +112,7 @@ template <typename T> void vk_fill(long n, T v, T* x, hipStream_t s);
 template <typename T> void vk_copy(long n, const T* x, T* y, hipStream_t s);
 template <typename T> void vk_axpy(long n, T a, const T* x, T* y, hipStream_t s);           // y += a x
 template <typename T> void vk_axpby(long n, T a, const T* x, T b, T* y, hipStream_t s);     // y = a x + b y
+template <typename T> void vk_scale(long n, T a, const T* x, T* y, hipStream_t s);          // y = a x (y may alias x)
 template <typename T> void vk_xmy_scale(long n, const T* d, const T* x, T* y, hipStream_t s); // y = d .* x
 void vk_d2f(long n, double scale, const double* x, float* y, hipStream_t s);
 void vk_f2d_axpy(long n, double scale, const float* x, double* y, hipStream_t s);  // y += scale*x
