@@ -70,7 +70,7 @@ def make_solver(P, mode, rtol):
         s.set_outer_rtol(rtol)
     else:
         s.set_method("block")
-        s.set_precision("fp64" if mode == "fp64" else "mixed")
+        s.set_precision({"fp64": "fp64", "mixed-cycle": "mixed-cycle"}.get(mode, "mixed"))
         s.set_outer_rtol(rtol)
         if mode == "learned":
             from underworld2_amd import learned

@@ -147,8 +147,35 @@ int uwg_stokes_levels(uwg_stokes *st, int *nlev, int *res_out /*[nlev][3]*/);
  * approxS[E] */
 int uwg_stokes_get_assembly(uwg_stokes *st, int level, double *W, double *fe, double *approxS);
 int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
-/* learned coarse correction: weights blob + level at which it replaces the coarse cycle */
+/* ---- learned coarse correction (no counterpart in the reference: the preconditioner
+ * extension named by the build's method).  The network acts on the nodes of one multigrid
+ * level; mode 0 replaces the coarse cycle below that level, mode 1 adds a learned
+ * correction of the coarse cycle's remaining error.  FP32 cycles only (precision 1). */
 int uwg_stokes_load_nn(uwg_stokes *st, int level, int nparams, const float *params, int channels, int depth);
+int uwg_stokes_set_nn_mode(uwg_stokes *st, int mode);
+/* record up to max_samples coarse right-hand sides (every `every`-th) reaching `level`
+ * during multigrid cycles; retrieved as float[count][N_level*dim] */
+int uwg_stokes_capture(uwg_stokes *st, int level, int max_samples, int every);
+int uwg_stokes_get_captured(uwg_stokes *st, float *out, int *count);
+/* solve K_level x = b to rtol (FGMRES + cycles from that level, FP64) */
+int uwg_stokes_level_solve(uwg_stokes *st, int level, const double *b, double *x, double rtol, int *its);
+/* standardised log-viscosity per node of a level, and its Dirichlet mask */
+int uwg_stokes_level_feature(uwg_stokes *st, int level, float *feat);
+int uwg_stokes_level_mask(uwg_stokes *st, int level, unsigned char *mask);
+
+/* network training / evaluation on a node grid nx*ny*nz (inputs [B][4][N], outputs
+ * [B][3][N], channel-major); loss = mean_b |m (y - t)|^2 / |m t|^2 */
+typedef struct uwg_nn uwg_nn;
+int uwg_nn_create(int nx, int ny, int nz, int channels, int depth, long seed, uwg_nn **out);
+int uwg_nn_destroy(uwg_nn *h);
+int uwg_nn_param_count(uwg_nn *h, long *n);
+int uwg_nn_get_params(uwg_nn *h, float *out);
+int uwg_nn_set_params(uwg_nn *h, const float *in);
+int uwg_nn_forward(uwg_nn *h, int B, const float *x, float *y);
+int uwg_nn_loss_grad(uwg_nn *h, int B, const float *x, const float *t, const float *mask, float *loss, float *grads);
+int uwg_nn_set_data(uwg_nn *h, long S, const float *x, const float *t, const float *mask);
+int uwg_nn_train(uwg_nn *h, int steps, int B, float lr, float weight_decay, long seed, float *losses);
+int uwg_nn_eval(uwg_nn *h, int B, float *loss);
 /* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply */
 int uwg_set_apply_chunk(int kz3, int kz2);
 /* time `reps` applies of K (level 0) in the given precision; returns ms per apply */

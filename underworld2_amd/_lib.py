@@ -84,6 +84,23 @@ SIGNATURES = {
     "uwg_stokes_get_assembly": [_vp, ctypes.c_int, _dp, _dp, _dp],
     "uwg_stokes_get_rhs": [_vp, _dp],
     "uwg_stokes_load_nn": [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int],
+    "uwg_stokes_set_nn_mode": [_vp, ctypes.c_int],
+    "uwg_stokes_capture": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "uwg_stokes_get_captured": [_vp, _fp, _ip],
+    "uwg_stokes_level_solve": [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, _ip],
+    "uwg_stokes_level_feature": [_vp, ctypes.c_int, _fp],
+    "uwg_stokes_level_mask": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte)],
+    "uwg_nn_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
+                      ctypes.POINTER(_vp)],
+    "uwg_nn_destroy": [_vp],
+    "uwg_nn_param_count": [_vp, _lp],
+    "uwg_nn_get_params": [_vp, _fp],
+    "uwg_nn_set_params": [_vp, _fp],
+    "uwg_nn_forward": [_vp, ctypes.c_int, _fp, _fp],
+    "uwg_nn_loss_grad": [_vp, ctypes.c_int, _fp, _fp, _fp, _fp, _fp],
+    "uwg_nn_set_data": [_vp, ctypes.c_long, _fp, _fp, _fp],
+    "uwg_nn_train": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_long, _fp],
+    "uwg_nn_eval": [_vp, ctypes.c_int, _fp],
     "uwg_set_apply_chunk": [ctypes.c_int, ctypes.c_int],
     "uwg_stokes_time_apply": [_vp, ctypes.c_int, ctypes.c_int, _dp],
 }

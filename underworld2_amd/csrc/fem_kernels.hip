@@ -364,33 +364,82 @@ __device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, c
   }
   T dX[3] = {0, 0, 0}, dY[3] = {0, 0, 0}, dZ[3] = {0, 0, 0}, dXY[3] = {0, 0, 0}, dXZ[3] = {0, 0, 0},
     dYZ[3] = {0, 0, 0}, dXYZ[3] = {0, 0, 0};
+  // Sum factorisation.  du/dx is constant along x, so its values at the 27 points are 9
+  // "lines" indexed by (Y, Z); likewise du/dy by (X, Z) and du/dz by (X, Y).  The dual
+  // contraction factorises the same way: the x-stress is summed along each x-line first.
+  // Planes of constant Z are processed in turn so only the z-lines stay live throughout.
+  T GZ[3][9], PZ[3][9];
 #pragma unroll
-  for (int g = 0; g < 27; g++) {
-    const T X = (T)gpt(g % 3), Y = (T)gpt((g / 3) % 3), Z = (T)gpt(g / 9);
-    const T w = W[(long)g * E + e];
-    T G[3][3];
+  for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      G[i][0] = sx * (cX[i] + cXY[i] * Y + cXZ[i] * Z + cXYZ[i] * (Y * Z));
-      G[i][1] = sy * (cY[i] + cXY[i] * X + cYZ[i] * Z + cXYZ[i] * (X * Z));
-      G[i][2] = sz * (cZ[i] + cXZ[i] * X + cYZ[i] * Y + cXYZ[i] * (X * Y));
+    for (int q = 0; q < 9; q++) {
+      const T X = (T)gpt(q % 3), Y = (T)gpt(q / 3);
+      GZ[i][q] = sz * (cZ[i] + cXZ[i] * X + cYZ[i] * Y + cXYZ[i] * (X * Y));
+      PZ[i][q] = (T)0;
     }
-    const T w2 = w + w;
-    T S00 = w2 * G[0][0], S11 = w2 * G[1][1], S22 = w2 * G[2][2];
-    T S01 = w * (G[0][1] + G[1][0]), S02 = w * (G[0][2] + G[2][0]), S12 = w * (G[1][2] + G[2][1]);
-    T S[3][3] = {{S00, S01, S02}, {S01, S11, S12}, {S02, S12, S22}};
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      T px = S[i][0] * sx, py = S[i][1] * sy, pz = S[i][2] * sz;
-      dX[i] += px;
-      dY[i] += py;
-      dZ[i] += pz;
-      dXY[i] += px * Y + py * X;
-      dXZ[i] += px * Z + pz * X;
-      dYZ[i] += py * Z + pz * Y;
-      dXYZ[i] += px * (Y * Z) + py * (X * Z) + pz * (X * Y);
-    }
+  for (int gz = 0; gz < 3; gz++) {
+    const T Z = (T)gpt(gz);
+    T GX[3][3], GY[3][3], PX[3][3], PY[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const T K = (T)gpt(k);  // Y for x-lines, X for y-lines
+        GX[i][k] = sx * (cX[i] + cXY[i] * K + cXZ[i] * Z + cXYZ[i] * (K * Z));
+        GY[i][k] = sy * (cY[i] + cXY[i] * K + cYZ[i] * Z + cXYZ[i] * (K * Z));
+        PX[i][k] = (T)0;
+        PY[i][k] = (T)0;
+      }
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++)
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) {
+        const T w = W[(long)(gx + 3 * gy + 9 * gz) * E + e];
+        const T w2 = w + w;
+        const int qz = gx + 3 * gy;
+        const T G00 = GX[0][gy], G10 = GX[1][gy], G20 = GX[2][gy];
+        const T G01 = GY[0][gx], G11 = GY[1][gx], G21 = GY[2][gx];
+        const T G02 = GZ[0][qz], G12 = GZ[1][qz], G22 = GZ[2][qz];
+        const T S01 = w * (G01 + G10), S02 = w * (G02 + G20), S12 = w * (G12 + G21);
+        PX[0][gy] += w2 * G00;
+        PX[1][gy] += S01;
+        PX[2][gy] += S02;
+        PY[0][gx] += S01;
+        PY[1][gx] += w2 * G11;
+        PY[2][gx] += S12;
+        PZ[0][qz] += S02;
+        PZ[1][qz] += S12;
+        PZ[2][qz] += w2 * G22;
+      }
+    // fold the x- and y-lines of this plane into the dual coefficients
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const T K = (T)gpt(k);
+        const T px = PX[i][k] * sx, py = PY[i][k] * sy;
+        dX[i] += px;
+        dXY[i] += px * K;
+        dXZ[i] += px * Z;
+        dXYZ[i] += px * (K * Z);
+        dY[i] += py;
+        dXY[i] += py * K;
+        dYZ[i] += py * Z;
+        dXYZ[i] += py * (K * Z);
+      }
   }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const T X = (T)gpt(q % 3), Y = (T)gpt(q / 3);
+      const T pz = PZ[i][q] * sz;
+      dZ[i] += pz;
+      dXZ[i] += pz * X;
+      dYZ[i] += pz * Y;
+      dXYZ[i] += pz * (X * Y);
+    }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     // r_a = (1/8) (s_a dX + t_a dY + r_a dZ + s t dXY + s r dXZ + t r dYZ + s t r dXYZ)
@@ -605,7 +654,10 @@ void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* 
   A.nx = g.res[0] + 1;
   A.nxy = A.nx * (g.res[1] + 1);
   if (g.dim == 3) {
+    // shorter z-chunks on small grids so the launch still covers the 256 CUs
+    const long tiles = (long)((g.res[0] + 1 + 14) / 15) * ((g.res[1] + 1 + 14) / 15);
     A.KZ = g_kz3;
+    while (A.KZ > 2 && tiles * ((g.res[2] + A.KZ) / A.KZ) < 1024) A.KZ /= 2;
     dim3 grid((g.res[0] + 1 + 14) / 15, (g.res[1] + 1 + 14) / 15, (g.res[2] + 1 + A.KZ - 1) / A.KZ);
     hipLaunchKernelGGL(k_apply3<T>, grid, 256, 0, s, A, W, x, y, bc);
   } else {
@@ -626,14 +678,14 @@ struct GeConst {
   double v[24];
 };
 
-template <int DIM>
-__global__ void k_apply_G(int rx, int ry, int rz, long N, GeConst Ge, const double* __restrict__ p,
-                          const uint8_t* __restrict__ bc, double* __restrict__ y) {
+template <typename T, int DIM>
+__global__ void k_apply_G(int rx, int ry, int rz, long N, GeConst Ge, const T* __restrict__ p,
+                          const uint8_t* __restrict__ bc, T* __restrict__ y) {
   long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   long nx = rx + 1, ny = ry + 1;
   int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / (nx * ny));
-  double acc[3] = {0.0, 0.0, 0.0};
+  T acc[3] = {0, 0, 0};
   for (int oz = 0; oz < (DIM == 3 ? 2 : 1); oz++)
     for (int oy = 0; oy < 2; oy++)
       for (int ox = 0; ox < 2; ox++) {
@@ -641,50 +693,56 @@ __global__ void k_apply_G(int rx, int ry, int rz, long N, GeConst Ge, const doub
         if (ei < 0 || ei >= rx || ej < 0 || ej >= ry || ek < 0 || ek >= rz) continue;
         long e = ei + (long)rx * (ej + (long)ry * ek);
         int a = (1 - ox) + 2 * (1 - oy) + (DIM == 3 ? 4 * (1 - oz) : 0);
-        double pe = p[e];
-        for (int d = 0; d < DIM; d++) acc[d] += Ge.v[a * DIM + d] * pe;
+        T pe = p[e];
+        for (int d = 0; d < DIM; d++) acc[d] += (T)Ge.v[a * DIM + d] * pe;
       }
   uint8_t m = bc[n];
-  for (int d = 0; d < DIM; d++) y[n * DIM + d] = ((m >> d) & 1) ? 0.0 : acc[d];
+  for (int d = 0; d < DIM; d++) y[n * DIM + d] = ((m >> d) & 1) ? (T)0 : acc[d];
 }
 
-template <int DIM>
-__global__ void k_apply_GT(int rx, int ry, int rz, long E, GeConst Ge, const double* __restrict__ u,
-                           const uint8_t* __restrict__ bc, double* __restrict__ q) {
+template <typename T, int DIM>
+__global__ void k_apply_GT(int rx, int ry, int rz, long E, GeConst Ge, const T* __restrict__ u,
+                           const uint8_t* __restrict__ bc, T* __restrict__ q) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   long nx = rx + 1, ny = ry + 1;
   int i = (int)(e % rx), j = (int)((e / rx) % ry), k = DIM == 3 ? (int)(e / ((long)rx * ry)) : 0;
   long base = i + nx * (j + ny * k);
-  double acc = 0.0;
+  T acc = 0;
   for (int a = 0; a < (DIM == 3 ? 8 : 4); a++) {
     long n = base + (a & 1) + ((a >> 1) & 1) * nx + (DIM == 3 ? ((a >> 2) & 1) * nx * ny : 0);
     uint8_t m = bc[n];
     for (int d = 0; d < DIM; d++)
-      if (!((m >> d) & 1)) acc += Ge.v[a * DIM + d] * u[n * DIM + d];
+      if (!((m >> d) & 1)) acc += (T)Ge.v[a * DIM + d] * u[n * DIM + d];
   }
   q[e] = acc;
 }
 
-void launch_apply_G(const Geom& g, const double* Ge, const double* p, const uint8_t* bc, double* y, hipStream_t s) {
+template <typename T>
+void launch_apply_G(const Geom& g, const double* Ge, const T* p, const uint8_t* bc, T* y, hipStream_t s) {
   GeConst G;
   for (int q = 0; q < 24; q++) G.v[q] = q < (1 << g.dim) * g.dim ? Ge[q] : 0.0;
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_apply_G<3>, nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.N, G, p, bc, y);
+    hipLaunchKernelGGL((k_apply_G<T, 3>), nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.N, G, p, bc, y);
   else
-    hipLaunchKernelGGL(k_apply_G<2>, nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], 1, g.N, G, p, bc, y);
+    hipLaunchKernelGGL((k_apply_G<T, 2>), nblocks(g.N, 256), 256, 0, s, g.res[0], g.res[1], 1, g.N, G, p, bc, y);
   UWG_HIP(hipGetLastError());
 }
 
-void launch_apply_GT(const Geom& g, const double* Ge, const double* u, const uint8_t* bc, double* q, hipStream_t s) {
+template <typename T>
+void launch_apply_GT(const Geom& g, const double* Ge, const T* u, const uint8_t* bc, T* q, hipStream_t s) {
   GeConst G;
   for (int k = 0; k < 24; k++) G.v[k] = k < (1 << g.dim) * g.dim ? Ge[k] : 0.0;
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_apply_GT<3>, nblocks(g.E, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.E, G, u, bc, q);
+    hipLaunchKernelGGL((k_apply_GT<T, 3>), nblocks(g.E, 256), 256, 0, s, g.res[0], g.res[1], g.res[2], g.E, G, u, bc, q);
   else
-    hipLaunchKernelGGL(k_apply_GT<2>, nblocks(g.E, 256), 256, 0, s, g.res[0], g.res[1], 1, g.E, G, u, bc, q);
+    hipLaunchKernelGGL((k_apply_GT<T, 2>), nblocks(g.E, 256), 256, 0, s, g.res[0], g.res[1], 1, g.E, G, u, bc, q);
   UWG_HIP(hipGetLastError());
 }
+template void launch_apply_G<double>(const Geom&, const double*, const double*, const uint8_t*, double*, hipStream_t);
+template void launch_apply_G<float>(const Geom&, const double*, const float*, const uint8_t*, float*, hipStream_t);
+template void launch_apply_GT<double>(const Geom&, const double*, const double*, const uint8_t*, double*, hipStream_t);
+template void launch_apply_GT<float>(const Geom&, const double*, const float*, const uint8_t*, float*, hipStream_t);
 
 // ---------------------------------------------------------------------------------------
 // Transfer operators: trilinear prolongation (a fine node averages its 1, 2, 4 or 8
@@ -902,34 +960,39 @@ void vk_f2d(long n, double scale, const float* x, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_f2d, vgrid(n), 256, 0, s, n, scale, x, y);
 }
 
-// y (+)= sum_j c[j] X[j]; c and X live in device memory (k <= 64)
-__global__ void k_multi_axpy(long n, int k, const double* const* X, const double* c, double* y) {
+// y (+)= sum_j c[j] X[j]; c and X live in device memory; accumulation in double
+template <typename T>
+__global__ void k_multi_axpy(long n, int k, const T* const* X, const double* c, T* y) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double acc = y[i];
-    for (int j = 0; j < k; j++) acc += c[j] * X[j][i];
-    y[i] = acc;
+    double acc = (double)y[i];
+    for (int j = 0; j < k; j++) acc += c[j] * (double)X[j][i];
+    y[i] = (T)acc;
   }
 }
-void vk_multi_axpy(long n, int k, const double* const* X, const double* c, double* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_multi_axpy, vgrid(n), 256, 0, s, n, k, X, c, y);
+template <typename T>
+void vk_multi_axpy(long n, int k, const T* const* X, const double* c, T* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_multi_axpy<T>, vgrid(n), 256, 0, s, n, k, X, c, y);
 }
+template void vk_multi_axpy<double>(long, int, const double* const*, const double*, double*, hipStream_t);
+template void vk_multi_axpy<float>(long, int, const float* const*, const double*, float*, hipStream_t);
 
 // deterministic multi-dot: fixed grid of RB blocks, fixed-order tree in each block, then
-// one block sums the RB partials in order.
+// one block sums the RB partials in order.  Products and sums in double for either input
+// precision (FP32 Krylov bases keep FP64 reductions).
 #define RB 1024
 int vk_reduce_blocks() { return RB; }
-template <int KM>
-__global__ __launch_bounds__(256) void k_mdot(long n, int k, const double* __restrict__ x, const double* const* Y,
+template <int KM, typename T>
+__global__ __launch_bounds__(256) void k_mdot(long n, int k, const T* __restrict__ x, const T* const* Y,
                                               double* partial) {
   __shared__ double sm[KM][256];
   double acc[KM];
 #pragma unroll
   for (int j = 0; j < KM; j++) acc[j] = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double xv = x[i];
+    double xv = (double)x[i];
 #pragma unroll
     for (int j = 0; j < KM; j++)
-      if (j < k) acc[j] += xv * Y[j][i];
+      if (j < k) acc[j] += xv * (double)Y[j][i];
   }
 #pragma unroll
   for (int j = 0; j < KM; j++) sm[j][threadIdx.x] = acc[j];
@@ -957,27 +1020,36 @@ __global__ void k_mdot_final(int k, const double* partial, double* out) {
     __syncthreads();
   }
 }
-void vk_multi_dot(long n, int k, const double* x, const double* const* Y, double* partial, double* out, hipStream_t s) {
+template <typename T>
+void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s) {
   if (k <= 1)
-    hipLaunchKernelGGL(k_mdot<1>, RB, 256, 0, s, n, k, x, Y, partial);
+    hipLaunchKernelGGL((k_mdot<1, T>), RB, 256, 0, s, n, k, x, Y, partial);
   else if (k <= 4)
-    hipLaunchKernelGGL(k_mdot<4>, RB, 256, 0, s, n, k, x, Y, partial);
+    hipLaunchKernelGGL((k_mdot<4, T>), RB, 256, 0, s, n, k, x, Y, partial);
   else
-    hipLaunchKernelGGL(k_mdot<8>, RB, 256, 0, s, n, k, x, Y, partial);
+    hipLaunchKernelGGL((k_mdot<8, T>), RB, 256, 0, s, n, k, x, Y, partial);
   hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, k, partial, out);
   UWG_HIP(hipGetLastError());
 }
+template void vk_multi_dot<double>(long, int, const double*, const double* const*, double*, double*, hipStream_t);
+template void vk_multi_dot<float>(long, int, const float*, const float* const*, double*, double*, hipStream_t);
 
-// dense y = A x (A symmetric, n x n, column-major)
-__global__ void k_dense_mv(int n, const double* A, const double* x, double* y) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+// dense y = A x (A symmetric, n x n): one block per row, coalesced along the row
+__global__ __launch_bounds__(256) void k_dense_mv(int n, const double* A, const double* x, double* y) {
+  __shared__ double sm[256];
+  const int r = blockIdx.x;
   double acc = 0.0;
-  for (int c = 0; c < n; c++) acc += A[(long)c * n + r] * x[c];
-  y[r] = acc;
+  for (int c = threadIdx.x; c < n; c += 256) acc += A[(long)r * n + c] * x[c];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[r] = sm[0];
 }
 void launch_dense_mv(int n, const double* A, const double* x, double* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_dense_mv, (n + 127) / 128, 128, 0, s, n, A, x, y);
+  hipLaunchKernelGGL(k_dense_mv, n, 256, 0, s, n, A, x, y);
   UWG_HIP(hipGetLastError());
 }
 

@@ -1,9 +1,744 @@
-// placeholder, replaced by the learned coarse-correction kernels
-#include "uwg_nn.h"
+// Learned coarse-grid correction: a 4-level convolutional U-net on the nodal grid of one
+// multigrid level, evaluated and trained with the kernels in this file (FP32 storage and
+// arithmetic, deterministic reductions).
+//
+// Input channels per node: the residual (dim components, scaled by 1/rms) and a log-viscosity
+// feature.  Output: the correction (dim components, rescaled by rms), zero on Dirichlet dofs.
+//
+// Network (levels l = 0..3 on node grids n_l, n_{l+1} = (n_l + 1) / 2):
+//   e_0 = relu(conv(relu(conv(x, Cin -> C))))            ; level 0
+//   e_l = relu(conv(relu(conv(R e_{l-1} / 8))))           ; l = 1..3 (R: full weighting)
+//   d_3 = e_3 ; d_l = relu(conv(P d_{l+1} + e_l))         ; l = 2..0 (P: trilinear)
+//   y   = conv(d_0, C -> dim)
+// R is the transpose of P, so the backward pass of a pooling is the other operator.
+//
+// Every convolution is 3x3x3 with zero padding; activations are [B][C][N] (N = nx*ny*nz,
+// x fastest).  Grids here are at most 33^3 nodes, where direct convolution with weights
+// broadcast from the scalar cache is latency-bound rather than FLOP-bound, so MFMA tiles
+// would not pay.
+#include <cmath>
+#include <cstring>
 #include <stdexcept>
+#include <vector>
+
+#include "uwg_nn.h"
+
 namespace uwg {
-struct NNModel { int dummy; };
-NNModel* nn_create(const Geom&, int, int, const float*, int) { throw std::runtime_error("learned model not built"); }
-void nn_destroy(NNModel* m) { delete m; }
-void nn_coarse_correction(NNModel*, const Geom&, const float*, float*, const uint8_t*, hipStream_t) {}
+
+static inline unsigned nbk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Grid3 {
+  int nx, ny, nz;
+  __host__ __device__ long N() const { return (long)nx * ny * nz; }
+};
+static Grid3 coarser(Grid3 g) { return Grid3{(g.nx + 1) / 2, (g.ny + 1) / 2, (g.nz + 1) / 2}; }
+
+// ------------------------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------------------------
+__global__ void k_conv_fwd(int B, int Ci, int Co, Grid3 g, const float* __restrict__ in, const float* __restrict__ w,
+                           const float* __restrict__ bias, float* __restrict__ out, int relu) {
+  const long N = g.N();
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * Co * N) return;
+  const long p = t % N;
+  const int co = (int)((t / N) % Co);
+  const int b = (int)(t / (N * Co));
+  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
+  float acc = bias[co];
+  const float* wb = w + (long)co * Ci * 27;
+  for (int ci = 0; ci < Ci; ci++) {
+    const float* src = in + ((long)b * Ci + ci) * N;
+    const float* wc = wb + ci * 27;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+      int kk = k + dz;
+      if (kk < 0 || kk >= g.nz) continue;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        int jj = j + dy;
+        if (jj < 0 || jj >= g.ny) continue;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          int ii = i + dx;
+          if (ii < 0 || ii >= g.nx) continue;
+          acc += wc[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] * src[ii + (long)g.nx * (jj + (long)g.ny * kk)];
+        }
+      }
+    }
+  }
+  out[t] = relu ? fmaxf(acc, 0.0f) : acc;
+}
+
+// din[b][ci][p] = sum_co sum_tap w[co][ci][tap] dout[b][co][p - off(tap)]
+__global__ void k_conv_bwd_data(int B, int Ci, int Co, Grid3 g, const float* __restrict__ dout,
+                                const float* __restrict__ w, float* __restrict__ din) {
+  const long N = g.N();
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * Ci * N) return;
+  const long p = t % N;
+  const int ci = (int)((t / N) % Ci);
+  const int b = (int)(t / (N * Ci));
+  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
+  float acc = 0.0f;
+  for (int co = 0; co < Co; co++) {
+    const float* src = dout + ((long)b * Co + co) * N;
+    const float* wc = w + ((long)co * Ci + ci) * 27;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+      int kk = k - dz;
+      if (kk < 0 || kk >= g.nz) continue;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        int jj = j - dy;
+        if (jj < 0 || jj >= g.ny) continue;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          int ii = i - dx;
+          if (ii < 0 || ii >= g.nx) continue;
+          acc += wc[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] * src[ii + (long)g.nx * (jj + (long)g.ny * kk)];
+        }
+      }
+    }
+  }
+  din[t] = acc;
+}
+
+// dw[co][ci][tap] = sum_b sum_p dout[b][co][p] in[b][ci][p + off(tap)] ; one block per (co, ci)
+__global__ __launch_bounds__(256) void k_conv_bwd_w(int B, int Ci, int Co, Grid3 g, const float* __restrict__ dout,
+                                                    const float* __restrict__ in, float* __restrict__ dw) {
+  __shared__ float sm[27][256];
+  const int co = blockIdx.x / Ci, ci = blockIdx.x % Ci;
+  const long N = g.N();
+  float acc[27];
+#pragma unroll
+  for (int q = 0; q < 27; q++) acc[q] = 0.0f;
+  for (long t = threadIdx.x; t < (long)B * N; t += 256) {
+    const int b = (int)(t / N);
+    const long p = t % N;
+    const float go = dout[((long)b * Co + co) * N + p];
+    if (go == 0.0f) continue;
+    const float* src = in + ((long)b * Ci + ci) * N;
+    const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+      int kk = k + dz;
+      bool okz = kk >= 0 && kk < g.nz;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        int jj = j + dy;
+        bool oky = okz && jj >= 0 && jj < g.ny;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          int ii = i + dx;
+          if (oky && ii >= 0 && ii < g.nx)
+            acc[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] += go * src[ii + (long)g.nx * (jj + (long)g.ny * kk)];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 27; q++) sm[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int q = 0; q < 27; q++) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 27) dw[((long)co * Ci + ci) * 27 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// db[co] = sum_b sum_p dout[b][co][p]
+__global__ __launch_bounds__(256) void k_bias_grad(int B, int Co, long N, const float* __restrict__ dout,
+                                                   float* __restrict__ db) {
+  __shared__ float sm[256];
+  const int co = blockIdx.x;
+  float a = 0.0f;
+  for (long t = threadIdx.x; t < (long)B * N; t += 256) a += dout[((long)(t / N) * Co + co) * N + t % N];
+  sm[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[co] = sm[0];
+}
+
+// dy *= (y > 0)
+__global__ void k_relu_bwd(long n, const float* __restrict__ y, float* __restrict__ dy) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && !(y[t] > 0.0f)) dy[t] = 0.0f;
+}
+
+// out[B*C][coarse] = s * R in[B*C][fine]   (full weighting, weights 1, 1/2 per offset)
+__global__ void k_restrict_nodes(long BC, Grid3 f, Grid3 c, float s, const float* __restrict__ in,
+                                 float* __restrict__ out) {
+  const long Nc = c.N(), Nf = f.N();
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= BC * Nc) return;
+  const long bc = t / Nc, p = t % Nc;
+  const int i = (int)(p % c.nx), j = (int)((p / c.nx) % c.ny), k = (int)(p / ((long)c.nx * c.ny));
+  const float* src = in + bc * Nf;
+  float acc = 0.0f;
+  for (int oz = -1; oz <= 1; oz++) {
+    int K = 2 * k + oz;
+    if (K < 0 || K >= f.nz) continue;
+    for (int oy = -1; oy <= 1; oy++) {
+      int J = 2 * j + oy;
+      if (J < 0 || J >= f.ny) continue;
+      for (int ox = -1; ox <= 1; ox++) {
+        int I = 2 * i + ox;
+        if (I < 0 || I >= f.nx) continue;
+        float w = (ox ? 0.5f : 1.0f) * (oy ? 0.5f : 1.0f) * (oz ? 0.5f : 1.0f);
+        acc += w * src[I + (long)f.nx * (J + (long)f.ny * K)];
+      }
+    }
+  }
+  out[t] = s * acc;
+}
+
+// out[B*C][fine] (+)= s * P in[B*C][coarse]
+__global__ void k_prolong_nodes(long BC, Grid3 f, Grid3 c, float s, const float* __restrict__ in,
+                                float* __restrict__ out, int accumulate) {
+  const long Nc = c.N(), Nf = f.N();
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= BC * Nf) return;
+  const long bc = t / Nf, q = t % Nf;
+  const int I = (int)(q % f.nx), J = (int)((q / f.nx) % f.ny), K = (int)(q / ((long)f.nx * f.ny));
+  const float* src = in + bc * Nc;
+  float acc = 0.0f;
+  const int ni = (I & 1) ? 2 : 1, nj = (J & 1) ? 2 : 1, nk = (K & 1) ? 2 : 1;
+  const float w = ((I & 1) ? 0.5f : 1.0f) * ((J & 1) ? 0.5f : 1.0f) * ((K & 1) ? 0.5f : 1.0f);
+  for (int c3 = 0; c3 < nk; c3++)
+    for (int b2 = 0; b2 < nj; b2++)
+      for (int a1 = 0; a1 < ni; a1++) {
+        int ci = (I >> 1) + a1, cj = (J >> 1) + b2, ck = (K >> 1) + c3;
+        if (ci < c.nx && cj < c.ny && ck < c.nz) acc += src[ci + (long)c.nx * (cj + (long)c.ny * ck)];
+      }
+  out[t] = (accumulate ? out[t] : 0.0f) + s * w * acc;
+}
+
+__global__ void k_add_inplace(long n, const float* __restrict__ a, float* __restrict__ y) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] += a[t];
+}
+
+// loss_b = sum m (o - e)^2 / (sum m e^2 + tiny); grad = 2 m (o - e) / den / B.  One block per sample.
+__global__ __launch_bounds__(256) void k_loss(int B, int Cout, long N, const float* __restrict__ o,
+                                              const float* __restrict__ e, const float* __restrict__ m,
+                                              float* __restrict__ grad, float* __restrict__ loss_b) {
+  __shared__ float s1[256], s2[256];
+  const int b = blockIdx.x;
+  const long n = (long)Cout * N;
+  const float* ob = o + (long)b * n;
+  const float* eb = e + (long)b * n;
+  const float* mb = m + (long)b * n;
+  float a1 = 0.0f, a2 = 0.0f;
+  for (long t = threadIdx.x; t < n; t += 256) {
+    float d = ob[t] - eb[t];
+    a1 += mb[t] * d * d;
+    a2 += mb[t] * eb[t] * eb[t];
+  }
+  s1[threadIdx.x] = a1;
+  s2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  const float den = s2[0] + 1e-30f;
+  if (threadIdx.x == 0) loss_b[b] = s1[0] / den;
+  if (grad)
+    for (long t = threadIdx.x; t < n; t += 256) grad[(long)b * n + t] = 2.0f * mb[t] * (ob[t] - eb[t]) / den / B;
+}
+
+__global__ void k_adam(long n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2, float wd) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float gg = g[t] + wd * p[t];
+  float a = b1 * m1[t] + (1.0f - b1) * gg;
+  float v = b2 * m2[t] + (1.0f - b2) * gg * gg;
+  m1[t] = a;
+  m2[t] = v;
+  p[t] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+// inference helpers --------------------------------------------------------------------
+// rms of the masked residual (single block, deterministic) -> scale[0] = rms, scale[1] = 1/rms
+__global__ __launch_bounds__(256) void k_rms(long n, const float* __restrict__ r, float* __restrict__ scale) {
+  __shared__ float sm[256];
+  float a = 0.0f;
+  for (long t = threadIdx.x; t < n; t += 256) a += r[t] * r[t];
+  sm[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float rms = sqrtf(sm[0] / (float)n);
+    scale[0] = rms;
+    scale[1] = rms > 0.0f ? 1.0f / rms : 0.0f;
+  }
+}
+// interleaved [N][dim] residual -> channels 0..dim-1 of the input (scaled); channel dim = feature
+__global__ void k_pack_input(long N, int dim, const float* __restrict__ r, const float* __restrict__ feat,
+                             const float* __restrict__ scale, float* __restrict__ x) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const float s = scale[1];
+  for (int d = 0; d < dim; d++) x[(long)d * N + t] = s * r[t * dim + d];
+  x[(long)dim * N + t] = feat[t];
+}
+__global__ void k_unpack_output(long N, int dim, const float* __restrict__ y, const float* __restrict__ scale,
+                                const uint8_t* __restrict__ bc, float* __restrict__ out) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const float s = scale[0];
+  const uint8_t m = bc[t];
+  for (int d = 0; d < dim; d++) out[t * dim + d] = ((m >> d) & 1) ? 0.0f : s * y[(long)d * N + t];
+}
+
+// ------------------------------------------------------------------------------------------
+// the network
+// ------------------------------------------------------------------------------------------
+struct ConvSpec {
+  int ci, co;
+  long woff, boff;  // offsets into the flat parameter vector
+};
+
+// conv index layout for depth D: encoder 2l, 2l+1 (l = 0..D-1); decoder cd(l) = 2D + (D-2-l)
+// (l = D-2..0); head 3D-1.
+struct NNModel {
+  int C = 32, depth = 4, cin = 4, cout = 3;
+  std::vector<Grid3> grid;
+  std::vector<ConvSpec> conv;
+  long nparams = 0;
+  float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;
+  long step = 0;
+  int bmax = 0;
+  std::vector<float*> act, gact;    // per conv output
+  std::vector<float*> pool, gpool;  // per level (l >= 1): R(e_{l-1})/8
+  std::vector<float*> sum, gsum;    // per level (l <= D-2): P(up) + e_l
+  float *xin = nullptr, *y = nullptr, *gy = nullptr, *lossb = nullptr;
+  float *scale = nullptr, *feat = nullptr;
+  int cd(int l) const { return 2 * depth + (depth - 2 - l); }
+  int head() const { return 3 * depth - 1; }
+  int level_of(int c) const {
+    if (c < 2 * depth) return c / 2;
+    if (c < 3 * depth - 1) return depth - 2 - (c - 2 * depth);
+    return 0;
+  }
+};
+
+template <typename T>
+static T* nalloc(long n) {
+  T* p = nullptr;
+  UWG_HIP(hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)));
+  return p;
+}
+
+static void build_specs(NNModel* m) {
+  m->conv.clear();
+  long off = 0;
+  auto add = [&](int ci, int co) {
+    ConvSpec cs{ci, co, off, 0};
+    off += (long)ci * co * 27;
+    cs.boff = off;
+    off += co;
+    m->conv.push_back(cs);
+  };
+  for (int l = 0; l < m->depth; l++) {
+    add(l == 0 ? m->cin : m->C, m->C);
+    add(m->C, m->C);
+  }
+  for (int l = m->depth - 2; l >= 0; l--) add(m->C, m->C);
+  add(m->C, m->cout);
+  m->nparams = off;
+}
+
+static void free_batch(NNModel* m) {
+  for (auto v : {&m->act, &m->gact, &m->pool, &m->gpool, &m->sum, &m->gsum})
+    for (auto p : *v) (void)hipFree(p);
+  for (auto v : {&m->act, &m->gact, &m->pool, &m->gpool, &m->sum, &m->gsum}) v->clear();
+  for (void* p : {(void*)m->xin, (void*)m->y, (void*)m->gy, (void*)m->lossb}) (void)hipFree(p);
+  m->xin = m->y = m->gy = m->lossb = nullptr;
+  m->bmax = 0;
+}
+
+static void ensure_batch(NNModel* m, int B) {
+  if (B <= m->bmax) return;
+  free_batch(m);
+  const int D = m->depth;
+  for (size_t c = 0; c < m->conv.size(); c++) {
+    long n = (long)B * m->conv[c].co * m->grid[m->level_of((int)c)].N();
+    m->act.push_back(nalloc<float>(n));
+    m->gact.push_back(nalloc<float>(n));
+  }
+  for (int l = 0; l < D; l++) {
+    long n = (long)B * m->C * m->grid[l].N();
+    m->pool.push_back(l >= 1 ? nalloc<float>(n) : nullptr);
+    m->gpool.push_back(l >= 1 ? nalloc<float>(n) : nullptr);
+    m->sum.push_back(l <= D - 2 ? nalloc<float>(n) : nullptr);
+    m->gsum.push_back(l <= D - 2 ? nalloc<float>(n) : nullptr);
+  }
+  m->xin = nalloc<float>((long)B * m->cin * m->grid[0].N());
+  m->y = nalloc<float>((long)B * m->cout * m->grid[0].N());
+  m->gy = nalloc<float>((long)B * m->cout * m->grid[0].N());
+  m->lossb = nalloc<float>(B);
+  m->bmax = B;
+}
+
+static void conv_fwd(NNModel* m, int c, int B, const float* in, float* out, bool relu, hipStream_t s) {
+  const ConvSpec& cs = m->conv[c];
+  Grid3 g = m->grid[m->level_of(c)];
+  long n = (long)B * cs.co * g.N();
+  hipLaunchKernelGGL(k_conv_fwd, nbk(n, 256), 256, 0, s, B, cs.ci, cs.co, g, in, m->params + cs.woff,
+                     m->params + cs.boff, out, relu ? 1 : 0);
+}
+
+// forward for a batch packed in m->xin; output in m->y
+static void nn_forward(NNModel* m, int B, hipStream_t s) {
+  const int D = m->depth, C = m->C;
+  for (int l = 0; l < D; l++) {
+    const float* in = m->xin;
+    if (l > 0) {
+      Grid3 g = m->grid[l];
+      hipLaunchKernelGGL(k_restrict_nodes, nbk((long)B * C * g.N(), 256), 256, 0, s, (long)B * C, m->grid[l - 1], g,
+                         0.125f, m->act[2 * l - 1], m->pool[l]);
+      in = m->pool[l];
+    }
+    conv_fwd(m, 2 * l, B, in, m->act[2 * l], true, s);
+    conv_fwd(m, 2 * l + 1, B, m->act[2 * l], m->act[2 * l + 1], true, s);
+  }
+  const float* up = m->act[2 * D - 1];
+  for (int l = D - 2; l >= 0; l--) {
+    Grid3 g = m->grid[l];
+    long n = (long)B * C * g.N();
+    hipLaunchKernelGGL(k_prolong_nodes, nbk(n, 256), 256, 0, s, (long)B * C, g, m->grid[l + 1], 1.0f, up, m->sum[l], 0);
+    hipLaunchKernelGGL(k_add_inplace, nbk(n, 256), 256, 0, s, n, m->act[2 * l + 1], m->sum[l]);
+    conv_fwd(m, m->cd(l), B, m->sum[l], m->act[m->cd(l)], true, s);
+    up = m->act[m->cd(l)];
+  }
+  conv_fwd(m, m->head(), B, up, m->y, false, s);
+  UWG_HIP(hipGetLastError());
+}
+
+// conv backward: relu mask (if outact), weight and bias gradients, input gradient (if gin)
+static void conv_bwd(NNModel* m, int c, int B, const float* in, float* gout, float* gin, const float* outact,
+                     hipStream_t s) {
+  const ConvSpec& cs = m->conv[c];
+  Grid3 g = m->grid[m->level_of(c)];
+  if (outact) {
+    long n = (long)B * cs.co * g.N();
+    hipLaunchKernelGGL(k_relu_bwd, nbk(n, 256), 256, 0, s, n, outact, gout);
+  }
+  hipLaunchKernelGGL(k_conv_bwd_w, cs.co * cs.ci, 256, 0, s, B, cs.ci, cs.co, g, gout, in, m->grads + cs.woff);
+  hipLaunchKernelGGL(k_bias_grad, cs.co, 256, 0, s, B, cs.co, g.N(), gout, m->grads + cs.boff);
+  if (gin) {
+    long n = (long)B * cs.ci * g.N();
+    hipLaunchKernelGGL(k_conv_bwd_data, nbk(n, 256), 256, 0, s, B, cs.ci, cs.co, g, gout, m->params + cs.woff, gin);
+  }
+}
+
+// backward from m->gy (dL/dy); parameter gradients written into m->grads
+static void nn_backward(NNModel* m, int B, hipStream_t s) {
+  const int D = m->depth, C = m->C;
+  // head
+  conv_bwd(m, m->head(), B, m->act[m->cd(0)], m->gy, m->gact[m->cd(0)], nullptr, s);
+  // decoder, shallow to deep
+  for (int l = 0; l <= D - 2; l++) {
+    const int c = m->cd(l);
+    conv_bwd(m, c, B, m->sum[l], m->gact[c], m->gsum[l], m->act[c], s);
+    // sum_l = P(up) + e_l : g_up = R g_sum  (up = next decoder output, or the encoder top)
+    const int cup = (l == D - 2) ? 2 * D - 1 : m->cd(l + 1);
+    Grid3 gc = m->grid[l + 1];
+    hipLaunchKernelGGL(k_restrict_nodes, nbk((long)B * C * gc.N(), 256), 256, 0, s, (long)B * C, m->grid[l], gc, 1.0f,
+                       m->gsum[l], m->gact[cup]);
+  }
+  // encoder, deep to shallow
+  for (int l = D - 1; l >= 0; l--) {
+    Grid3 g = m->grid[l];
+    const long n = (long)B * C * g.N();
+    const int c1 = 2 * l + 1, c0 = 2 * l;
+    if (l <= D - 2) {
+      // g_e_l = g_sum_l (skip) + P(g_pool_{l+1}) / 8 (pooling into the next level)
+      hipLaunchKernelGGL(k_prolong_nodes, nbk(n, 256), 256, 0, s, (long)B * C, g, m->grid[l + 1], 0.125f,
+                         m->gpool[l + 1], m->gact[c1], 0);
+      hipLaunchKernelGGL(k_add_inplace, nbk(n, 256), 256, 0, s, n, m->gsum[l], m->gact[c1]);
+    }
+    conv_bwd(m, c1, B, m->act[c0], m->gact[c1], m->gact[c0], m->act[c1], s);
+    conv_bwd(m, c0, B, l == 0 ? m->xin : m->pool[l], m->gact[c0], l == 0 ? nullptr : m->gpool[l], m->act[c0], s);
+  }
+  UWG_HIP(hipGetLastError());
+}
+
+static NNModel* nn_alloc(Grid3 g0, int channels, int depth, int cin, int cout) {
+  NNModel* m = new NNModel();
+  m->C = channels;
+  m->depth = depth;
+  m->cin = cin;
+  m->cout = cout;
+  m->grid.push_back(g0);
+  for (int l = 1; l < depth; l++) m->grid.push_back(coarser(m->grid.back()));
+  build_specs(m);
+  m->params = nalloc<float>(m->nparams);
+  m->grads = nalloc<float>(m->nparams);
+  m->m1 = nalloc<float>(m->nparams);
+  m->m2 = nalloc<float>(m->nparams);
+  UWG_HIP(hipMemset(m->m1, 0, sizeof(float) * m->nparams));
+  UWG_HIP(hipMemset(m->m2, 0, sizeof(float) * m->nparams));
+  m->scale = nalloc<float>(2);
+  m->feat = nalloc<float>(g0.N());
+  UWG_HIP(hipMemset(m->feat, 0, sizeof(float) * g0.N()));
+  return m;
+}
+
+void nn_destroy(NNModel* m) {
+  if (!m) return;
+  free_batch(m);
+  for (void* p : {(void*)m->params, (void*)m->grads, (void*)m->m1, (void*)m->m2, (void*)m->scale, (void*)m->feat})
+    (void)hipFree(p);
+  delete m;
+}
+
+NNModel* nn_create(const Geom& g, int channels, int depth, const float* params, int nparams) {
+  if (g.dim != 3) throw std::runtime_error("the learned correction is provided for 3D meshes");
+  if (depth < 2 || depth > 6 || channels < 1 || channels > 128) throw std::runtime_error("bad network shape");
+  NNModel* m = nn_alloc(Grid3{g.nn[0], g.nn[1], g.nn[2]}, channels, depth, 4, 3);
+  if (nparams != m->nparams) {
+    long want = m->nparams;
+    nn_destroy(m);
+    throw std::runtime_error("learned model has " + std::to_string(nparams) + " parameters, network needs " +
+                             std::to_string(want));
+  }
+  UWG_HIP(hipMemcpy(m->params, params, sizeof(float) * nparams, hipMemcpyHostToDevice));
+  ensure_batch(m, 1);
+  return m;
+}
+
+void nn_set_feature(NNModel* m, const float* feat_host) {
+  UWG_HIP(hipMemcpy(m->feat, feat_host, sizeof(float) * m->grid[0].N(), hipMemcpyHostToDevice));
+}
+
+void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s) {
+  const long N = m->grid[0].N();
+  hipLaunchKernelGGL(k_rms, 1, 256, 0, s, N * 3, r, m->scale);
+  hipLaunchKernelGGL(k_pack_input, nbk(N, 256), 256, 0, s, N, 3, r, m->feat, m->scale, m->xin);
+  nn_forward(m, 1, s);
+  hipLaunchKernelGGL(k_unpack_output, nbk(N, 256), 256, 0, s, N, 3, m->y, m->scale, bc, x);
+  UWG_HIP(hipGetLastError());
+  (void)g;
+}
+
+long nn_param_count(NNModel* m) { return m->nparams; }
+
+}  // namespace uwg
+
+// ------------------------------------------------------------------------------------------
+// C ABI for training and testing the network
+// ------------------------------------------------------------------------------------------
+#include "../../include/uwg.h"
+
+using namespace uwg;
+struct uwg_nn {
+  NNModel* m;
+  float* data_x = nullptr;
+  float* data_t = nullptr;
+  float* data_m = nullptr;
+  long nsamples = 0;
+};
+
+#define NN_API_BEGIN try {
+#define NN_API_END                      \
+  }                                     \
+  catch (const std::exception& ex) {    \
+    uwg::set_error(ex.what());          \
+    return UWG_ERR_DEVICE;              \
+  }
+
+extern "C" int uwg_nn_create(int nx, int ny, int nz, int channels, int depth, long seed, uwg_nn** out) {
+  if (nx < 3 || ny < 3 || nz < 3 || depth < 2 || depth > 6 || channels < 1 || channels > 128) {
+    uwg::set_error("bad network shape");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  uwg_nn* h = new uwg_nn();
+  h->m = nn_alloc(Grid3{nx, ny, nz}, channels, depth, 4, 3);
+  // He-normal initialisation from a 64-bit LCG (deterministic for a seed); small head
+  std::vector<float> p(h->m->nparams, 0.0f);
+  unsigned long long st = (unsigned long long)seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  auto unif = [&]() {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return ((st >> 11) + 0.5) / 9007199254740992.0;
+  };
+  for (size_t c = 0; c < h->m->conv.size(); c++) {
+    const ConvSpec& cs = h->m->conv[c];
+    double sd = sqrt(2.0 / (cs.ci * 27.0)) * (c + 1 == h->m->conv.size() ? 0.1 : 1.0);
+    for (long k = 0; k < (long)cs.ci * cs.co * 27; k++) {
+      double u1 = unif(), u2 = unif();
+      p[cs.woff + k] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+  }
+  UWG_HIP(hipMemcpy(h->m->params, p.data(), sizeof(float) * p.size(), hipMemcpyHostToDevice));
+  *out = h;
+  return 0;
+  NN_API_END
+}
+
+extern "C" int uwg_nn_destroy(uwg_nn* h) {
+  if (!h) return 0;
+  nn_destroy(h->m);
+  for (void* p : {(void*)h->data_x, (void*)h->data_t, (void*)h->data_m}) (void)hipFree(p);
+  delete h;
+  return 0;
+}
+
+extern "C" int uwg_nn_param_count(uwg_nn* h, long* n) {
+  *n = h->m->nparams;
+  return 0;
+}
+extern "C" int uwg_nn_get_params(uwg_nn* h, float* out) {
+  NN_API_BEGIN
+  UWG_HIP(hipMemcpy(out, h->m->params, sizeof(float) * h->m->nparams, hipMemcpyDeviceToHost));
+  return 0;
+  NN_API_END
+}
+extern "C" int uwg_nn_set_params(uwg_nn* h, const float* in) {
+  NN_API_BEGIN
+  UWG_HIP(hipMemcpy(h->m->params, in, sizeof(float) * h->m->nparams, hipMemcpyHostToDevice));
+  return 0;
+  NN_API_END
+}
+
+// y[B][3][N] = NN(x[B][4][N])
+extern "C" int uwg_nn_forward(uwg_nn* h, int B, const float* x, float* y) {
+  NN_API_BEGIN
+  NNModel* m = h->m;
+  ensure_batch(m, B);
+  const long N = m->grid[0].N();
+  UWG_HIP(hipMemcpy(m->xin, x, sizeof(float) * B * m->cin * N, hipMemcpyHostToDevice));
+  nn_forward(m, B, 0);
+  UWG_HIP(hipMemcpy(y, m->y, sizeof(float) * B * m->cout * N, hipMemcpyDeviceToHost));
+  return 0;
+  NN_API_END
+}
+
+// mean over the batch of |m (y - t)|^2 / |m t|^2 and its gradient w.r.t. the parameters
+extern "C" int uwg_nn_loss_grad(uwg_nn* h, int B, const float* x, const float* t, const float* mask, float* loss,
+                                float* grads) {
+  NN_API_BEGIN
+  NNModel* m = h->m;
+  ensure_batch(m, B);
+  const long N = m->grid[0].N();
+  float* dt = nalloc<float>((long)B * m->cout * N);
+  float* dm = nalloc<float>((long)B * m->cout * N);
+  UWG_HIP(hipMemcpy(m->xin, x, sizeof(float) * B * m->cin * N, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(dt, t, sizeof(float) * B * m->cout * N, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(dm, mask, sizeof(float) * B * m->cout * N, hipMemcpyHostToDevice));
+  nn_forward(m, B, 0);
+  hipLaunchKernelGGL(k_loss, B, 256, 0, 0, B, m->cout, N, m->y, dt, dm, m->gy, m->lossb);
+  nn_backward(m, B, 0);
+  std::vector<float> lb(B);
+  UWG_HIP(hipMemcpy(lb.data(), m->lossb, sizeof(float) * B, hipMemcpyDeviceToHost));
+  double acc = 0.0;
+  for (float v : lb) acc += v;
+  *loss = (float)(acc / B);
+  if (grads) UWG_HIP(hipMemcpy(grads, m->grads, sizeof(float) * m->nparams, hipMemcpyDeviceToHost));
+  (void)hipFree(dt);
+  (void)hipFree(dm);
+  return 0;
+  NN_API_END
+}
+
+// upload a training set: x[S][4][N], t[S][3][N], mask[S][3][N]
+extern "C" int uwg_nn_set_data(uwg_nn* h, long S, const float* x, const float* t, const float* mask) {
+  NN_API_BEGIN
+  NNModel* m = h->m;
+  const long N = m->grid[0].N();
+  for (void* p : {(void*)h->data_x, (void*)h->data_t, (void*)h->data_m}) (void)hipFree(p);
+  h->data_x = nalloc<float>(S * m->cin * N);
+  h->data_t = nalloc<float>(S * m->cout * N);
+  h->data_m = nalloc<float>(S * m->cout * N);
+  UWG_HIP(hipMemcpy(h->data_x, x, sizeof(float) * S * m->cin * N, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(h->data_t, t, sizeof(float) * S * m->cout * N, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(h->data_m, mask, sizeof(float) * S * m->cout * N, hipMemcpyHostToDevice));
+  h->nsamples = S;
+  return 0;
+  NN_API_END
+}
+
+// Adam on minibatches drawn (deterministically from seed) from the uploaded set.
+// losses[steps] receives each step's batch loss.
+extern "C" int uwg_nn_train(uwg_nn* h, int steps, int B, float lr, float weight_decay, long seed, float* losses) {
+  if (h->nsamples < 1) {
+    uwg::set_error("no training data");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  NNModel* m = h->m;
+  ensure_batch(m, B);
+  const long N = m->grid[0].N();
+  float* bt = nalloc<float>((long)B * m->cout * N);
+  float* bm = nalloc<float>((long)B * m->cout * N);
+  unsigned long long st = (unsigned long long)seed * 2862933555777941757ULL + 3037000493ULL;
+  std::vector<float> lb(B);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int it = 0; it < steps; it++) {
+    for (int b = 0; b < B; b++) {
+      st = st * 2862933555777941757ULL + 3037000493ULL;
+      long k = (long)((st >> 17) % (unsigned long long)h->nsamples);
+      UWG_HIP(hipMemcpyAsync(m->xin + (long)b * m->cin * N, h->data_x + k * m->cin * N, sizeof(float) * m->cin * N,
+                             hipMemcpyDeviceToDevice, 0));
+      UWG_HIP(hipMemcpyAsync(bt + (long)b * m->cout * N, h->data_t + k * m->cout * N, sizeof(float) * m->cout * N,
+                             hipMemcpyDeviceToDevice, 0));
+      UWG_HIP(hipMemcpyAsync(bm + (long)b * m->cout * N, h->data_m + k * m->cout * N, sizeof(float) * m->cout * N,
+                             hipMemcpyDeviceToDevice, 0));
+    }
+    nn_forward(m, B, 0);
+    hipLaunchKernelGGL(k_loss, B, 256, 0, 0, B, m->cout, N, m->y, bt, bm, m->gy, m->lossb);
+    nn_backward(m, B, 0);
+    m->step++;
+    const float c1 = 1.0f - powf(b1, (float)m->step), c2 = 1.0f - powf(b2, (float)m->step);
+    hipLaunchKernelGGL(k_adam, nbk(m->nparams, 256), 256, 0, 0, m->nparams, m->params, m->grads, m->m1, m->m2, lr, b1,
+                       b2, eps, c1, c2, weight_decay);
+    if (losses) {
+      UWG_HIP(hipMemcpy(lb.data(), m->lossb, sizeof(float) * B, hipMemcpyDeviceToHost));
+      double acc = 0.0;
+      for (float v : lb) acc += v;
+      losses[it] = (float)(acc / B);
+    }
+  }
+  UWG_HIP(hipDeviceSynchronize());
+  (void)hipFree(bt);
+  (void)hipFree(bm);
+  return 0;
+  NN_API_END
+}
+
+// mean loss over the uploaded set (evaluation, batches of B)
+extern "C" int uwg_nn_eval(uwg_nn* h, int B, float* loss) {
+  NN_API_BEGIN
+  NNModel* m = h->m;
+  ensure_batch(m, B);
+  const long N = m->grid[0].N();
+  double acc = 0.0;
+  std::vector<float> lb(B);
+  for (long s0 = 0; s0 < h->nsamples; s0 += B) {
+    int nb = (int)std::min<long>(B, h->nsamples - s0);
+    UWG_HIP(hipMemcpy(m->xin, h->data_x + s0 * m->cin * N, sizeof(float) * nb * m->cin * N, hipMemcpyDeviceToDevice));
+    nn_forward(m, nb, 0);
+    hipLaunchKernelGGL(k_loss, nb, 256, 0, 0, nb, m->cout, N, m->y, h->data_t + s0 * m->cout * N,
+                       h->data_m + s0 * m->cout * N, nullptr, m->lossb);
+    UWG_HIP(hipMemcpy(lb.data(), m->lossb, sizeof(float) * nb, hipMemcpyDeviceToHost));
+    for (int b = 0; b < nb; b++) acc += lb[b];
+  }
+  *loss = (float)(acc / h->nsamples);
+  return 0;
+  NN_API_END
 }

@@ -77,6 +77,7 @@ struct uwg_swarm {
 namespace uwg {
 
 struct KrylovSpace;
+template <typename T> struct KSpaceT;
 
 struct Level {
   Geom g;
@@ -139,6 +140,13 @@ struct uwg_stokes {
   KrylovSpace* kin = nullptr; // Krylov workspaces, owned by the handle
   KrylovSpace* kout = nullptr;
   KrylovSpace* kb = nullptr;
+  KSpaceT<float>* kf = nullptr;  // FP32 inner Krylov space of the mixed-precision solver
+  // learned-correction support
+  bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
+  int nn_mode = 0;               // 0: replace the coarse cycle, 1: add to it
+  int capture_level = -1;        // record coarse right-hand sides seen by the cycle
+  int capture_max = 0, captured = 0, capture_every = 1, capture_seen = 0;
+  float* capture_buf = nullptr;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -809,6 +817,47 @@ static void estimate_lmax(uwg_stokes* st, Level& A) {
   A.lmax = lam;
 }
 
+// Standardised log-viscosity per node of a level: element mean viscosity sum_g W_g / volume
+// (W reproduces the particle moments, so sum_g W_g = sum_p eta_p w_p detJ), log-averaged over
+// the elements around each node, then shifted/scaled to zero mean and unit spread.
+static void level_feature(uwg_stokes* st, int level, std::vector<float>& out) {
+  Level& A = st->L[level];
+  const Geom& g = A.g;
+  const int Q = g.dim == 3 ? 27 : 9;
+  std::vector<double> W((size_t)Q * g.E);
+  UWG_HIP(hipMemcpy(W.data(), A.W, sizeof(double) * Q * g.E, hipMemcpyDeviceToHost));
+  double vol = 1.0;
+  for (int d = 0; d < g.dim; d++) vol *= (g.hi[d] - g.lo[d]) / g.res[d];
+  std::vector<double> le(g.E);
+  for (long e = 0; e < g.E; e++) {
+    double s = 0.0;
+    for (int q = 0; q < Q; q++) s += W[(size_t)q * g.E + e];
+    le[e] = log(std::max(s / vol, 1e-300));
+  }
+  std::vector<double> f(g.N, 0.0);
+  for (long n = 0; n < g.N; n++) {
+    long i = n % g.nn[0], j = (n / g.nn[0]) % g.nn[1], k = n / ((long)g.nn[0] * g.nn[1]);
+    double acc = 0.0;
+    int cnt = 0;
+    for (int oz = (g.dim == 3 ? -1 : 0); oz <= 0; oz++)
+      for (int oy = -1; oy <= 0; oy++)
+        for (int ox = -1; ox <= 0; ox++) {
+          long ei = i + ox, ej = j + oy, ek = k + oz;
+          if (ei < 0 || ei >= g.res[0] || ej < 0 || ej >= g.res[1] || ek < 0 || ek >= g.res[2]) continue;
+          acc += le[ei + g.res[0] * (ej + (long)g.res[1] * ek)];
+          cnt++;
+        }
+    f[n] = acc / cnt;
+  }
+  double mean = 0.0, var = 0.0;
+  for (double v : f) mean += v;
+  mean /= g.N;
+  for (double v : f) var += (v - mean) * (v - mean);
+  double sd = std::max(sqrt(var / g.N), 1.0);
+  out.resize(g.N);
+  for (long n = 0; n < g.N; n++) out[n] = (float)((f[n] - mean) / sd);
+}
+
 static void setup_hierarchy(uwg_stokes* st) {
   for (size_t l = 1; l < st->L.size(); l++) {
     Level& F = st->L[l - 1];
@@ -834,6 +883,11 @@ static void setup_hierarchy(uwg_stokes* st) {
   for (auto& A : st->L) estimate_lmax(st, A);
   build_coarse(st);
   st->fp32_ready = false;
+  if (st->nn && st->nn_level < (int)st->L.size()) {
+    std::vector<float> f;
+    level_feature(st, st->nn_level, f);
+    nn_set_feature(st->nn.get(), f.data());
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -922,10 +976,32 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
   vk_residual<T>(n, b, r, 0);
   launch_restrict2<T>(A.g, C.g, r, LV<T>::b(C), C.bc, A.bc, 0);
-  if (st->nn && l + 1 == st->nn_level && std::is_same<T, float>::value) {
-    nn_coarse_correction(st->nn.get(), C.g, (const float*)LV<T>::b(C), (float*)LV<T>::x(C), C.bc, 0);
+  if (st->capture_level == l + 1 && st->captured < st->capture_max && (st->capture_seen++ % st->capture_every) == 0) {
+    float* dst = st->capture_buf + (long)st->captured * C.n();
+    if constexpr (std::is_same<T, float>::value)
+      vk_copy<float>(C.n(), LV<T>::b(C), dst, 0);
+    else
+      vk_d2f(C.n(), 1.0, LV<T>::b(C), dst, 0);
+    st->captured++;
+  }
+  bool use_nn = false;
+  if constexpr (std::is_same<T, float>::value)
+    use_nn = st->nn && !st->nn_disabled && l + 1 == st->nn_level;
+  if (use_nn && st->nn_mode == 0) {
+    // the learned network stands in for the coarse-grid cycle below this level
+    if constexpr (std::is_same<T, float>::value)
+      nn_coarse_correction(st->nn.get(), C.g, LV<T>::b(C), LV<T>::x(C), C.bc, 0);
   } else {
     vcycle<T>(st, l + 1, LV<T>::b(C), LV<T>::x(C));
+    if constexpr (std::is_same<T, float>::value) {
+      if (use_nn && st->nn_mode == 1) {
+        // learned correction of the coarse cycle's remaining error: x_C += NN(b_C - K_C x_C)
+        launch_apply_K<float>(C.g, C.W32, C.x32, C.r32, C.bc, 0);
+        vk_residual<float>(C.n(), C.b32, C.r32, 0);
+        nn_coarse_correction(st->nn.get(), C.g, C.r32, C.d32, C.bc, 0);
+        vk_axpy<float>(C.n(), 1.0f, C.d32, C.x32, 0);
+      }
+    }
   }
   launch_prolong_add<T>(A.g, C.g, LV<T>::x(C), x, A.bc, 0);
   chebyshev<T>(A, b, x, g_smooth_degree, false);
@@ -936,12 +1012,13 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
 // Converged when |r| <= max(rtol |r0|, atol) with r0 the initial residual
 // (KSPConvergedDefault with zero initial guess).
 // ---------------------------------------------------------------------------------------
-struct uwg::KrylovSpace {
+template <typename T>
+struct uwg::KSpaceT {
   long n = 0;
   int m = 0;
-  std::vector<double*> V, Z;
-  double* w = nullptr;
-  const double** dV = nullptr;
+  std::vector<T*> V, Z;
+  T* w = nullptr;
+  const T** dV = nullptr;
   void init(long n_, int m_) {
     if (n_ == n && m_ == m) return;
     free_all();
@@ -949,11 +1026,11 @@ struct uwg::KrylovSpace {
     m = m_;
     V.resize(m + 1);
     Z.resize(m);
-    for (auto& v : V) v = dalloc<double>(n);
-    for (auto& z : Z) z = dalloc<double>(n);
-    w = dalloc<double>(n);
-    UWG_HIP(hipMalloc(&dV, sizeof(double*) * (m + 1)));
-    UWG_HIP(hipMemcpy(dV, V.data(), sizeof(double*) * (m + 1), hipMemcpyHostToDevice));
+    for (auto& v : V) v = dalloc<T>(n);
+    for (auto& z : Z) z = dalloc<T>(n);
+    w = dalloc<T>(n);
+    UWG_HIP(hipMalloc(&dV, sizeof(T*) * (m + 1)));
+    UWG_HIP(hipMemcpy(dV, V.data(), sizeof(T*) * (m + 1), hipMemcpyHostToDevice));
   }
   void free_all() {
     for (auto& v : V) dfree(v);
@@ -965,31 +1042,40 @@ struct uwg::KrylovSpace {
     w = nullptr;
     dV = nullptr;
   }
-  ~KrylovSpace() { free_all(); }
+  ~KSpaceT() { free_all(); }
 };
+struct uwg::KrylovSpace : uwg::KSpaceT<double> {};
 
 using OpFn = std::function<void(const double*, double*)>;
+using OpFn32 = std::function<void(const float*, float*)>;
 
-static double dev_dot(uwg_stokes* st, long n, const double* a, const double* b) {
-  const double* ptr[1] = {b};
-  UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
-  vk_multi_dot(n, 1, a, st->dptrs, st->red_partial, st->red_out, 0);
+template <typename T>
+static double dev_dot(uwg_stokes* st, long n, const T* a, const T* b) {
+  const T* ptr[1] = {b};
+  UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(T*), hipMemcpyHostToDevice));
+  vk_multi_dot<T>(n, 1, a, (const T* const*)st->dptrs, st->red_partial, st->red_out, 0);
   double r;
   UWG_HIP(hipMemcpy(&r, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
   return r;
 }
 
 // multi-dot against K.V[0..k-1] in groups of 8
-static void dev_mdot(uwg_stokes* st, KrylovSpace& K, int k, const double* x, double* out) {
+template <typename T>
+static void dev_mdot(uwg_stokes* st, KSpaceT<T>& K, int k, const T* x, double* out) {
   for (int j0 = 0; j0 < k; j0 += 8) {
     int kk = std::min(8, k - j0);
-    vk_multi_dot(K.n, kk, x, K.dV + j0, st->red_partial, st->red_out, 0);
+    vk_multi_dot<T>(K.n, kk, x, K.dV + j0, st->red_partial, st->red_out, 0);
     UWG_HIP(hipMemcpy(out + j0, st->red_out, sizeof(double) * kk, hipMemcpyDeviceToHost));
   }
 }
 
-static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, const double* b, double* x,
-                  double rtol, double atol, int maxit, int* its_out, double* rnorm_out, bool x0_zero = true) {
+// Right-preconditioned FGMRES with classical Gram-Schmidt; vectors of type T, every
+// reduction and the Hessenberg least-squares problem in double.
+template <typename T>
+static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const T*, T*)>& A,
+                  const std::function<void(const T*, T*)>& M, const T* b, T* x, double rtol, double atol, int maxit,
+                  int* its_out, double* rnorm_out, bool x0_zero = true,
+                  const std::function<void(int, double)>* monitor = nullptr) {
   const long n = K.n;
   const int m = K.m;
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1);
@@ -999,33 +1085,33 @@ static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, 
   while (true) {
     // r = b - A x
     if (x0_zero && its == 0) {
-      vk_copy<double>(n, b, K.V[0], 0);
+      vk_copy<T>(n, b, K.V[0], 0);
     } else {
       A(x, K.V[0]);
-      vk_axpby<double>(n, 1.0, b, -1.0, K.V[0], 0);
+      vk_axpby<T>(n, (T)1, b, (T)-1, K.V[0], 0);
     }
-    double beta = sqrt(std::max(0.0, dev_dot(st, n, K.V[0], K.V[0])));
+    double beta = sqrt(std::max(0.0, dev_dot<T>(st, n, K.V[0], K.V[0])));
     if (r0 < 0) r0 = beta;
     rn = beta;
     if (beta <= std::max(rtol * r0, atol) || its >= maxit) {
       conv = beta <= std::max(rtol * r0, atol);
       break;
     }
-    vk_scale<double>(n, 1.0 / beta, K.V[0], K.V[0], 0);
+    vk_scale<T>(n, (T)(1.0 / beta), K.V[0], K.V[0], 0);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
     int j = 0;
     for (; j < m && its < maxit; j++) {
       M(K.V[j], K.Z[j]);
       A(K.Z[j], K.w);
-      dev_mdot(st, K, j + 1, K.w, hcol.data());
+      dev_mdot<T>(st, K, j + 1, K.w, hcol.data());
       for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
       UWG_HIP(hipMemcpy(st->dcoef, hcol.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice));
-      vk_multi_axpy(n, j + 1, K.dV, st->dcoef, K.w, 0);
+      vk_multi_axpy<T>(n, j + 1, K.dV, st->dcoef, K.w, 0);
       for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
-      double hn = sqrt(std::max(0.0, dev_dot(st, n, K.w, K.w)));
+      double hn = sqrt(std::max(0.0, dev_dot<T>(st, n, K.w, K.w)));
       hcol[j + 1] = hn;
-      if (hn > 0) vk_scale<double>(n, 1.0 / hn, K.w, K.V[j + 1], 0);
+      if (hn > 0) vk_scale<T>(n, (T)(1.0 / hn), K.w, K.V[j + 1], 0);
       // Givens
       for (int i = 0; i < j; i++) {
         double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
@@ -1042,13 +1128,15 @@ static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, 
       for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hcol[i];
       its++;
       rn = fabs(g[j + 1]);
-      if (g_verbose) fprintf(stderr, "  fgmres n=%ld it %d |r| %.3e rel %.3e\n", n, its, rn, rn / r0);
+      if (monitor) (*monitor)(its, rn / r0);
+      if (g_verbose) fprintf(stderr, "  fgmres<%s> n=%ld it %d |r| %.3e rel %.3e\n", sizeof(T) == 4 ? "f32" : "f64", n,
+                             its, rn, rn / r0);
       if (rn <= std::max(rtol * r0, atol) || hn == 0.0) {
         j++;
         break;
       }
     }
-    // back substitution
+    // back substitution and update x += Z y
     int k = j;
     for (int i = k - 1; i >= 0; i--) {
       double t = g[i];
@@ -1056,11 +1144,9 @@ static int fgmres(uwg_stokes* st, KrylovSpace& K, const OpFn& A, const OpFn& M, 
       y[i] = t / H[(size_t)i * m + i];
     }
     UWG_HIP(hipMemcpy(st->dcoef, y.data(), sizeof(double) * k, hipMemcpyHostToDevice));
-    const double** dZ = K.dV + (m + 1);  // Z pointer table follows V in dptrs block
-    (void)dZ;
-    std::vector<const double*> zp(K.Z.begin(), K.Z.begin() + k);
-    UWG_HIP(hipMemcpy(st->dptrs, zp.data(), sizeof(double*) * k, hipMemcpyHostToDevice));
-    vk_multi_axpy(n, k, st->dptrs, st->dcoef, x, 0);
+    std::vector<const T*> zp(K.Z.begin(), K.Z.begin() + k);
+    UWG_HIP(hipMemcpy(st->dptrs, zp.data(), sizeof(T*) * k, hipMemcpyHostToDevice));
+    vk_multi_axpy<T>(n, k, (const T* const*)st->dptrs, st->dcoef, x, 0);
     x0_zero = false;
     if (rn <= std::max(rtol * r0, atol)) {
       conv = true;
@@ -1117,6 +1203,8 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
   delete st->kin;
   delete st->kout;
   delete st->kb;
+  delete st->kf;
+  dfree(st->capture_buf);
   for (void* p : {(void*)st->fe, (void*)st->f, (void*)st->ubc, (void*)st->approxS, (void*)st->h, (void*)st->eta_tab,
                   (void*)st->force_tab, (void*)st->eta_p, (void*)st->force_p, (void*)st->red_partial,
                   (void*)st->red_out, (void*)st->dcoef, (void*)st->dptrs, (void*)st->coarse.Ainv})
@@ -1381,7 +1469,7 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   OpFn Kop = [&](const double* x, double* y) { launch_apply_K<double>(A.g, A.W, x, y, A.bc, 0); };
   OpFn Mv = [&](const double* x, double* y) {
     st->vcycles++;
-    if (o.precision == 1) {
+    if (o.precision != 0) {
       vk_d2f(n, 1.0, x, A.b32, 0);
       vcycle<float>(st, 0, A.b32, A.x32);
       vk_f2d(n, 1.0, A.x32, y, 0);
@@ -1471,7 +1559,7 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   };
   OpFn Mop = [&](const double* rr, double* zz) {
     st->vcycles++;
-    if (o.precision == 1) {
+    if (o.precision != 0) {
       vk_d2f(n, 1.0, rr, A.b32, 0);
       vcycle<float>(st, 0, A.b32, A.x32);
       vk_f2d(n, 1.0, A.x32, zz, 0);
@@ -1485,12 +1573,122 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   };
   int its = 0;
   double rn;
-  int rc = fgmres(st, Kb, Aop, Mop, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true);
+  // learned-correction safeguard: the FP64 residual must keep falling; if 10 iterations
+  // reduce it by less than 10 %, the network is switched off and multigrid takes over
+  // (FGMRES admits a preconditioner that changes between iterations)
+  std::vector<double> hist;
+  std::function<void(int, double)> mon = [&](int k, double rel) {
+    hist.push_back(rel);
+    if (o.learned && st->nn && !st->nn_disabled && k >= 10 && rel > 0.9 * hist[k - 10]) {
+      st->nn_disabled = true;
+      S->fallback_used = 1;
+      if (g_verbose) fprintf(stderr, " learned correction disabled at iteration %d (rel %.3e)\n", k, rel);
+    }
+  };
+  st->nn_disabled = !o.learned;
+  int rc = fgmres<double>(st, Kb, Aop, Mop, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true, &mon);
   S->outer_its = its;
   vk_copy<double>(n, x, u, 0);
   vk_copy<double>(E, x + n, p, 0);
   for (void* pp : {(void*)b, (void*)x, (void*)diagS, (void*)q, (void*)gp}) dfree(pp);
   return rc ? UWG_WARN_NOT_CONVERGED : 0;
+}
+
+// Mixed precision (precision 1): FP64 iterative refinement around an FP32 inner FGMRES.
+//   r = b - A x            FP64 operators (K with FP64 weights, G, G^T)
+//   A32 d = r / |r|        FGMRES with FP32 vectors, FP32 K/G/G^T applies, FP32 multigrid
+//                          cycle and Schur scaling; dot products and the Hessenberg problem
+//                          accumulate in FP64
+//   x += |r| d
+// The stopping test is always the FP64 true residual.
+static int solve_block_mixed(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
+                             const double* hp, double* u, double* p) {
+  Level& A = st->L[0];
+  const long n = A.n(), E = st->g.E, nt = n + E;
+  if (!st->kf) st->kf = new KSpaceT<float>();
+  KSpaceT<float>& Kf = *st->kf;
+  Kf.init(nt, o.restart);
+  double* b = dalloc<double>(nt);
+  double* x = dalloc<double>(nt);
+  double* r = dalloc<double>(nt);
+  double* gp = dalloc<double>(n);
+  float* r32 = dalloc<float>(nt);
+  float* d32 = dalloc<float>(nt);
+  float* diagS32 = dalloc<float>(E);
+  float* q32 = dalloc<float>(E);
+  float* gp32 = dalloc<float>(n);
+  vk_copy<double>(n, fu, b, 0);
+  vk_copy<double>(E, hp, b + n, 0);
+  vk_fill<double>(nt, 0.0, x, 0);
+  {
+    std::vector<double> sd(E);
+    std::vector<float> sf(E);
+    UWG_HIP(hipMemcpy(sd.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
+    for (long e = 0; e < E; e++) sf[e] = sd[e] != 0.0 ? (float)(-1.0 / sd[e]) : 0.0f;
+    UWG_HIP(hipMemcpy(diagS32, sf.data(), sizeof(float) * E, hipMemcpyHostToDevice));
+  }
+  auto A64 = [&](const double* xx, double* yy) {
+    launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
+    launch_apply_G<double>(st->g, st->Ge, xx + n, A.bc, gp, 0);
+    vk_axpy<double>(n, 1.0, gp, yy, 0);
+    launch_apply_GT<double>(st->g, st->Ge, xx, A.bc, yy + n, 0);
+  };
+  OpFn32 A32 = [&](const float* xx, float* yy) {
+    launch_apply_K<float>(A.g, A.W32, xx, yy, A.bc, 0);
+    launch_apply_G<float>(st->g, st->Ge, xx + n, A.bc, gp32, 0);
+    vk_axpy<float>(n, 1.0f, gp32, yy, 0);
+    launch_apply_GT<float>(st->g, st->Ge, xx, A.bc, yy + n, 0);
+  };
+  OpFn32 M32 = [&](const float* rr, float* zz) {
+    st->vcycles++;
+    vcycle<float>(st, 0, rr, zz);
+    launch_apply_GT<float>(st->g, st->Ge, zz, A.bc, q32, 0);
+    vk_axpby<float>(E, 1.0f, rr + n, -1.0f, q32, 0);  // q = r_p - G^T z_u
+    vk_xmy_scale<float>(E, diagS32, q32, zz + n, 0);
+  };
+  const double bn = sqrt(dev_dot<double>(st, nt, b, b));
+  int total = 0, sweeps = 0, status = 0;
+  double rn = bn;
+  while (true) {
+    if (sweeps == 0) {
+      vk_copy<double>(nt, b, r, 0);
+    } else {
+      A64(x, r);
+      vk_axpby<double>(nt, 1.0, b, -1.0, r, 0);
+      rn = sqrt(dev_dot<double>(st, nt, r, r));
+    }
+    if (g_verbose) fprintf(stderr, " refinement sweep %d: FP64 |r|/|b| = %.3e\n", sweeps, rn / bn);
+    if (rn <= o.outer_rtol * bn || total >= o.max_outer_its) break;
+    if (sweeps >= 12) {
+      status = UWG_WARN_NOT_CONVERGED;
+      break;
+    }
+    vk_d2f(nt, 1.0 / rn, r, r32, 0);
+    vk_fill<float>(nt, 0.0f, d32, 0);
+    // ask the FP32 solve for the remaining reduction, bounded by what FP32 can deliver
+    double want = 0.5 * o.outer_rtol * bn / rn;
+    static const double floor32 = getenv("UWG_FP32_FLOOR") ? atof(getenv("UWG_FP32_FLOOR")) : 1e-6;
+    double inner = std::min(0.5, std::max(want, floor32));
+    int its = 0;
+    double irn;
+    fgmres<float>(st, Kf, A32, M32, r32, d32, inner, 0.0, o.max_outer_its - total, &its, &irn, true);
+    total += its;
+    vk_f2d_axpy(nt, rn, d32, x, 0);
+    sweeps++;
+    if (its == 0) {
+      status = UWG_WARN_NOT_CONVERGED;
+      break;
+    }
+  }
+  S->outer_its = total;
+  S->pressure_its = sweeps;  // refinement sweeps
+  vk_copy<double>(n, x, u, 0);
+  vk_copy<double>(E, x + n, p, 0);
+  for (void* pp : {(void*)b, (void*)x, (void*)r, (void*)gp, (void*)r32, (void*)d32, (void*)diagS32, (void*)q32,
+                   (void*)gp32})
+    dfree(pp);
+  if (rn > o.outer_rtol * bn) status = UWG_WARN_NOT_CONVERGED;
+  return status;
 }
 
 extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_solve_stats* S) {
@@ -1519,7 +1717,7 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   if (!st->assembled) return arg_error("system not assembled");
   g_smooth_degree = std::max(1, o.smooth_degree);
   g_verbose = o.verbose || getenv("UWG_VERBOSE") != nullptr;
-  if (o.precision == 1) ensure_fp32(st);
+  if (o.precision != 0) ensure_fp32(st);
   UWG_HIP(hipDeviceSynchronize());
   double t1 = now();
   st->vcycles = 0;
@@ -1534,6 +1732,8 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   int status;
   if (o.method == 0)
     status = solve_schur(st, o, S, fu, hp, u, p);
+  else if (o.precision == 2)
+    status = solve_block_mixed(st, o, S, fu, hp, u, p);
   else
     status = solve_block(st, o, S, fu, hp, u, p);
   // u (free dofs) + Dirichlet values -> velocity field ; p -> pressure field
@@ -1672,6 +1872,90 @@ extern "C" int uwg_stokes_load_nn(uwg_stokes* st, int level, int nparams, const 
   API_BEGIN
   st->nn.reset(nn_create(st->L[level].g, channels, depth, params, nparams));
   st->nn_level = level;
+  st->nn_disabled = false;
+  if (st->assembled) {
+    std::vector<float> f;
+    level_feature(st, level, f);
+    nn_set_feature(st->nn.get(), f.data());
+  }
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_set_nn_mode(uwg_stokes* st, int mode) {
+  if (mode != 0 && mode != 1) return arg_error("mode must be 0 (replace) or 1 (add)");
+  st->nn_mode = mode;
+  return 0;
+}
+
+extern "C" int uwg_stokes_capture(uwg_stokes* st, int level, int max_samples, int every) {
+  if (level < 1 || level >= (int)st->L.size()) return arg_error("capture level out of range");
+  API_BEGIN
+  dfree(st->capture_buf);
+  st->capture_buf = max_samples > 0 ? dalloc<float>((long)max_samples * st->L[level].n()) : nullptr;
+  st->capture_level = max_samples > 0 ? level : -1;
+  st->capture_max = max_samples;
+  st->capture_every = std::max(1, every);
+  st->captured = 0;
+  st->capture_seen = 0;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_get_captured(uwg_stokes* st, float* out, int* count) {
+  API_BEGIN
+  *count = st->captured;
+  if (out && st->captured > 0)
+    UWG_HIP(hipMemcpy(out, st->capture_buf, sizeof(float) * (long)st->captured * st->L[st->capture_level].n(),
+                      hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+
+// K_l x = b on one level: FGMRES preconditioned by V-cycles starting at that level (FP64)
+extern "C" int uwg_stokes_level_solve(uwg_stokes* st, int level, const double* b, double* x, double rtol, int* its) {
+  if (!st->assembled) return arg_error("system not assembled");
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
+  API_BEGIN
+  Level& A = st->L[level];
+  const long n = A.n();
+  KrylovSpace K;
+  K.init(n, 30);
+  double* db = dalloc<double>(n);
+  double* dx = dalloc<double>(n);
+  double* wb = dalloc<double>(n);
+  UWG_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+  vk_fill<double>(n, 0.0, dx, 0);
+  OpFn Aop = [&](const double* xx, double* yy) { launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0); };
+  OpFn Mop = [&](const double* rr, double* zz) {
+    vk_copy<double>(n, rr, wb, 0);
+    vcycle<double>(st, level, wb, zz);
+  };
+  int k = 0;
+  double rn;
+  int rc = fgmres<double>(st, K, Aop, Mop, db, dx, rtol, 0.0, 2000, &k, &rn, true);
+  UWG_HIP(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (its) *its = k;
+  for (void* p : {(void*)db, (void*)dx, (void*)wb}) dfree(p);
+  return rc ? UWG_WARN_NOT_CONVERGED : 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_level_feature(uwg_stokes* st, int level, float* feat) {
+  if (!st->assembled) return arg_error("system not assembled");
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
+  API_BEGIN
+  std::vector<float> f;
+  level_feature(st, level, f);
+  memcpy(feat, f.data(), sizeof(float) * f.size());
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_level_mask(uwg_stokes* st, int level, unsigned char* mask) {
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
+  API_BEGIN
+  UWG_HIP(hipMemcpy(mask, st->L[level].bc, st->L[level].g.N, hipMemcpyDeviceToHost));
   return 0;
   API_END
 }

@@ -83,10 +83,10 @@ void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, d
 void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
 template <typename T>
 void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
-void launch_apply_G(const Geom& g, const double* Ge, const double* p, const uint8_t* bc, double* y,
-                    hipStream_t s);
-void launch_apply_GT(const Geom& g, const double* Ge, const double* u, const uint8_t* bc, double* q,
-                     hipStream_t s);
+template <typename T>
+void launch_apply_G(const Geom& g, const double* Ge, const T* p, const uint8_t* bc, T* y, hipStream_t s);
+template <typename T>
+void launch_apply_GT(const Geom& g, const double* Ge, const T* u, const uint8_t* bc, T* q, hipStream_t s);
 template <typename T>
 void launch_prolong_add(const Geom& fine, const Geom& coarse, const T* xc, T* xf, const uint8_t* bcf,
                         hipStream_t s);
@@ -119,10 +119,11 @@ void vk_f2d_axpy(long n, double scale, const float* x, double* y, hipStream_t s)
 void vk_f2d(long n, double scale, const float* x, double* y, hipStream_t s);        // y = scale*x
 void vk_d2f_vec(long n, const double* x, float* y, hipStream_t s);
 // y = sum_j c[j] * X[j]  (+ y if accumulate)   X: array of device pointers (device memory)
-void vk_multi_axpy(long n, int k, const double* const* X, const double* c, double* y, hipStream_t s);
+template <typename T>
+void vk_multi_axpy(long n, int k, const T* const* X, const double* c, T* y, hipStream_t s);
 // partial sums for dot products: out[j] = <x, Y[j]> for j < k ; deterministic
-void vk_multi_dot(long n, int k, const double* x, const double* const* Y, double* partial,
-                  double* out, hipStream_t s);
+template <typename T>
+void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s);
 int vk_reduce_blocks();
 // chebyshev helper: d = a*d + b*(dinv .* r) ; x += d  (fused)
 template <typename T>

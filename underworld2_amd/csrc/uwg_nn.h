@@ -8,8 +8,11 @@ namespace uwg {
 struct NNModel;
 NNModel* nn_create(const Geom& g, int channels, int depth, const float* params, int nparams);
 void nn_destroy(NNModel* m);
-// x = NN(features(level), r) on the nodes of level g (FP32); Dirichlet dofs of x are zero
+// per-node input feature (standardised log viscosity) of the level the model acts on
+void nn_set_feature(NNModel* m, const float* feat_host);
+// x = NN(feature, r) on the nodes of level g (FP32); Dirichlet dofs of x are zero
 void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s);
+long nn_param_count(NNModel* m);
 
 struct NNDeleter {
   void operator()(NNModel* m) const { nn_destroy(m); }

@@ -302,7 +302,12 @@ class StokesSolver:
         self._method = {"schur": 0, "bsscr": 0, "block": 1}[method]
 
     def set_precision(self, precision):
-        self._precision = {"fp64": 0, "mixed": 1}[precision]
+        """"fp64": every operator, cycle and Krylov vector in FP64.
+        "mixed": FP64 Krylov (residual, dot products, orthogonalisation) preconditioned by an
+                 FP32 multigrid cycle (FP32 operator and smoother; FP64 coarsest solve).
+        "mixed-refine": FP64 iterative refinement around an FP32 Krylov solve whose
+                 operator, cycle and basis vectors are FP32 (reductions in FP64)."""
+        self._precision = {"fp64": 0, "mixed": 1, "mixed-cycle": 1, "mixed-refine": 2}[precision]
 
     def set_learned(self, model):
         """Attach a trained learned coarse correction (``underworld2_amd.learned``)."""
