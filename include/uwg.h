@@ -153,14 +153,16 @@ int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
  * correction of the coarse cycle's remaining error.  FP32 cycles only (precision 1). */
 int uwg_stokes_load_nn(uwg_stokes *st, int level, int nparams, const float *params, int channels, int depth);
 int uwg_stokes_set_nn_mode(uwg_stokes *st, int mode);
-/* record up to max_samples coarse right-hand sides (every `every`-th) reaching `level`
- * during multigrid cycles; retrieved as float[count][N_level*dim] */
-int uwg_stokes_capture(uwg_stokes *st, int level, int max_samples, int every);
+/* record up to max_samples vectors of `level` (every `every`-th visit) during multigrid
+ * cycles: kind 0 the right-hand side restricted to the level, kind 1 the residual that the
+ * cycle below the level leaves (b - K x after it); retrieved as float[count][N_level*dim] */
+int uwg_stokes_capture(uwg_stokes *st, int level, int kind, int max_samples, int every);
 int uwg_stokes_get_captured(uwg_stokes *st, float *out, int *count);
 /* solve K_level x = b to rtol (FGMRES + cycles from that level, FP64) */
 int uwg_stokes_level_solve(uwg_stokes *st, int level, const double *b, double *x, double rtol, int *its);
-/* standardised log-viscosity per node of a level, and its Dirichlet mask */
-int uwg_stokes_level_feature(uwg_stokes *st, int level, float *feat);
+/* log10(eta / eta_ref) per node of a level (eta: element-mean viscosity averaged over the
+ * elements around the node; eta_ref: geometric mean over the level), and its Dirichlet mask */
+int uwg_stokes_level_feature(uwg_stokes *st, int level, float *feat, double *eta_ref);
 int uwg_stokes_level_mask(uwg_stokes *st, int level, unsigned char *mask);
 
 /* network training / evaluation on a node grid nx*ny*nz (inputs [B][4][N], outputs

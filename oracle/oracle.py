@@ -50,6 +50,8 @@ def lib():
         L.orc_advect_rk2.restype = ctypes.c_long
         L.orc_max_dt.restype = ctypes.c_double
         L.orc_update_owners.restype = ctypes.c_long
+        L.orc_bsscr_solve.restype = ctypes.c_int
+        L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -335,6 +337,36 @@ class System:
         u[self.free] = x[:self.nfree]
         p = x[self.nfree:self.nfree + E]
         return u.reshape(self.mesh.N, self.mesh.dim), p
+
+    def bsscr_solve(self, outer_rtol=1e-6, inner_rtol=1e-6, coarse_max=800):
+        """The reference's iterative solve restated on the CPU (cpu_baseline.c): Schur FGMRES
+        with the approxS preconditioner around FGMRES + Galerkin multigrid inner solves.
+        Free-slip walls on every face.  Returns (u, p, stats dict)."""
+        K = self.K.tocsr()
+        G = self.G.tocsr()
+        kp, kc, kv = (np.ascontiguousarray(K.indptr, np.int64), np.ascontiguousarray(K.indices, np.int32),
+                      np.ascontiguousarray(K.data, np.float64))
+        gp, gc, gv = (np.ascontiguousarray(G.indptr, np.int64), np.ascontiguousarray(G.indices, np.int32),
+                      np.ascontiguousarray(G.data, np.float64))
+        res = np.asarray(self.mesh.res, dtype=np.int32)
+        E = self.mesh.E
+        uf = np.zeros(self.nfree)
+        p = np.zeros(E)
+        st = np.zeros(4, np.int64)
+        tm = np.zeros(2)
+        f = np.ascontiguousarray(self.f, np.float64)
+        S = np.ascontiguousarray(self.approxS, np.float64)
+        rc = lib().orc_bsscr_solve(self.mesh.dim, _i(res), ctypes.c_long(self.nfree), _l(kp), _i(kc), _d(kv),
+                                   ctypes.c_long(E), _l(gp), _i(gc), _d(gv), _d(S), _d(f),
+                                   ctypes.c_double(outer_rtol), ctypes.c_double(inner_rtol), ctypes.c_int(coarse_max),
+                                   _d(uf), _d(p), _l(st), _d(tm))
+        if rc < 0:
+            raise RuntimeError("bsscr_solve: free-DOF numbering does not match free-slip walls")
+        u = np.zeros(self.ndof)
+        u[self.free] = uf
+        stats = dict(outer_its=int(st[0]), inner_its=int(st[1]), vcycles=int(st[2]), levels=int(st[3]),
+                     setup_time=float(tm[0]), solve_time=float(tm[1]), threads=int(lib().orc_num_threads()))
+        return u.reshape(self.mesh.N, self.mesh.dim), p, stats
 
     def residuals(self, u, p):
         """(|f - K u - G p|, |G^T u|) on free dofs (Stokes_SLE.c:204-256)."""

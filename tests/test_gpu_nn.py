@@ -1,0 +1,137 @@
+"""The learned coarse correction's HIP forward/backward kernels against the numpy oracle
+(oracle/nn_oracle.py, whose backward is pinned by finite differences on the CPU), the training
+loop, and the solver's fallback when a model does not help."""
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as nno
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(grid=(9, 7, 5), C=8, D=3, seed=3):
+    from underworld2_amd.learned import ConvNet
+    return ConvNet(grid, channels=C, depth=D, seed=seed)
+
+
+def _data(net, B, seed):
+    rng = np.random.default_rng(seed)
+    N = net.N
+    x = rng.normal(size=(B, 4, N)).astype(np.float32)
+    t = rng.normal(size=(B, 3, N)).astype(np.float32)
+    m = (rng.random((B, 3, N)) > 0.15).astype(np.float32)
+    return x, t, m
+
+
+@pytest.mark.parametrize("shape", [((9, 7, 5), 8, 3), ((6, 5, 7), 4, 2), ((17, 17, 17), 8, 4)])
+def test_forward_matches_oracle(shape):
+    grid, C, D = shape
+    print(f"forward grid={grid} C={C} D={D}", flush=True)
+    net = _net(grid, C, D)
+    # perturb the biases too so that every term of the layout is exercised
+    p = net.get_params()
+    rng = np.random.default_rng(0)
+    p = (p + 0.05 * rng.normal(size=p.size)).astype(np.float32)
+    net.set_params(p)
+    x, _, _ = _data(net, 2, 1)
+    y = net.forward(x)
+    ref = nno.Net(grid, C, D)
+    yo, _ = ref.forward(p.astype(np.float64), x.astype(np.float64))
+    err = np.abs(y - yo).max() / np.abs(yo).max()
+    assert err < 2e-5, err
+
+
+@pytest.mark.parametrize("shape", [((9, 7, 5), 8, 3), ((6, 5, 7), 4, 2)])
+def test_gradients_match_oracle(shape):
+    grid, C, D = shape
+    print(f"gradients grid={grid} C={C} D={D}", flush=True)
+    net = _net(grid, C, D, seed=5)
+    p = net.get_params()
+    rng = np.random.default_rng(1)
+    p = (p + 0.05 * rng.normal(size=p.size)).astype(np.float32)
+    net.set_params(p)
+    x, t, m = _data(net, 3, 2)
+    loss, g = net.loss_grad(x, t, m)
+    ref = nno.Net(grid, C, D)
+    lo, go, _ = ref.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64),
+                              m.astype(np.float64))
+    assert abs(loss - lo) < 1e-5 * abs(lo)
+    scale = np.abs(go).max()
+    assert np.abs(g - go).max() < 1e-4 * scale, np.abs(g - go).max() / scale
+    # per convolution, relative to that convolution's own gradient size
+    for ci, co, wo, bo in ref.specs:
+        gw, gwo = g[wo:bo + co], go[wo:bo + co]
+        assert np.linalg.norm(gw - gwo) < 1e-4 * np.linalg.norm(gwo) + 1e-7 * scale
+
+
+def test_training_reduces_loss_and_is_deterministic():
+    net = _net((9, 9, 9), 8, 3, seed=7)
+    rng = np.random.default_rng(4)
+    S, N = 24, net.N
+    x = rng.normal(size=(S, 4, N)).astype(np.float32)
+    # a learnable target: a fixed local linear map of the inputs
+    t = (0.7 * x[:, :3] - 0.2 * x[:, 3:4]).astype(np.float32)
+    m = np.ones((S, 3, N), dtype=np.float32)
+    net.set_data(x, t, m)
+    l0 = net.evaluate(8)
+    losses = net.fit(300, batch=8, lr=3e-3, seed=11)
+    l1 = net.evaluate(8)
+    print(f"training loss {l0:.4f} -> {l1:.4f}", flush=True)
+    assert np.all(np.isfinite(losses))
+    assert l1 < 0.1 * l0
+    twin = _net((9, 9, 9), 8, 3, seed=7)
+    twin.set_data(x, t, m)
+    twin.fit(300, batch=8, lr=3e-3, seed=11)
+    assert np.array_equal(twin.get_params(), net.get_params())
+
+
+def _small_problem(res=32, seed=123):
+    from underworld2_amd import learned
+    return learned.random_problem(res, seed, log10_contrast=3.0, corr_len=0.2)
+
+
+def test_capture_and_level_targets():
+    """Captured level vectors and their level-solve targets satisfy K_l x = b."""
+    import ctypes
+    import underworld2_amd as uw
+    from underworld2_amd import learned, _lib as L
+    st, v, p, _ = _small_problem(32)
+    solver = uw.systems.Solver(st)
+    solver.set_method("block")
+    solver.set_precision("mixed")
+    solver.set_outer_rtol(1e-6)
+    for kind in (0, 1):
+        X, T, M = learned.collect(st, solver, (8, 8, 8), kind, max_samples=4, every=3)
+        assert len(X) >= 2
+        lev = learned.level_index(st, (8, 8, 8))
+        N = 9 ** 3
+        _, eta_ref = learned.level_feature(st, lev, N)
+        for k in range(len(X)):
+            # scaled pair: x_in[:3] = b / rms, t = x eta_ref / rms with K x = b
+            b = X[k, :3].T.reshape(-1).astype(np.float64)
+            x = T[k].T.reshape(-1).astype(np.float64) / eta_ref
+            Kx = np.zeros_like(x)
+            L.check(L.lib().uwg_stokes_apply(st._h, 0, lev, L.dptr(x), L.dptr(Kx)))
+            assert np.linalg.norm(Kx - b) < 1e-5 * np.linalg.norm(b)
+            assert np.allclose(np.sqrt(np.mean(b * b)), 1.0, rtol=1e-5)
+    assert solver.get_stats().converged
+
+
+def test_fallback_with_useless_model():
+    """A network that makes the cycle worse is dropped by the FP64 residual monitor and the
+    solve still reaches its tolerance on plain multigrid."""
+    import underworld2_amd as uw
+    from underworld2_amd import learned
+    st, v, p, _ = _small_problem(32)
+    net = _net((9, 9, 9), 8, 3, seed=1)
+    bad = learned.LearnedCoarseCorrection(net.get_params() * 50.0, (8, 8, 8), 8, 3, mode="replace")
+    solver = uw.systems.Solver(st)
+    solver.set_method("block")
+    solver.set_precision("mixed")
+    solver.set_outer_rtol(1e-6)
+    solver.set_learned(bad)
+    solver.solve()
+    s = solver.get_stats()
+    print(f"fallback_used={s.fallback_used} its={s.outer_its}", flush=True)
+    assert s.converged and s.fallback_used == 1
+    assert s.res_momentum < 1e-5 * s.norm_f

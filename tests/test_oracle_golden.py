@@ -372,3 +372,15 @@ def test_owner_update_swap_and_append():
     # cell 1: [3,4,5] -append 0-> [3,4,5,0] -remove 4 (swap with last)-> [3,0,5]
     np.testing.assert_array_equal(T.tbl[0, :T.cnt[0]], [2, 1, 4])
     np.testing.assert_array_equal(T.tbl[1, :T.cnt[1]], [3, 0, 5])
+
+
+def test_cpu_baseline_bsscr_matches_direct_solve():
+    """The CPU restatement of the iterative Schur/multigrid solve (the CPU baseline) converges
+    to the oracle's exact discrete solution."""
+    from sinker import build_oracle
+    o = build_oracle((8, 8, 8), 8, 0, (0.5, 0.5, 0.6), 0.15, eta=(1.0, 1e3))
+    u, p, st = o["system"].bsscr_solve(1e-9, 1e-9)
+    assert st["levels"] >= 2 and st["outer_its"] > 0
+    assert np.linalg.norm(u - o["u"]) < 1e-6 * np.linalg.norm(o["u"])
+    pe = o["p"] - o["p"].mean()
+    assert np.linalg.norm((p - p.mean()) - pe) < 1e-4 * np.linalg.norm(pe)

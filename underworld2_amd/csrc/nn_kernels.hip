@@ -70,6 +70,94 @@ __global__ void k_conv_fwd(int B, int Ci, int Co, Grid3 g, const float* __restri
   out[t] = relu ? fmaxf(acc, 0.0f) : acc;
 }
 
+// Channel-blocked forward: each thread produces COB output channels of one node, so every
+// input value fetched serves COB FMAs; the weight index depends only on the block's channel
+// group and the loop counters (wave-uniform), so weights come through scalar loads.
+template <int COB>
+__global__ __launch_bounds__(256) void k_conv_fwd_blk(int B, int Ci, int Co, Grid3 g, const float* __restrict__ in,
+                                                      const float* __restrict__ w, const float* __restrict__ bias,
+                                                      float* __restrict__ out, int relu) {
+  const long N = g.N();
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int co0 = blockIdx.y * COB;
+  if (t >= (long)B * N) return;
+  const long p = t % N;
+  const int b = (int)(t / N);
+  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
+  float acc[COB];
+#pragma unroll
+  for (int c = 0; c < COB; c++) acc[c] = bias[co0 + c];
+  const long cs = (long)Ci * 27;
+  for (int ci = 0; ci < Ci; ci++) {
+    const float* src = in + ((long)b * Ci + ci) * N;
+    const float* wc = w + (long)co0 * cs + ci * 27;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+      const int kk = k + dz;
+      const bool okz = kk >= 0 && kk < g.nz;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        const int jj = j + dy;
+        const bool oky = okz && jj >= 0 && jj < g.ny;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          const int ii = i + dx;
+          const float v = (oky && ii >= 0 && ii < g.nx) ? src[ii + (long)g.nx * (jj + (long)g.ny * kk)] : 0.0f;
+          const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+#pragma unroll
+          for (int c = 0; c < COB; c++) acc[c] += wc[c * cs + tap] * v;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < COB; c++) {
+    const float a = acc[c];
+    out[((long)b * Co + co0 + c) * N + p] = relu ? fmaxf(a, 0.0f) : a;
+  }
+}
+
+// Channel-blocked input gradient: each thread produces CIB input channels of one node.
+template <int CIB>
+__global__ __launch_bounds__(256) void k_conv_bwd_data_blk(int B, int Ci, int Co, Grid3 g,
+                                                           const float* __restrict__ dout,
+                                                           const float* __restrict__ w, float* __restrict__ din) {
+  const long N = g.N();
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ci0 = blockIdx.y * CIB;
+  if (t >= (long)B * N) return;
+  const long p = t % N;
+  const int b = (int)(t / N);
+  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
+  float acc[CIB];
+#pragma unroll
+  for (int c = 0; c < CIB; c++) acc[c] = 0.0f;
+  for (int co = 0; co < Co; co++) {
+    const float* src = dout + ((long)b * Co + co) * N;
+    const float* wc = w + ((long)co * Ci + ci0) * 27;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+      const int kk = k - dz;
+      const bool okz = kk >= 0 && kk < g.nz;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++) {
+        const int jj = j - dy;
+        const bool oky = okz && jj >= 0 && jj < g.ny;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          const int ii = i - dx;
+          const float v = (oky && ii >= 0 && ii < g.nx) ? src[ii + (long)g.nx * (jj + (long)g.ny * kk)] : 0.0f;
+          const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+#pragma unroll
+          for (int c = 0; c < CIB; c++) acc[c] += wc[c * 27 + tap] * v;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CIB; c++) din[((long)b * Ci + ci0 + c) * N + p] = acc[c];
+}
+
 // din[b][ci][p] = sum_co sum_tap w[co][ci][tap] dout[b][co][p - off(tap)]
 __global__ void k_conv_bwd_data(int B, int Ci, int Co, Grid3 g, const float* __restrict__ dout,
                                 const float* __restrict__ w, float* __restrict__ din) {
@@ -296,10 +384,10 @@ __global__ void k_pack_input(long N, int dim, const float* __restrict__ r, const
   x[(long)dim * N + t] = feat[t];
 }
 __global__ void k_unpack_output(long N, int dim, const float* __restrict__ y, const float* __restrict__ scale,
-                                const uint8_t* __restrict__ bc, float* __restrict__ out) {
+                                float out_scale, const uint8_t* __restrict__ bc, float* __restrict__ out) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N) return;
-  const float s = scale[0];
+  const float s = scale[0] * out_scale;
   const uint8_t m = bc[t];
   for (int d = 0; d < dim; d++) out[t * dim + d] = ((m >> d) & 1) ? 0.0f : s * y[(long)d * N + t];
 }
@@ -316,6 +404,7 @@ struct ConvSpec {
 // (l = D-2..0); head 3D-1.
 struct NNModel {
   int C = 32, depth = 4, cin = 4, cout = 3;
+  float out_scale = 1.0f;  // 1 / reference viscosity of the level (outputs are non-dimensional)
   std::vector<Grid3> grid;
   std::vector<ConvSpec> conv;
   long nparams = 0;
@@ -397,9 +486,18 @@ static void ensure_batch(NNModel* m, int B) {
 static void conv_fwd(NNModel* m, int c, int B, const float* in, float* out, bool relu, hipStream_t s) {
   const ConvSpec& cs = m->conv[c];
   Grid3 g = m->grid[m->level_of(c)];
-  long n = (long)B * cs.co * g.N();
-  hipLaunchKernelGGL(k_conv_fwd, nbk(n, 256), 256, 0, s, B, cs.ci, cs.co, g, in, m->params + cs.woff,
-                     m->params + cs.boff, out, relu ? 1 : 0);
+  const long nb = (long)B * g.N();
+  const float* w = m->params + cs.woff;
+  const float* bias = m->params + cs.boff;
+  if (cs.co % 4 == 0)
+    hipLaunchKernelGGL(k_conv_fwd_blk<4>, dim3(nbk(nb, 256), cs.co / 4), 256, 0, s, B, cs.ci, cs.co, g, in, w, bias,
+                       out, relu ? 1 : 0);
+  else if (cs.co == 3)
+    hipLaunchKernelGGL(k_conv_fwd_blk<3>, dim3(nbk(nb, 256), 1), 256, 0, s, B, cs.ci, cs.co, g, in, w, bias, out,
+                       relu ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_conv_fwd, nbk(nb * cs.co, 256), 256, 0, s, B, cs.ci, cs.co, g, in, w, bias, out,
+                       relu ? 1 : 0);
 }
 
 // forward for a batch packed in m->xin; output in m->y
@@ -441,8 +539,13 @@ static void conv_bwd(NNModel* m, int c, int B, const float* in, float* gout, flo
   hipLaunchKernelGGL(k_conv_bwd_w, cs.co * cs.ci, 256, 0, s, B, cs.ci, cs.co, g, gout, in, m->grads + cs.woff);
   hipLaunchKernelGGL(k_bias_grad, cs.co, 256, 0, s, B, cs.co, g.N(), gout, m->grads + cs.boff);
   if (gin) {
-    long n = (long)B * cs.ci * g.N();
-    hipLaunchKernelGGL(k_conv_bwd_data, nbk(n, 256), 256, 0, s, B, cs.ci, cs.co, g, gout, m->params + cs.woff, gin);
+    const long nb = (long)B * g.N();
+    if (cs.ci % 4 == 0)
+      hipLaunchKernelGGL(k_conv_bwd_data_blk<4>, dim3(nbk(nb, 256), cs.ci / 4), 256, 0, s, B, cs.ci, cs.co, g, gout,
+                         m->params + cs.woff, gin);
+    else
+      hipLaunchKernelGGL(k_conv_bwd_data, nbk(nb * cs.ci, 256), 256, 0, s, B, cs.ci, cs.co, g, gout,
+                         m->params + cs.woff, gin);
   }
 }
 
@@ -522,7 +625,8 @@ NNModel* nn_create(const Geom& g, int channels, int depth, const float* params, 
   return m;
 }
 
-void nn_set_feature(NNModel* m, const float* feat_host) {
+void nn_set_feature(NNModel* m, const float* feat_host, double eta_ref) {
+  m->out_scale = (float)(1.0 / eta_ref);
   UWG_HIP(hipMemcpy(m->feat, feat_host, sizeof(float) * m->grid[0].N(), hipMemcpyHostToDevice));
 }
 
@@ -531,7 +635,7 @@ void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, c
   hipLaunchKernelGGL(k_rms, 1, 256, 0, s, N * 3, r, m->scale);
   hipLaunchKernelGGL(k_pack_input, nbk(N, 256), 256, 0, s, N, 3, r, m->feat, m->scale, m->xin);
   nn_forward(m, 1, s);
-  hipLaunchKernelGGL(k_unpack_output, nbk(N, 256), 256, 0, s, N, 3, m->y, m->scale, bc, x);
+  hipLaunchKernelGGL(k_unpack_output, nbk(N, 256), 256, 0, s, N, 3, m->y, m->scale, m->out_scale, bc, x);
   UWG_HIP(hipGetLastError());
   (void)g;
 }

@@ -144,7 +144,8 @@ struct uwg_stokes {
   // learned-correction support
   bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
   int nn_mode = 0;               // 0: replace the coarse cycle, 1: add to it
-  int capture_level = -1;        // record coarse right-hand sides seen by the cycle
+  int capture_level = -1;        // record coarse vectors seen by the cycle
+  int capture_kind = 0;          // 0: right-hand side reaching the level, 1: residual left by its cycle
   int capture_max = 0, captured = 0, capture_every = 1, capture_seen = 0;
   float* capture_buf = nullptr;
 };
@@ -817,10 +818,10 @@ static void estimate_lmax(uwg_stokes* st, Level& A) {
   A.lmax = lam;
 }
 
-// Standardised log-viscosity per node of a level: element mean viscosity sum_g W_g / volume
-// (W reproduces the particle moments, so sum_g W_g = sum_p eta_p w_p detJ), log-averaged over
-// the elements around each node, then shifted/scaled to zero mean and unit spread.
-static void level_feature(uwg_stokes* st, int level, std::vector<float>& out) {
+// Log-viscosity per node of a level: element mean viscosity sum_g W_g / volume (W reproduces
+// the particle moments, so sum_g W_g = sum_p eta_p w_p detJ), log-averaged over the elements
+// around each node; returned as log10(eta / eta_ref) with eta_ref the level's geometric mean.
+static void level_feature(uwg_stokes* st, int level, std::vector<float>& out, double* eta_ref) {
   Level& A = st->L[level];
   const Geom& g = A.g;
   const int Q = g.dim == 3 ? 27 : 9;
@@ -849,13 +850,12 @@ static void level_feature(uwg_stokes* st, int level, std::vector<float>& out) {
         }
     f[n] = acc / cnt;
   }
-  double mean = 0.0, var = 0.0;
+  double mean = 0.0;
   for (double v : f) mean += v;
   mean /= g.N;
-  for (double v : f) var += (v - mean) * (v - mean);
-  double sd = std::max(sqrt(var / g.N), 1.0);
   out.resize(g.N);
-  for (long n = 0; n < g.N; n++) out[n] = (float)((f[n] - mean) / sd);
+  for (long n = 0; n < g.N; n++) out[n] = (float)((f[n] - mean) / log(10.0));
+  *eta_ref = exp(mean);
 }
 
 static void setup_hierarchy(uwg_stokes* st) {
@@ -885,8 +885,9 @@ static void setup_hierarchy(uwg_stokes* st) {
   st->fp32_ready = false;
   if (st->nn && st->nn_level < (int)st->L.size()) {
     std::vector<float> f;
-    level_feature(st, st->nn_level, f);
-    nn_set_feature(st->nn.get(), f.data());
+    double eta_ref;
+    level_feature(st, st->nn_level, f, &eta_ref);
+    nn_set_feature(st->nn.get(), f.data(), eta_ref);
   }
 }
 
@@ -963,6 +964,16 @@ void coarse_solve<float>(uwg_stokes* st, Level& A, const float* b, float* x) {
 }
 
 template <typename T>
+static void capture_vector(uwg_stokes* st, long n, const T* v) {
+  float* dst = st->capture_buf + (long)st->captured * n;
+  if constexpr (std::is_same<T, float>::value)
+    vk_copy<float>(n, v, dst, 0);
+  else
+    vk_d2f(n, 1.0, v, dst, 0);
+  st->captured++;
+}
+
+template <typename T>
 static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   Level& A = st->L[l];
   const long n = A.n();
@@ -976,14 +987,9 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
   vk_residual<T>(n, b, r, 0);
   launch_restrict2<T>(A.g, C.g, r, LV<T>::b(C), C.bc, A.bc, 0);
-  if (st->capture_level == l + 1 && st->captured < st->capture_max && (st->capture_seen++ % st->capture_every) == 0) {
-    float* dst = st->capture_buf + (long)st->captured * C.n();
-    if constexpr (std::is_same<T, float>::value)
-      vk_copy<float>(C.n(), LV<T>::b(C), dst, 0);
-    else
-      vk_d2f(C.n(), 1.0, LV<T>::b(C), dst, 0);
-    st->captured++;
-  }
+  const bool capture = st->capture_level == l + 1 && st->captured < st->capture_max &&
+                       (st->capture_seen++ % st->capture_every) == 0;
+  if (capture && st->capture_kind == 0) capture_vector<T>(st, C.n(), LV<T>::b(C));
   bool use_nn = false;
   if constexpr (std::is_same<T, float>::value)
     use_nn = st->nn && !st->nn_disabled && l + 1 == st->nn_level;
@@ -993,6 +999,13 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
       nn_coarse_correction(st->nn.get(), C.g, LV<T>::b(C), LV<T>::x(C), C.bc, 0);
   } else {
     vcycle<T>(st, l + 1, LV<T>::b(C), LV<T>::x(C));
+    if (capture && st->capture_kind == 1) {
+      // residual the coarse cycle leaves behind: its error K_C^{-1} r is what an additive
+      // correction has to supply
+      launch_apply_K<T>(C.g, LV<T>::W(C), LV<T>::x(C), LV<T>::r(C), C.bc, 0);
+      vk_residual<T>(C.n(), LV<T>::b(C), LV<T>::r(C), 0);
+      capture_vector<T>(st, C.n(), LV<T>::r(C));
+    }
     if constexpr (std::is_same<T, float>::value) {
       if (use_nn && st->nn_mode == 1) {
         // learned correction of the coarse cycle's remaining error: x_C += NN(b_C - K_C x_C)
@@ -1702,14 +1715,19 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   memset(S, 0, sizeof(*S));
   if (o.restart < 2 || o.restart > 200) return arg_error("restart must be in [2, 200]");
   if (o.method == 0 && o.learned) return arg_error("the learned correction is only available with method 1");
-  if (o.learned && !st->nn) return arg_error("no learned model loaded");
+  if (o.learned && o.precision != 1) return arg_error("the learned correction runs inside the FP32 cycle (precision 1)");
   API_BEGIN
   double t0 = now();
   if (o.mg_levels > 0 && o.mg_levels != (int)st->L.size()) {
     st->levels_req = o.mg_levels;
     build_levels(st, o.mg_levels);  // keeps the level-0 weights
+    if (st->nn && st->nn_level >= (int)st->L.size()) {
+      st->nn.reset();  // its level is gone
+      st->nn_level = -1;
+    }
     if (st->assembled) setup_hierarchy(st);
   }
+  if (o.learned && !st->nn) return arg_error("no learned model loaded");
   if (st->swarm && st->assembled_state != st->swarm->state && st->assembled_state != -2) {
     int rc = uwg_stokes_assemble(st, st->swarm);
     if (rc) return rc;
@@ -1875,8 +1893,9 @@ extern "C" int uwg_stokes_load_nn(uwg_stokes* st, int level, int nparams, const 
   st->nn_disabled = false;
   if (st->assembled) {
     std::vector<float> f;
-    level_feature(st, level, f);
-    nn_set_feature(st->nn.get(), f.data());
+    double eta_ref;
+    level_feature(st, level, f, &eta_ref);
+    nn_set_feature(st->nn.get(), f.data(), eta_ref);
   }
   return 0;
   API_END
@@ -1888,12 +1907,15 @@ extern "C" int uwg_stokes_set_nn_mode(uwg_stokes* st, int mode) {
   return 0;
 }
 
-extern "C" int uwg_stokes_capture(uwg_stokes* st, int level, int max_samples, int every) {
+extern "C" int uwg_stokes_capture(uwg_stokes* st, int level, int kind, int max_samples, int every) {
   if (level < 1 || level >= (int)st->L.size()) return arg_error("capture level out of range");
+  if (kind != 0 && kind != 1) return arg_error("capture kind must be 0 (right-hand side) or 1 (residual)");
+  if (kind == 1 && level == (int)st->L.size() - 1) return arg_error("the coarsest level leaves no residual");
   API_BEGIN
   dfree(st->capture_buf);
   st->capture_buf = max_samples > 0 ? dalloc<float>((long)max_samples * st->L[level].n()) : nullptr;
   st->capture_level = max_samples > 0 ? level : -1;
+  st->capture_kind = kind;
   st->capture_max = max_samples;
   st->capture_every = std::max(1, every);
   st->captured = 0;
@@ -1941,12 +1963,14 @@ extern "C" int uwg_stokes_level_solve(uwg_stokes* st, int level, const double* b
   API_END
 }
 
-extern "C" int uwg_stokes_level_feature(uwg_stokes* st, int level, float* feat) {
+extern "C" int uwg_stokes_level_feature(uwg_stokes* st, int level, float* feat, double* eta_ref) {
   if (!st->assembled) return arg_error("system not assembled");
   if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
   API_BEGIN
   std::vector<float> f;
-  level_feature(st, level, f);
+  double er;
+  level_feature(st, level, f, &er);
+  if (eta_ref) *eta_ref = er;
   memcpy(feat, f.data(), sizeof(float) * f.size());
   return 0;
   API_END

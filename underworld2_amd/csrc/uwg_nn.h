@@ -8,8 +8,9 @@ namespace uwg {
 struct NNModel;
 NNModel* nn_create(const Geom& g, int channels, int depth, const float* params, int nparams);
 void nn_destroy(NNModel* m);
-// per-node input feature (standardised log viscosity) of the level the model acts on
-void nn_set_feature(NNModel* m, const float* feat_host);
+// per-node input feature (log10 of viscosity over its geometric mean eta_ref) of the level the
+// model acts on; outputs are scaled by rms(residual) / eta_ref
+void nn_set_feature(NNModel* m, const float* feat_host, double eta_ref);
 // x = NN(feature, r) on the nodes of level g (FP32); Dirichlet dofs of x are zero
 void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s);
 long nn_param_count(NNModel* m);
