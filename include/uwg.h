@@ -180,6 +180,9 @@ int uwg_nn_train(uwg_nn *h, int steps, int B, float lr, float weight_decay, long
 int uwg_nn_eval(uwg_nn *h, int B, float *loss);
 /* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply */
 int uwg_set_apply_chunk(int kz3, int kz2);
+/* STREAM-style device copy of n doubles (reads 8n bytes, writes 8n bytes); ms per copy.
+ * Measures attainable HBM bandwidth and calibrates byte counters. */
+int uwg_stream_copy(long n, int reps, double *ms);
 /* time `reps` applies of K (level 0) in the given precision; returns ms per apply */
 int uwg_stokes_time_apply(uwg_stokes *st, int precision, int reps, double *ms);
 

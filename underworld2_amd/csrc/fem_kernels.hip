@@ -364,19 +364,12 @@ __device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, c
   }
   T dX[3] = {0, 0, 0}, dY[3] = {0, 0, 0}, dZ[3] = {0, 0, 0}, dXY[3] = {0, 0, 0}, dXZ[3] = {0, 0, 0},
     dYZ[3] = {0, 0, 0}, dXYZ[3] = {0, 0, 0};
-  // Sum factorisation.  du/dx is constant along x, so its values at the 27 points are 9
-  // "lines" indexed by (Y, Z); likewise du/dy by (X, Z) and du/dz by (X, Y).  The dual
-  // contraction factorises the same way: the x-stress is summed along each x-line first.
-  // Planes of constant Z are processed in turn so only the z-lines stay live throughout.
-  T GZ[3][9], PZ[3][9];
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      const T X = (T)gpt(q % 3), Y = (T)gpt(q / 3);
-      GZ[i][q] = sz * (cZ[i] + cXZ[i] * X + cYZ[i] * Y + cXYZ[i] * (X * Y));
-      PZ[i][q] = (T)0;
-    }
+  // Partial sum factorisation.  du/dx is constant along x, so within a plane of constant Z its
+  // values at the 9 points are 3 "lines" indexed by Y; likewise du/dy by X.  The dual
+  // contraction factorises the same way: the x-stress is summed along each x-line before it
+  // is folded into the dual coefficients.  du/dz and the z-stress are formed point by point
+  // and folded at once, which keeps only one plane's lines live (register pressure sets the
+  // occupancy of this kernel).
 #pragma unroll
   for (int gz = 0; gz < 3; gz++) {
     const T Z = (T)gpt(gz);
@@ -395,22 +388,29 @@ __device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, c
     for (int gy = 0; gy < 3; gy++)
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) {
+        const T X = (T)gpt(gx), Y = (T)gpt(gy);
         const T w = W[(long)(gx + 3 * gy + 9 * gz) * E + e];
         const T w2 = w + w;
-        const int qz = gx + 3 * gy;
         const T G00 = GX[0][gy], G10 = GX[1][gy], G20 = GX[2][gy];
         const T G01 = GY[0][gx], G11 = GY[1][gx], G21 = GY[2][gx];
-        const T G02 = GZ[0][qz], G12 = GZ[1][qz], G22 = GZ[2][qz];
-        const T S01 = w * (G01 + G10), S02 = w * (G02 + G20), S12 = w * (G12 + G21);
+        T G2[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) G2[i] = sz * (cZ[i] + cXZ[i] * X + cYZ[i] * Y + cXYZ[i] * (X * Y));
+        const T S01 = w * (G01 + G10), S02 = w * (G2[0] + G20), S12 = w * (G2[1] + G21);
         PX[0][gy] += w2 * G00;
         PX[1][gy] += S01;
         PX[2][gy] += S02;
         PY[0][gx] += S01;
         PY[1][gx] += w2 * G11;
         PY[2][gx] += S12;
-        PZ[0][qz] += S02;
-        PZ[1][qz] += S12;
-        PZ[2][qz] += w2 * G22;
+        const T pz[3] = {S02 * sz, S12 * sz, (w2 * G2[2]) * sz};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          dZ[i] += pz[i];
+          dXZ[i] += pz[i] * X;
+          dYZ[i] += pz[i] * Y;
+          dXYZ[i] += pz[i] * (X * Y);
+        }
       }
     // fold the x- and y-lines of this plane into the dual coefficients
 #pragma unroll
@@ -429,17 +429,6 @@ __device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, c
         dXYZ[i] += py * (K * Z);
       }
   }
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      const T X = (T)gpt(q % 3), Y = (T)gpt(q / 3);
-      const T pz = PZ[i][q] * sz;
-      dZ[i] += pz;
-      dXZ[i] += pz * X;
-      dYZ[i] += pz * Y;
-      dXYZ[i] += pz * (X * Y);
-    }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     // r_a = (1/8) (s_a dX + t_a dY + r_a dZ + s t dXY + s r dXZ + t r dYZ + s t r dXYZ)
@@ -471,14 +460,30 @@ __device__ __forceinline__ void load_level3(const KArgs<T>& A, const T* __restri
   }
 }
 
+// Occupancy: the FP64 instance is held to 256 VGPRs (two waves per SIMD; the cap costs a few
+// bytes of spill), the FP32 instance needs no cap (153 VGPRs, three waves per SIMD).
 template <typename T>
-__global__ __launch_bounds__(256) void k_apply3(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
                                                 T* __restrict__ y, const uint8_t* __restrict__ bc) {
   __shared__ T sb[12][256];
   __shared__ T st[12][256];
   const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
-  const int ei = blockIdx.x * 15 - 1 + lx, ej = blockIdx.y * 15 - 1 + ly;
-  const int k0 = blockIdx.z * A.KZ;
+  // Workgroups are dealt to the 8 XCDs round-robin by linear id; renumber them so that each
+  // XCD receives a contiguous run of tiles (x fastest).  Neighbouring tiles then share the
+  // XCD's L2, where the cache lines straddling their common element column/row are reused.
+  int bx, by, bz;
+  {
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int total = gx * gy * gridDim.z;
+    const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int xcd = id & 7, loc = id >> 3, per = total >> 3, rem = total & 7;
+    const int t = xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
+  }
+  const int ei = bx * 15 - 1 + lx, ej = by * 15 - 1 + ly;
+  const int k0 = bz * A.KZ;
   const int k1 = min(k0 + A.KZ, A.rz + 1);
   const bool colE = ei >= 0 && ei < A.rx && ej >= 0 && ej < A.ry;
   const bool ownN = lx >= 1 && ly >= 1 && ei <= A.rx && ej <= A.ry;

@@ -1856,6 +1856,33 @@ extern "C" int uwg_set_apply_chunk(int kz3, int kz2) {
   return 0;
 }
 
+// STREAM-style copy probe: y = x over n doubles, timed with HIP events; the kernel is also the
+// calibration point for hardware byte counters (known bytes read and written per launch).
+extern "C" int uwg_stream_copy(long n, int reps, double* ms) {
+  if (n < 1 || reps < 1) return arg_error("stream copy: n and reps must be positive");
+  API_BEGIN
+  double* x = dalloc<double>(n);
+  double* y = dalloc<double>(n);
+  vk_fill<double>(n, 1.0, x, 0);
+  vk_copy<double>(n, x, y, 0);
+  hipEvent_t e0, e1;
+  UWG_HIP(hipEventCreate(&e0));
+  UWG_HIP(hipEventCreate(&e1));
+  UWG_HIP(hipEventRecord(e0, 0));
+  for (int i = 0; i < reps; i++) vk_copy<double>(n, x, y, 0);
+  UWG_HIP(hipEventRecord(e1, 0));
+  UWG_HIP(hipEventSynchronize(e1));
+  float t;
+  UWG_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  UWG_HIP(hipEventDestroy(e0));
+  UWG_HIP(hipEventDestroy(e1));
+  dfree(x);
+  dfree(y);
+  return 0;
+  API_END
+}
+
 extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int reps, double* ms) {
   if (!st->assembled) return arg_error("system not assembled");
   API_BEGIN
