@@ -106,7 +106,9 @@ def time_mode(P, mode, steps, warmup, rtol):
 
 def roofline(P, precision, reps=50):
     """Average duration of the K apply (the dominant kernel) measured with HIP events on the
-    stream it runs on; algorithmic bytes = E*27*s (weights) + 2*3*N*s (read x, write y)."""
+    stream it runs on; algorithmic bytes = E*27*s (weights) + 2*3*N*s (read x, write y).
+    ``traffic`` (HBM bytes per launch) comes from the FETCH_SIZE / WRITE_SIZE counter runs
+    summarised in profiles/k_apply3_sinker128_traffic.json (128^3 only)."""
     import underworld2_amd._lib as L
     st = P["stokes"]
     ms = ctypes.c_double()
@@ -117,8 +119,14 @@ def roofline(P, precision, reps=50):
     Q = 3 ** d
     bytes_ = E * Q * s + 2 * d * N * s
     ach = bytes_ / (ms.value * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_apply3_sinker128_traffic.json")
+    if os.path.exists(tfile) and mesh.elementRes == (128, 128, 128):
+        with open(tfile) as f:
+            t = json.load(f)
+        traffic = t["fp32" if precision == 1 else "fp64"]["traffic_bytes"]
     return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
-            "traffic": None, "kernel": f"k_apply3<{'float' if precision else 'double'}>",
+            "traffic": traffic, "kernel": f"k_apply3<{'float' if precision else 'double'}>",
             "kernel_ms": ms.value, "algorithmic_bytes": bytes_}
 
 

@@ -384,3 +384,71 @@ def test_cpu_baseline_bsscr_matches_direct_solve():
     assert np.linalg.norm(u - o["u"]) < 1e-6 * np.linalg.norm(o["u"])
     pe = o["p"] - o["p"].mean()
     assert np.linalg.norm((p - p.mean()) - pe) < 1e-4 * np.linalg.norm(pe)
+
+
+# ---------------------------------------------------------------------------------------
+# A7 force vector, A16 interpolation, A18 time step: closed forms
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_element_force_closed_forms(dim):
+    """f_e = sum_p w_p detJ f(x_p) N_a(xi_p): with the 3^d Gauss points as particles it
+    integrates polynomial forces of degree <= 3 per direction exactly: a constant force gives
+    f vol / 2^d per node, and f = x gives (vol / 2^d) (x_a + 2 x_c) / 3 per node (x_c: centre)."""
+    h = np.array([0.3, 0.5, 0.7])[:dim]
+    lo = np.array([0.2, -0.1, 0.4])[:dim]
+    verts = np.array([[lo[d] + h[d] * ((a >> d) & 1) for d in range(dim)] for a in range(2 ** dim)])
+    g = np.sqrt(0.6)
+    pts1, w1 = np.array([-g, 0.0, g]), np.array([5.0, 8.0, 5.0]) / 9.0
+    grids = np.meshgrid(*([pts1] * dim), indexing="ij")
+    xi = np.stack([gg.ravel() for gg in grids[::-1]], 1)
+    wg = np.meshgrid(*([w1] * dim), indexing="ij")
+    w = np.prod(np.stack([ww.ravel() for ww in wg], 1), axis=1)
+    x = lo + 0.5 * (xi + 1.0) * h
+    vol = np.prod(h)
+    nn = 2 ** dim
+    f0 = np.array([1.5, -2.0, 0.25])[:dim]
+    fe = np.zeros(nn * dim)
+    O.lib().orc_element_f(O.ctypes.c_int(dim), O._d(np.ascontiguousarray(verts)), O.ctypes.c_int(len(w)),
+                          O._d(np.ascontiguousarray(xi)), O._d(np.ascontiguousarray(w)),
+                          O._d(np.ascontiguousarray(np.tile(f0, (len(w), 1)))), O._d(fe))
+    np.testing.assert_allclose(fe.reshape(nn, dim), np.tile(f0 * vol / nn, (nn, 1)), rtol=1e-13)
+    force = np.zeros((len(w), dim))
+    force[:, 0] = x[:, 0]
+    O.lib().orc_element_f(O.ctypes.c_int(dim), O._d(np.ascontiguousarray(verts)), O.ctypes.c_int(len(w)),
+                          O._d(np.ascontiguousarray(xi)), O._d(np.ascontiguousarray(w)),
+                          O._d(np.ascontiguousarray(force)), O._d(fe))
+    xc = lo[0] + 0.5 * h[0]
+    exp = vol / nn * (verts[:, 0] + 2.0 * xc) / 3.0
+    np.testing.assert_allclose(fe.reshape(nn, dim)[:, 0], exp, rtol=1e-13)
+    assert np.all(fe.reshape(nn, dim)[:, 1:] == 0.0)
+
+
+def test_interpolation_exact_for_trilinear_fields():
+    """Q1 interpolation reproduces any field that is trilinear on every element, here
+    a + b x + c y + d z + e xy + f yz + g xz + k xyz on a uniform mesh."""
+    m = O.Mesh((4, 3, 5), (0.0, -1.0, 0.5), (2.0, 1.0, 1.5))
+    X = m.coords()
+
+    def field(P):
+        x, y, z = P[:, 0], P[:, 1], P[:, 2]
+        return np.stack([1 + 2 * x - y + 0.5 * z + 0.3 * x * y - 0.2 * y * z + 0.1 * x * z + 0.05 * x * y * z,
+                         -x + 3 * y, 0.7 * x * y * z], 1)
+
+    vel = field(X)
+    rng = np.random.default_rng(3)
+    P = np.stack([rng.uniform(0, 2, 300), rng.uniform(-1, 1, 300), rng.uniform(0.5, 1.5, 300)], 1)
+    v, ok = O.interpolate(m, vel, P)
+    assert ok.all()
+    # the xyz terms are trilinear only per element; on a uniform grid every term above is
+    # a product of per-axis linear functions, hence exact
+    np.testing.assert_allclose(v, field(P), atol=1e-12)
+
+
+def test_max_dt_closed_form():
+    """dt = 0.5 minSep / max_nodes |v| (SwarmAdvector_MaxDt): minSep is the shortest element
+    edge; the fastest node sets the speed."""
+    m = O.Mesh((4, 5, 6), (0.0, 0.0, 0.0), (1.0, 2.0, 3.0))
+    vel = np.zeros((m.N, 3))
+    vel[:, 0] = 0.1
+    vel[17] = [3.0, 4.0, 0.0]
+    assert O.max_dt(m, vel) == pytest.approx(0.5 * 0.25 / 5.0, rel=1e-15)

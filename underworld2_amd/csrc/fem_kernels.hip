@@ -857,6 +857,85 @@ void launch_bc_mask_from_walls(const Geom& g, uint8_t* bc, int wallbits, hipStre
   UWG_HIP(hipGetLastError());
 }
 
+struct Grid3i {
+  int nx, ny, nz;
+};
+
+// coarse Dirichlet mask by injection: a coarse dof is constrained when its coincident fine
+// dof is (nested grids, fine node 2i)
+__global__ void k_inject_bc(Grid3i f, Grid3i c, long Nc, const uint8_t* __restrict__ fbc, uint8_t* __restrict__ cbc) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= Nc) return;
+  long i = n % c.nx, j = (n / c.nx) % c.ny, k = n / ((long)c.nx * c.ny);
+  cbc[n] = fbc[2 * i + (long)f.nx * (2 * j + (long)f.ny * 2 * k)];
+}
+
+void launch_inject_bc(const Geom& f, const Geom& c, const uint8_t* fbc, uint8_t* cbc, hipStream_t s) {
+  Grid3i gf{f.nn[0], f.nn[1], f.dim == 3 ? f.nn[2] : 1}, gc{c.nn[0], c.nn[1], c.dim == 3 ? c.nn[2] : 1};
+  hipLaunchKernelGGL(k_inject_bc, nblocks(c.N, 256), 256, 0, s, gf, gc, c.N, fbc, cbc);
+  UWG_HIP(hipGetLastError());
+}
+
+// Dirichlet helpers.  ubc = v on constrained dofs, 0 elsewhere; *flag = 1 if any constrained
+// value is nonzero (plain stores of the same value: no ordering needed).
+__global__ void k_bc_split(long N, int dim, const uint8_t* __restrict__ bc, const double* __restrict__ v,
+                           double* __restrict__ ubc, int* __restrict__ flag) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint8_t m = bc[i];
+  for (int d = 0; d < dim; d++) {
+    const long k = i * dim + d;
+    const bool c = (m >> d) & 1;
+    const double x = v[k];
+    ubc[k] = c ? x : 0.0;
+    if (c && x != 0.0) flag[0] = 1;
+  }
+}
+__global__ void k_zero_bc(long N, int dim, const uint8_t* __restrict__ bc, double* __restrict__ v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint8_t m = bc[i];
+  for (int d = 0; d < dim; d++)
+    if ((m >> d) & 1) v[i * dim + d] = 0.0;
+}
+template <typename T>
+__global__ void k_neg_recip(long n, const double* __restrict__ s, T* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = s[i] != 0.0 ? (T)(-1.0 / s[i]) : (T)0;
+}
+
+void launch_bc_split(const Geom& g, const uint8_t* bc, const double* v, double* ubc, int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_bc_split, nblocks(g.N, 256), 256, 0, s, g.N, g.dim, bc, v, ubc, flag);
+  UWG_HIP(hipGetLastError());
+}
+void launch_zero_bc(const Geom& g, const uint8_t* bc, double* v, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_bc, nblocks(g.N, 256), 256, 0, s, g.N, g.dim, bc, v);
+  UWG_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_neg_recip(long n, const double* sv, T* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_neg_recip<T>, nblocks(n, 256), 256, 0, s, n, sv, out);
+  UWG_HIP(hipGetLastError());
+}
+template void launch_neg_recip<double>(long, const double*, double*, hipStream_t);
+template void launch_neg_recip<float>(long, const double*, float*, hipStream_t);
+
+// deterministic positive start vector for power iterations: 0.25 + hash(i) / 2^16 in [0.25, 1.25)
+__global__ void k_fill_hash(long n, double* __restrict__ x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    x[i] = (double)(h & 0xffffu) / 65536.0 + 0.25;
+  }
+}
+
+void launch_fill_hash(long n, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_hash, nblocks(n < 2097152 ? n : 2097152, 256), 256, 0, s, n, x);
+  UWG_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------------------
 // vector kernels
 // ---------------------------------------------------------------------------------------

@@ -31,6 +31,17 @@ static T* dalloc(long n) {
 static void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
+
+// Per-handle workspace: persistent device buffers reused across solves (grown on demand), so
+// a solve performs no device allocation once the first solve has run.
+enum WsSlot {
+  WS_FU, WS_HP, WS_U, WS_P, WS_UBC_FLAG, WS_LIFT_T, WS_LIFT_Q, WS_FREEMASK,
+  WS_RES_FU, WS_RES_HP, WS_RES_UF, WS_RES_T, WS_RES_GP, WS_RES_Q,
+  WS_BLK_B, WS_BLK_X, WS_BLK_DS, WS_BLK_Q, WS_BLK_GP,
+  WS_SCH_USTAR, WS_SCH_HHAT, WS_SCH_T, WS_SCH_Z, WS_SCH_DS,
+  WS_MIX_B, WS_MIX_X, WS_MIX_R, WS_MIX_GP, WS_MIX_R32, WS_MIX_D32, WS_MIX_DS32, WS_MIX_Q32, WS_MIX_GP32,
+  WS_COUNT
+};
 static double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -148,7 +159,23 @@ struct uwg_stokes {
   int capture_kind = 0;          // 0: right-hand side reaching the level, 1: residual left by its cycle
   int capture_max = 0, captured = 0, capture_every = 1, capture_seen = 0;
   float* capture_buf = nullptr;
+  // persistent per-solve buffers (see WsSlot)
+  void* ws_ptr[WS_COUNT] = {};
+  size_t ws_size[WS_COUNT] = {};
 };
+
+template <typename T>
+static T* wsbuf(uwg_stokes* st, WsSlot slot, long n) {
+  const size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
+  if (st->ws_size[slot] < bytes) {
+    dfree(st->ws_ptr[slot]);
+    st->ws_ptr[slot] = nullptr;
+    st->ws_size[slot] = 0;
+    UWG_HIP(hipMalloc(&st->ws_ptr[slot], bytes));
+    st->ws_size[slot] = bytes;
+  }
+  return (T*)st->ws_ptr[slot];
+}
 
 // ---------------------------------------------------------------------------------------
 // error wrapper
@@ -789,13 +816,7 @@ static void build_coarse(uwg_stokes* st) {
 // power iteration for lambda_max(D^-1 K) on each level
 static void estimate_lmax(uwg_stokes* st, Level& A) {
   long n = A.n();
-  std::vector<double> h(n);
-  uint32_t s = 12345u;
-  for (long i = 0; i < n; i++) {
-    s = s * 1664525u + 1013904223u;
-    h[i] = ((s >> 8) & 0xffff) / 65536.0 + 0.25;
-  }
-  UWG_HIP(hipMemcpy(A.x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  launch_fill_hash(n, A.x, 0);
   double lam = 0.0;
   const double* ptr[1];
   for (int it = 0; it < 12; it++) {
@@ -867,18 +888,7 @@ static void setup_hierarchy(uwg_stokes* st) {
   }
   // coarse Dirichlet masks: a coarse dof is constrained when the coincident fine dof is
   // (nested walls); derived from level 0's mask by injection
-  for (size_t l = 1; l < st->L.size(); l++) {
-    Level& F = st->L[l - 1];
-    Level& C = st->L[l];
-    std::vector<uint8_t> fb(F.g.N), cb(C.g.N);
-    UWG_HIP(hipMemcpy(fb.data(), F.bc, F.g.N, hipMemcpyDeviceToHost));
-    for (long n = 0; n < C.g.N; n++) {
-      long i = n % C.g.nn[0], j = (n / C.g.nn[0]) % C.g.nn[1], k = n / ((long)C.g.nn[0] * C.g.nn[1]);
-      long fn = 2 * i + (long)F.g.nn[0] * (2 * j + (long)F.g.nn[1] * 2 * k);
-      cb[n] = fb[fn];
-    }
-    UWG_HIP(hipMemcpy(C.bc, cb.data(), C.g.N, hipMemcpyHostToDevice));
-  }
+  for (size_t l = 1; l < st->L.size(); l++) launch_inject_bc(st->L[l - 1].g, st->L[l].g, st->L[l - 1].bc, st->L[l].bc, 0);
   for (auto& A : st->L) launch_diag(A.g, A.W, A.bc, A.dinv, 0);
   for (auto& A : st->L) estimate_lmax(st, A);
   build_coarse(st);
@@ -1218,6 +1228,7 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
   delete st->kb;
   delete st->kf;
   dfree(st->capture_buf);
+  for (int k = 0; k < WS_COUNT; k++) dfree(st->ws_ptr[k]);
   for (void* p : {(void*)st->fe, (void*)st->f, (void*)st->ubc, (void*)st->approxS, (void*)st->h, (void*)st->eta_tab,
                   (void*)st->force_tab, (void*)st->eta_p, (void*)st->force_p, (void*)st->red_partial,
                   (void*)st->red_out, (void*)st->dcoef, (void*)st->dptrs, (void*)st->coarse.Ainv})
@@ -1382,44 +1393,25 @@ static void lifted_rhs(uwg_stokes* st, double* fu, double* hp, bool* has_bc) {
   const long n = A.n();
   const Geom& g = st->g;
   // ubc = vel on constrained dofs, 0 elsewhere
-  std::vector<double> v(n);
-  std::vector<uint8_t> bc(g.N);
-  UWG_HIP(hipMemcpy(v.data(), st->vel->d, sizeof(double) * n, hipMemcpyDeviceToHost));
-  UWG_HIP(hipMemcpy(bc.data(), A.bc, g.N, hipMemcpyDeviceToHost));
-  bool any = false;
-  for (long i = 0; i < g.N; i++)
-    for (int d = 0; d < g.dim; d++) {
-      long k = i * g.dim + d;
-      if ((bc[i] >> d) & 1) {
-        if (v[k] != 0.0) any = true;
-      } else {
-        v[k] = 0.0;
-      }
-    }
-  *has_bc = any;
+  int* flag = wsbuf<int>(st, WS_UBC_FLAG, 1);
+  UWG_HIP(hipMemsetAsync(flag, 0, sizeof(int), 0));
+  launch_bc_split(g, A.bc, st->vel->d, st->ubc, flag, 0);
+  int any = 0;
+  UWG_HIP(hipMemcpy(&any, flag, sizeof(int), hipMemcpyDeviceToHost));
+  *has_bc = any != 0;
   vk_copy<double>(n, st->f, fu, 0);
   vk_copy<double>(g.E, st->h, hp, 0);
   if (!any) return;
-  UWG_HIP(hipMemcpy(st->ubc, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  // K applied to ubc with BC dofs treated as free: use an all-free mask temporarily
-  uint8_t* free_mask = dalloc<uint8_t>(g.N);
-  UWG_HIP(hipMemset(free_mask, 0, g.N));
-  double* t = dalloc<double>(n);
+  // K applied to ubc with BC dofs treated as free (an all-free mask), BC rows then zeroed
+  uint8_t* free_mask = wsbuf<uint8_t>(st, WS_FREEMASK, g.N);
+  UWG_HIP(hipMemsetAsync(free_mask, 0, g.N, 0));
+  double* t = wsbuf<double>(st, WS_LIFT_T, n);
   launch_apply_K<double>(g, A.W, st->ubc, t, free_mask, 0);
-  // zero BC rows of t then fu -= t
-  std::vector<double> th(n);
-  UWG_HIP(hipMemcpy(th.data(), t, sizeof(double) * n, hipMemcpyDeviceToHost));
-  for (long i = 0; i < g.N; i++)
-    for (int d = 0; d < g.dim; d++)
-      if ((bc[i] >> d) & 1) th[i * g.dim + d] = 0.0;
-  UWG_HIP(hipMemcpy(t, th.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  launch_zero_bc(g, A.bc, t, 0);
   vk_axpy<double>(n, -1.0, t, fu, 0);
-  double* q = dalloc<double>(g.E);
+  double* q = wsbuf<double>(st, WS_LIFT_Q, g.E);
   launch_apply_GT(g, st->Ge, st->ubc, free_mask, q, 0);
   vk_axpy<double>(g.E, -1.0, q, hp, 0);
-  dfree(q);
-  dfree(t);
-  dfree(free_mask);
 }
 
 static void true_residuals(uwg_stokes* st, const double* u, const double* p, double* out3) {
@@ -1428,33 +1420,25 @@ static void true_residuals(uwg_stokes* st, const double* u, const double* p, dou
   const long n = A.n();
   const Geom& g = st->g;
   bool has_bc;
-  double* fu = dalloc<double>(n);
-  double* hp = dalloc<double>(g.E);
+  double* fu = wsbuf<double>(st, WS_RES_FU, n);
+  double* hp = wsbuf<double>(st, WS_RES_HP, g.E);
   lifted_rhs(st, fu, hp, &has_bc);
   // uf = u with BC dofs zeroed (K with identity BC rows maps those to themselves)
-  double* uf = dalloc<double>(n);
+  double* uf = wsbuf<double>(st, WS_RES_UF, n);
   vk_copy<double>(n, u, uf, 0);
-  std::vector<double> h(n);
-  std::vector<uint8_t> bc(g.N);
-  UWG_HIP(hipMemcpy(h.data(), uf, sizeof(double) * n, hipMemcpyDeviceToHost));
-  UWG_HIP(hipMemcpy(bc.data(), A.bc, g.N, hipMemcpyDeviceToHost));
-  for (long i = 0; i < g.N; i++)
-    for (int d = 0; d < g.dim; d++)
-      if ((bc[i] >> d) & 1) h[i * g.dim + d] = 0.0;
-  UWG_HIP(hipMemcpy(uf, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  double* t = dalloc<double>(n);
-  double* gp = dalloc<double>(n);
+  launch_zero_bc(g, A.bc, uf, 0);
+  double* t = wsbuf<double>(st, WS_RES_T, n);
+  double* gp = wsbuf<double>(st, WS_RES_GP, n);
   launch_apply_K<double>(g, A.W, uf, t, A.bc, 0);
   launch_apply_G(g, st->Ge, p, A.bc, gp, 0);
   vk_axpy<double>(n, 1.0, gp, t, 0);
   vk_axpby<double>(n, 1.0, fu, -1.0, t, 0);  // t = fu - K uf - G p  (BC rows: fu=0, K uf = 0)
-  double* q = dalloc<double>(g.E);
+  double* q = wsbuf<double>(st, WS_RES_Q, g.E);
   launch_apply_GT(g, st->Ge, uf, A.bc, q, 0);
   vk_axpby<double>(g.E, 1.0, hp, -1.0, q, 0);
   out3[0] = sqrt(dev_dot(st, n, t, t));
   out3[1] = sqrt(dev_dot(st, g.E, q, q));
   out3[2] = sqrt(dev_dot(st, n, fu, fu));
-  for (void* pp : {(void*)fu, (void*)hp, (void*)uf, (void*)t, (void*)gp, (void*)q}) dfree(pp);
 }
 
 extern "C" int uwg_stokes_residuals(uwg_stokes* st, double* out3) {
@@ -1497,11 +1481,11 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
     int rc = fgmres(st, Kin, Kop, Mv, rhs, x, o.inner_rtol, 1e-300, o.max_inner_its, its, &rn, true);
     if (rc) status = UWG_WARN_NOT_CONVERGED;
   };
-  double* ustar = dalloc<double>(n);
-  double* hhat = dalloc<double>(E);
-  double* t = dalloc<double>(n);
-  double* z = dalloc<double>(n);
-  double* diagS = dalloc<double>(E);
+  double* ustar = wsbuf<double>(st, WS_SCH_USTAR, n);
+  double* hhat = wsbuf<double>(st, WS_SCH_HHAT, E);
+  double* t = wsbuf<double>(st, WS_SCH_T, n);
+  double* z = wsbuf<double>(st, WS_SCH_Z, n);
+  double* diagS = wsbuf<double>(st, WS_SCH_DS, E);
   int its = 0;
   inner(fu, ustar, &its);
   S->velocity_presolve_its = its;
@@ -1517,12 +1501,7 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
     vk_scale<double>(E, -1.0, y, y, 0);  // S v = -G^T K^-1 G v
   };
   // preconditioner: -approxS^-1 (approxS approximates G^T diag(K)^-1 G > 0)
-  {
-    std::vector<double> s(E);
-    UWG_HIP(hipMemcpy(s.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
-    for (auto& v : s) v = v != 0.0 ? -1.0 / v : 0.0;
-    UWG_HIP(hipMemcpy(diagS, s.data(), sizeof(double) * E, hipMemcpyHostToDevice));
-  }
+  launch_neg_recip<double>(E, st->approxS, diagS, 0);
   OpFn Sprec = [&](const double* x, double* y) { vk_xmy_scale<double>(E, diagS, x, y, 0); };
   vk_fill<double>(E, 0.0, p, 0);
   double rn;
@@ -1537,7 +1516,6 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   inner(t, u, &its);
   S->velocity_backsolve_its = its;
   S->velocity_total_its = S->velocity_presolve_its + S->velocity_pressuresolve_its + S->velocity_backsolve_its;
-  for (void* pp : {(void*)ustar, (void*)hhat, (void*)t, (void*)z, (void*)diagS}) dfree(pp);
   return status;
 }
 
@@ -1550,20 +1528,15 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   if (!st->kb) st->kb = new KrylovSpace();
   KrylovSpace& Kb = *st->kb;
   Kb.init(nt, o.restart);
-  double* b = dalloc<double>(nt);
-  double* x = dalloc<double>(nt);
-  double* diagS = dalloc<double>(E);
-  double* q = dalloc<double>(E);
+  double* b = wsbuf<double>(st, WS_BLK_B, nt);
+  double* x = wsbuf<double>(st, WS_BLK_X, nt);
+  double* diagS = wsbuf<double>(st, WS_BLK_DS, E);
+  double* q = wsbuf<double>(st, WS_BLK_Q, E);
   vk_copy<double>(n, fu, b, 0);
   vk_copy<double>(E, hp, b + n, 0);
   vk_fill<double>(nt, 0.0, x, 0);
-  {
-    std::vector<double> s(E);
-    UWG_HIP(hipMemcpy(s.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
-    for (auto& v : s) v = v != 0.0 ? -1.0 / v : 0.0;
-    UWG_HIP(hipMemcpy(diagS, s.data(), sizeof(double) * E, hipMemcpyHostToDevice));
-  }
-  double* gp = dalloc<double>(n);
+  launch_neg_recip<double>(E, st->approxS, diagS, 0);
+  double* gp = wsbuf<double>(st, WS_BLK_GP, n);
   OpFn Aop = [&](const double* xx, double* yy) {
     launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
     launch_apply_G(st->g, st->Ge, xx + n, A.bc, gp, 0);
@@ -1603,7 +1576,6 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   S->outer_its = its;
   vk_copy<double>(n, x, u, 0);
   vk_copy<double>(E, x + n, p, 0);
-  for (void* pp : {(void*)b, (void*)x, (void*)diagS, (void*)q, (void*)gp}) dfree(pp);
   return rc ? UWG_WARN_NOT_CONVERGED : 0;
 }
 
@@ -1621,25 +1593,19 @@ static int solve_block_mixed(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_
   if (!st->kf) st->kf = new KSpaceT<float>();
   KSpaceT<float>& Kf = *st->kf;
   Kf.init(nt, o.restart);
-  double* b = dalloc<double>(nt);
-  double* x = dalloc<double>(nt);
-  double* r = dalloc<double>(nt);
-  double* gp = dalloc<double>(n);
-  float* r32 = dalloc<float>(nt);
-  float* d32 = dalloc<float>(nt);
-  float* diagS32 = dalloc<float>(E);
-  float* q32 = dalloc<float>(E);
-  float* gp32 = dalloc<float>(n);
+  double* b = wsbuf<double>(st, WS_MIX_B, nt);
+  double* x = wsbuf<double>(st, WS_MIX_X, nt);
+  double* r = wsbuf<double>(st, WS_MIX_R, nt);
+  double* gp = wsbuf<double>(st, WS_MIX_GP, n);
+  float* r32 = wsbuf<float>(st, WS_MIX_R32, nt);
+  float* d32 = wsbuf<float>(st, WS_MIX_D32, nt);
+  float* diagS32 = wsbuf<float>(st, WS_MIX_DS32, E);
+  float* q32 = wsbuf<float>(st, WS_MIX_Q32, E);
+  float* gp32 = wsbuf<float>(st, WS_MIX_GP32, n);
   vk_copy<double>(n, fu, b, 0);
   vk_copy<double>(E, hp, b + n, 0);
   vk_fill<double>(nt, 0.0, x, 0);
-  {
-    std::vector<double> sd(E);
-    std::vector<float> sf(E);
-    UWG_HIP(hipMemcpy(sd.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
-    for (long e = 0; e < E; e++) sf[e] = sd[e] != 0.0 ? (float)(-1.0 / sd[e]) : 0.0f;
-    UWG_HIP(hipMemcpy(diagS32, sf.data(), sizeof(float) * E, hipMemcpyHostToDevice));
-  }
+  launch_neg_recip<float>(E, st->approxS, diagS32, 0);
   auto A64 = [&](const double* xx, double* yy) {
     launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
     launch_apply_G<double>(st->g, st->Ge, xx + n, A.bc, gp, 0);
@@ -1697,9 +1663,6 @@ static int solve_block_mixed(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_
   S->pressure_its = sweeps;  // refinement sweeps
   vk_copy<double>(n, x, u, 0);
   vk_copy<double>(E, x + n, p, 0);
-  for (void* pp : {(void*)b, (void*)x, (void*)r, (void*)gp, (void*)r32, (void*)d32, (void*)diagS32, (void*)q32,
-                   (void*)gp32})
-    dfree(pp);
   if (rn > o.outer_rtol * bn) status = UWG_WARN_NOT_CONVERGED;
   return status;
 }
@@ -1742,11 +1705,11 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   Level& A = st->L[0];
   const long n = A.n(), E = st->g.E;
   bool has_bc;
-  double* fu = dalloc<double>(n);
-  double* hp = dalloc<double>(E);
+  double* fu = wsbuf<double>(st, WS_FU, n);
+  double* hp = wsbuf<double>(st, WS_HP, E);
   lifted_rhs(st, fu, hp, &has_bc);
-  double* u = dalloc<double>(n);
-  double* p = dalloc<double>(E);
+  double* u = wsbuf<double>(st, WS_U, n);
+  double* p = wsbuf<double>(st, WS_P, E);
   int status;
   if (o.method == 0)
     status = solve_schur(st, o, S, fu, hp, u, p);
@@ -1774,10 +1737,6 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   S->setup_time = t1 - t0;
   S->solve_time = t2 - t1;
   S->total_time = t2 - t0;
-  dfree(fu);
-  dfree(hp);
-  dfree(u);
-  dfree(p);
   if (!std::isfinite(r3[0]) || !std::isfinite(r3[1])) {
     set_error("A floating-point error has been detected during the solve.");
     return UWG_WARN_NONFINITE;

@@ -106,6 +106,11 @@ void reloc_destroy(RelocWork* w);
 long relocate(const Geom& g, long P, const double* coords, int* owner, int* cell_ptr, int* cell_pid,
               RelocWork* w, hipStream_t s);
 void launch_bc_mask_from_walls(const Geom& g, uint8_t* bc, int wallbits, hipStream_t s);
+void launch_inject_bc(const Geom& f, const Geom& c, const uint8_t* fbc, uint8_t* cbc, hipStream_t s);
+void launch_fill_hash(long n, double* x, hipStream_t s);
+void launch_bc_split(const Geom& g, const uint8_t* bc, const double* v, double* ubc, int* flag, hipStream_t s);
+void launch_zero_bc(const Geom& g, const uint8_t* bc, double* v, hipStream_t s);
+template <typename T> void launch_neg_recip(long n, const double* sv, T* out, hipStream_t s);
 
 // vector kernels (T = double or float)
 template <typename T> void vk_fill(long n, T v, T* x, hipStream_t s);
