@@ -117,6 +117,68 @@ __global__ __launch_bounds__(256) void k_conv_fwd_blk(int B, int Ci, int Co, Gri
   }
 }
 
+// Split-reduction forward for inference-sized batches: a 256-thread block computes COB output
+// channels of 16 nodes; the Ci*27-term reduction is split 16 ways across the block (input
+// channel ci goes to split ci % 16) and the partial sums are added in LDS in split order
+// (deterministic).  The per-thread serial chain is Ci/16*27 loads instead of Ci*27, which is
+// what bounds the small grids of the coarse-correction network.
+template <int COB>
+__global__ __launch_bounds__(256) void k_conv_fwd_ks(int B, int Ci, int Co, Grid3 g, const float* __restrict__ in,
+                                                     const float* __restrict__ w, const float* __restrict__ bias,
+                                                     float* __restrict__ out, int relu) {
+  __shared__ float part[COB][16][17];
+  const long N = g.N();
+  const int tid = threadIdx.x, nl = tid & 15, sp = tid >> 4;
+  const long t = (long)blockIdx.x * 16 + nl;  // (batch, node) index
+  const int co0 = blockIdx.y * COB;  // the launch guarantees co0 + COB <= Co
+  float acc[COB];
+#pragma unroll
+  for (int c = 0; c < COB; c++) acc[c] = 0.0f;
+  if (t < (long)B * N) {
+    const long p = t % N;
+    const int b = (int)(t / N);
+    const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
+    const long cs = (long)Ci * 27;
+    for (int ci = sp; ci < Ci; ci += 16) {
+      const float* src = in + ((long)b * Ci + ci) * N;
+      const float* wc = w + (long)co0 * cs + ci * 27;
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++) {
+        const int kk = k + dz;
+        const bool okz = kk >= 0 && kk < g.nz;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+          const int jj = j + dy;
+          const bool oky = okz && jj >= 0 && jj < g.ny;
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++) {
+            const int ii = i + dx;
+            const float v = (oky && ii >= 0 && ii < g.nx) ? src[ii + (long)g.nx * (jj + (long)g.ny * kk)] : 0.0f;
+            const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+#pragma unroll
+            for (int c = 0; c < COB; c++) acc[c] += wc[c * cs + tap] * v;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < COB; c++) part[c][sp][nl] = acc[c];
+  __syncthreads();
+  if (tid < COB * 16) {
+    const int c = tid >> 4, n2 = tid & 15;
+    const long t2 = (long)blockIdx.x * 16 + n2;
+    if (t2 < (long)B * N) {
+      float s = bias[co0 + c];
+#pragma unroll
+      for (int q = 0; q < 16; q++) s += part[c][q][n2];
+      const long p = t2 % N;
+      const int b = (int)(t2 / N);
+      out[((long)b * Co + co0 + c) * N + p] = relu ? fmaxf(s, 0.0f) : s;
+    }
+  }
+}
+
 // Channel-blocked input gradient: each thread produces CIB input channels of one node.
 template <int CIB>
 __global__ __launch_bounds__(256) void k_conv_bwd_data_blk(int B, int Ci, int Co, Grid3 g,
@@ -489,7 +551,13 @@ static void conv_fwd(NNModel* m, int c, int B, const float* in, float* out, bool
   const long nb = (long)B * g.N();
   const float* w = m->params + cs.woff;
   const float* bias = m->params + cs.boff;
-  if (cs.co % 4 == 0)
+  if (cs.co % 4 == 0 && nb <= 65536)
+    hipLaunchKernelGGL(k_conv_fwd_ks<4>, dim3(nbk(nb, 16), cs.co / 4), 256, 0, s, B, cs.ci, cs.co, g, in, w, bias, out,
+                       relu ? 1 : 0);
+  else if (cs.co == 3 && nb <= 65536)
+    hipLaunchKernelGGL(k_conv_fwd_ks<3>, dim3(nbk(nb, 16), 1), 256, 0, s, B, cs.ci, cs.co, g, in, w, bias, out,
+                       relu ? 1 : 0);
+  else if (cs.co % 4 == 0)
     hipLaunchKernelGGL(k_conv_fwd_blk<4>, dim3(nbk(nb, 256), cs.co / 4), 256, 0, s, B, cs.ci, cs.co, g, in, w, bias,
                        out, relu ? 1 : 0);
   else if (cs.co == 3)

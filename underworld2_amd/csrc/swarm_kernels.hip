@@ -312,6 +312,236 @@ __global__ void k_dvc(int num, long E0, long nE, const int* elist, const int* ce
   flag[t] = 0;
 }
 
+// One wavefront per element, grid-stride over elements.  The claim grid, a one-bit-per-cell
+// mark and the particles' local coordinates live in LDS (about 8 KB, so many elements are in
+// flight per CU); each particle's boundary chain lives in a per-workgroup global buffer
+// (double-buffered) that the 64 lanes read and write contiguously.
+//
+// Why the lanes may work on a chain at once: within one particle's step every chain cell is
+// examined once (the chain holds no duplicates) and its decision depends only on ownership
+// before the step, so the set of cells the particle claims does not depend on the order of its
+// chain; the rebuilt chain is the set of neighbours of those cells not owned by the particle,
+// which does not depend on order either.  Ownership after every step - and therefore the
+// weights and centroids, summed in grid order at the end - is exactly that of the sequential
+// algorithm above, bit for bit; only the order of cells inside a chain may differ.  Duplicate
+// neighbours are dropped by an atomic test-and-set on the mark bits.  Elements with more than
+// DVCW_MAXNP particles or a chain longer than `cap` are flagged and left to k_dvc.
+#define DVCW_MAXNC 4096
+#define DVCW_MAXNP 128
+template <int DIM, int NUM>
+__global__ __launch_bounds__(64) void k_dvc_wave(int num_rt, long E, const int* __restrict__ cell_ptr,
+                                                 double* __restrict__ xi_slot, double* __restrict__ w_slot,
+                                                 short* __restrict__ gbound, int cap, int* __restrict__ flag) {
+  const int num = NUM > 0 ? NUM : num_rt;  // compile-time grid size for the default resolution
+  __shared__ unsigned char own[DVCW_MAXNC];
+  __shared__ unsigned int mark[DVCW_MAXNC / 32];
+  __shared__ double ppos[DVCW_MAXNP * DIM];
+  __shared__ int nbound[DVCW_MAXNP];
+  __shared__ unsigned char which[DVCW_MAXNP];
+  __shared__ double cen[3][32];
+  const int lane = threadIdx.x;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int nx = num, ny = num, nz = DIM == 3 ? num : 1;
+  const int nxy = nx * ny, ncell = nxy * nz;
+  const int nnb = DIM == 3 ? 6 : 4;
+  const double dx = (1.0 - -1.0) / nx, dy = (1.0 - -1.0) / ny, dz = (1.0 - -1.0) / nz;
+  if (lane == 0) {
+    // cell centres: X = X + dx accumulation per axis (_DVCWeights_ConstructGrid :527-544)
+    double X = -1.0 - dx / 2.0;
+    for (int i = 0; i < nx; i++) { X = X + dx; cen[0][i] = X; }
+    double Y = -1.0 - dy / 2.0;
+    for (int j = 0; j < ny; j++) { Y = Y + dy; cen[1][j] = Y; }
+    double Z = -1.0 - dz / 2.0;
+    for (int k = 0; k < nz; k++) { Z = Z + dz; cen[2][k] = Z; }
+  }
+  for (int q = lane; q < DVCW_MAXNC / 32; q += 64) mark[q] = 0u;
+  // chains: buffer h (0/1) of particle p at gb + (2 p + h) * cap
+  short* gb = gbound + (long)blockIdx.x * 2 * DVCW_MAXNP * cap;
+  // neighbour k of cell c in the order S, N, E, W, U, D; -2 outside the grid
+  auto neighbour = [&](int c, int k) -> int {
+    const int i = c % nx, j = (c / nx) % ny, kk = c / nxy;
+    switch (k) {
+      case 0: return j > 0 ? c - nx : -2;
+      case 1: return j < ny - 1 ? c + nx : -2;
+      case 2: return i < nx - 1 ? c + 1 : -2;
+      case 3: return i > 0 ? c - 1 : -2;
+      case 4: return kk < nz - 1 ? c + nxy : -2;
+      default: return kk > 0 ? c - nxy : -2;
+    }
+  };
+  for (long e = blockIdx.x; e < E; e += gridDim.x) {
+    const int s0 = cell_ptr[e];
+    const int np = cell_ptr[e + 1] - s0;
+    if (np <= 0 || np > DVCW_MAXNP) {
+      if (lane == 0) flag[e] = np <= 0 ? 2 : 1;
+      continue;
+    }
+    for (int c = lane; c < ncell; c += 64) own[c] = 0xFF;
+    for (int i = lane; i < np * DIM; i += 64) ppos[i] = xi_slot[(long)s0 * DIM + i];
+    for (int i = lane; i < np; i += 64) which[i] = 0;
+    __syncthreads();
+    bool overflow = false;
+    // rebuild the chain of particle pi from the ncl claimed cells in `cl`
+    auto update = [&](int pi, const short* cl, int ncl) {
+      short* nxt = gb + (long)(2 * pi + (which[pi] ^ 1)) * cap;
+      const int ncand = ncl * nnb;
+      int nb = 0;
+      for (int base = 0; base < ncand; base += 64) {
+        const int t = base + lane;
+        bool keep = false;
+        int c1 = -2;
+        if (t < ncand) {
+          const int q = t / nnb, k = t - q * nnb;
+          c1 = neighbour(cl[q], k);
+          if (c1 != -2 && own[c1] != pi) {
+            const unsigned int bit = 1u << (c1 & 31);
+            keep = (atomicOr(&mark[c1 >> 5], bit) & bit) == 0u;
+          }
+        }
+        const unsigned long long m = __ballot(keep);
+        const int rank = __popcll(m & lt);
+        const int cntk = __popcll(m);
+        if (nb + cntk > cap) {
+          overflow = true;
+          break;
+        }
+        if (keep) nxt[nb + rank] = (short)c1;
+        nb += cntk;
+      }
+      __syncthreads();
+      // every set mark bit belongs to a cell of the new chain: clear those words
+      for (int q = lane; q < nb; q += 64) mark[nxt[q] >> 5] = 0u;
+      if (lane == 0) {
+        nbound[pi] = nb;
+        which[pi] ^= 1;
+      }
+      __syncthreads();
+    };
+    // seeds, in particle order (_DVCWeights_CreateVoronoi3D :676-721)
+    for (int i = 0; i < np && !overflow; i++) {
+      short* seed = gb + (long)(2 * i + 0) * cap;  // the seed cell, read by update()
+      if (lane == 0) {
+        int k = (int)((ppos[i * DIM + 0] - -1.0) / dx);
+        int j = (int)((ppos[i * DIM + 1] - -1.0) / dy);
+        int l = DIM == 3 ? (int)((ppos[i * DIM + 2] - -1.0) / dz) : 0;
+        if (k == nx) k--;
+        if (j == ny) j--;
+        if (DIM == 3 && l == nz) l--;
+        k = k < 0 ? 0 : (k > nx - 1 ? nx - 1 : k);
+        j = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+        l = l < 0 ? 0 : (l > nz - 1 ? nz - 1 : l);
+        const int id = k + j * nx + l * nxy;
+        own[id] = (unsigned char)i;
+        seed[0] = (short)id;
+      }
+      __syncthreads();
+      update(i, seed, 1);
+    }
+    // rounds of claims over the particles until nothing changes (_DVCWeights_ClaimCells :256-311)
+    int claimed_total = 1;
+    while (claimed_total != 0 && !overflow) {
+      claimed_total = 0;
+      for (int pi = 0; pi < np && !overflow; pi++) {
+        const int nb = nbound[pi];
+        if (nb == 0) continue;  // an empty chain claims nothing and stays empty
+        short* b = gb + (long)(2 * pi + which[pi]) * cap;
+        int ncl = 0;
+        for (int base = 0; base < nb; base += 64) {
+          const int q = base + lane;
+          bool take = false;
+          int c0 = 0;
+          if (q < nb) {
+            c0 = b[q];
+            const int o = own[c0];
+            if (o == 0xFF) {
+              take = true;
+            } else if (o != pi) {
+              const int ci = c0 % nx, cj = (c0 / nx) % ny, ck = c0 / nxy;
+              const double x0 = cen[0][ci], y0 = cen[1][cj], z0 = cen[2][ck];
+              const double* p1 = ppos + o * DIM;
+              const double* p2 = ppos + pi * DIM;
+              double r = (p1[0] + p2[0] - x0 - x0) * (p1[0] - p2[0]) + (p1[1] + p2[1] - y0 - y0) * (p1[1] - p2[1]);
+              if (DIM == 3) r = r + (p1[2] + p2[2] - z0 - z0) * (p1[2] - p2[2]);
+              take = r > 0.0;
+            }
+          }
+          const unsigned long long m = __ballot(take);
+          const int rank = __popcll(m & lt);
+          if (take) {
+            own[c0] = (unsigned char)pi;
+            b[ncl + rank] = (short)c0;  // claimed cells compacted in place (ncl + rank <= q)
+          }
+          ncl += __popcll(m);
+        }
+        claimed_total += ncl;
+        if (ncl == 0) {
+          // nothing claimed: the rebuilt chain is empty
+          if (lane == 0) nbound[pi] = 0;
+          __syncthreads();
+          continue;
+        }
+        __syncthreads();
+        update(pi, b, ncl);
+      }
+    }
+    if (overflow) {
+      // leave LDS marks clean for the next element
+      for (int q = lane; q < DVCW_MAXNC / 32; q += 64) mark[q] = 0u;
+      if (lane == 0) flag[e] = 1;
+      __syncthreads();
+      continue;
+    }
+    // centroids and weights, grid-index order (_DVCWeights_GetCentroids3D :198-226)
+    const double da = DIM == 3 ? dx * dy * dz : dx * dy;
+    for (int p = lane; p < np; p += 64) {
+      double sx = 0.0, sy = 0.0, sz = 0.0;
+      int cnt = 0;
+      for (int c = 0; c < ncell; c++) {
+        if (own[c] != p) continue;
+        const int ci = c % nx, cj = (c / nx) % ny, ck = c / nxy;
+        sx += cen[0][ci];
+        sy += cen[1][cj];
+        if (DIM == 3) sz += cen[2][ck];
+        cnt++;
+      }
+      double* out = xi_slot + (long)(s0 + p) * DIM;
+      w_slot[s0 + p] = cnt * da;
+      if (cnt != 0) {
+        out[0] = sx / cnt;
+        out[1] = sy / cnt;
+        if (DIM == 3) out[2] = sz / cnt;
+      } else {
+        out[0] = 0.0;
+        out[1] = 0.0;
+        if (DIM == 3) out[2] = 0.0;
+      }
+    }
+    if (lane == 0) flag[e] = 0;
+    __syncthreads();
+  }
+}
+
+bool dvc_wave_supported(int dim, int num) {
+  const long nc = (long)num * num * (dim == 3 ? num : 1);
+  return nc <= DVCW_MAXNC && num <= 32;
+}
+
+void launch_dvc_wave(int dim, int num, long E, const int* cell_ptr, double* xi_slot, double* w_slot, short* gbound,
+                     int nblk, int cap, int* flag, hipStream_t s) {
+  if (E == 0) return;
+  if (dim == 3 && num == 15)
+    hipLaunchKernelGGL((k_dvc_wave<3, 15>), nblk, 64, 0, s, num, E, cell_ptr, xi_slot, w_slot, gbound, cap, flag);
+  else if (dim == 3)
+    hipLaunchKernelGGL((k_dvc_wave<3, 0>), nblk, 64, 0, s, num, E, cell_ptr, xi_slot, w_slot, gbound, cap, flag);
+  else if (num == 15)
+    hipLaunchKernelGGL((k_dvc_wave<2, 15>), nblk, 64, 0, s, num, E, cell_ptr, xi_slot, w_slot, gbound, cap, flag);
+  else
+    hipLaunchKernelGGL((k_dvc_wave<2, 0>), nblk, 64, 0, s, num, E, cell_ptr, xi_slot, w_slot, gbound, cap, flag);
+  UWG_HIP(hipGetLastError());
+}
+
+long dvc_wave_bound_shorts(int cap) { return 2L * DVCW_MAXNP * cap; }
+
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap) {
   int ncell = num * num * (dim == 3 ? num : 1);
   return dvc_layout(ncell, max_np, cap).bytes;

@@ -477,22 +477,43 @@ static int run_dvc(uwg_swarm* s, int dvc_res) {
     maxc = std::max(maxc, c);
   }
   const int ncell = dvc_res * dvc_res * (g.dim == 3 ? dvc_res : 1);
-  int max_np = std::min(std::max(64, ((maxc + 15) / 16) * 16), 32000);
-  int cap = std::min(512, ncell);
-  long per = dvc_scratch_bytes(g.dim, dvc_res, max_np, cap);
-  long batch = std::max(1024L, std::min(E, (long)(3.0e9 / per)));
-  char* scratch = dalloc<char>(per * batch);
-  int* flag = dalloc<int>(batch);
-  std::vector<int> hflag(batch), retry;
-  for (long e0 = 0; e0 < E; e0 += batch) {
-    long nE = std::min(batch, E - e0);
-    launch_dvc_list(g.dim, dvc_res, e0, nE, nullptr, s->cell_ptr, s->xi_slot, s->w_slot, scratch, max_np, cap, flag, 0);
-    UWG_HIP(hipMemcpy(hflag.data(), flag, sizeof(int) * nE, hipMemcpyDeviceToHost));
-    for (long t = 0; t < nE; t++)
-      if (hflag[t]) retry.push_back((int)(e0 + t));
+  std::vector<int> retry;
+  if (dvc_wave_supported(g.dim, dvc_res)) {
+    // one wavefront per element; elements it cannot hold are flagged for the per-thread kernel
+    hipDeviceProp_t prop;
+    int dev = 0;
+    UWG_HIP(hipGetDevice(&dev));
+    UWG_HIP(hipGetDeviceProperties(&prop, dev));
+    const int cap = std::min(512, ncell);
+    const int nblk = (int)std::min<long>(E, (long)prop.multiProcessorCount * 16);
+    short* gb = dalloc<short>(dvc_wave_bound_shorts(cap) * nblk);
+    int* flag = dalloc<int>(E);
+    launch_dvc_wave(g.dim, dvc_res, E, s->cell_ptr, s->xi_slot, s->w_slot, gb, nblk, cap, flag, 0);
+    std::vector<int> hflag(E);
+    UWG_HIP(hipMemcpy(hflag.data(), flag, sizeof(int) * E, hipMemcpyDeviceToHost));
+    for (long e = 0; e < E; e++)
+      if (hflag[e]) retry.push_back((int)e);
+    dfree(gb);
+    dfree(flag);
+  } else {
+    int max_np = std::min(std::max(64, ((maxc + 15) / 16) * 16), 32000);
+    int cap = std::min(512, ncell);
+    long per = dvc_scratch_bytes(g.dim, dvc_res, max_np, cap);
+    long batch = std::max(1024L, std::min(E, (long)(3.0e9 / per)));
+    char* scratch = dalloc<char>(per * batch);
+    int* flag = dalloc<int>(batch);
+    std::vector<int> hflag(batch);
+    for (long e0 = 0; e0 < E; e0 += batch) {
+      long nE = std::min(batch, E - e0);
+      launch_dvc_list(g.dim, dvc_res, e0, nE, nullptr, s->cell_ptr, s->xi_slot, s->w_slot, scratch, max_np, cap,
+                      flag, 0);
+      UWG_HIP(hipMemcpy(hflag.data(), flag, sizeof(int) * nE, hipMemcpyDeviceToHost));
+      for (long t = 0; t < nE; t++)
+        if (hflag[t]) retry.push_back((int)(e0 + t));
+    }
+    dfree(scratch);
+    dfree(flag);
   }
-  dfree(scratch);
-  dfree(flag);
   if (!retry.empty()) {
     // larger lists for the few elements that overflowed
     int cap2 = ncell, np2 = std::max(maxc, 1);

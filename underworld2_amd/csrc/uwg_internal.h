@@ -95,6 +95,10 @@ void launch_restrict2(const Geom& fine, const Geom& coarse, const T* rf, T* rc, 
                       const uint8_t* bcf, hipStream_t s);
 void set_apply_chunk(int kz3, int kz2);
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap);
+bool dvc_wave_supported(int dim, int num);
+long dvc_wave_bound_shorts(int cap);
+void launch_dvc_wave(int dim, int num, long E, const int* cell_ptr, double* xi_slot, double* w_slot, short* gbound,
+                     int nblk, int cap, int* flag, hipStream_t s);
 void launch_dvc_list(int dim, int num, long E0, long nE, const int* elist, const int* cell_ptr,
                      double* xi_slot, double* w_slot, void* scratch, int max_np, int cap, int* d_flag,
                      hipStream_t s);
