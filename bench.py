@@ -112,7 +112,7 @@ def roofline(P, precision, reps=50):
     import underworld2_amd._lib as L
     st = P["stokes"]
     ms = ctypes.c_double()
-    L.check(L.lib().uwg_stokes_time_apply(st._h, precision, reps, ctypes.byref(ms)), "time_apply")
+    L.check(L.lib().uwg_stokes_time_apply(st._h, precision, 0, reps, ctypes.byref(ms)), "time_apply")
     mesh = P["mesh"]
     E, N, d = mesh.elementsGlobal, mesh.nodesGlobal, mesh.dim
     s = 4 if precision == 1 else 8

@@ -178,13 +178,14 @@ int uwg_nn_loss_grad(uwg_nn *h, int B, const float *x, const float *t, const flo
 int uwg_nn_set_data(uwg_nn *h, long S, const float *x, const float *t, const float *mask);
 int uwg_nn_train(uwg_nn *h, int steps, int B, float lr, float weight_decay, long seed, float *losses);
 int uwg_nn_eval(uwg_nn *h, int B, float *loss);
-/* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply */
+/* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply (a negative kz3 fixes the
+ * 3D chunk at -kz3 on every level instead of adapting it to the grid) */
 int uwg_set_apply_chunk(int kz3, int kz2);
 /* STREAM-style device copy of n doubles (reads 8n bytes, writes 8n bytes); ms per copy.
  * Measures attainable HBM bandwidth and calibrates byte counters. */
 int uwg_stream_copy(long n, int reps, double *ms);
-/* time `reps` applies of K (level 0) in the given precision; returns ms per apply */
-int uwg_stokes_time_apply(uwg_stokes *st, int precision, int reps, double *ms);
+/* time `reps` applies of K on a multigrid level in the given precision; ms per apply */
+int uwg_stokes_time_apply(uwg_stokes *st, int precision, int level, int reps, double *ms);
 
 #ifdef __cplusplus
 }

@@ -93,13 +93,13 @@ __device__ inline double kdiag_entry(const double* Wg, int a, int i, const doubl
 // A6/A7/A9 particle -> element moments, element force vector, approxS diagonal.
 // One thread per element, particles in cell-table (slot) order.  Function values by
 // particle id: material tables (mat != null) or per-particle arrays.
-//   W[g*E+e], fe[e*NN*DIM + a*DIM + i], approxS[e]
+//   W[g*Es+e] (Es: padded stride), fe[e*NN*DIM + a*DIM + i], approxS[e]
 // f_e: VectorAssemblyTerm_NA__Fn.cpp:143-202; approxS: UzawaPreconditionerTerm.c:124-209
 // ---------------------------------------------------------------------------------------
 struct MomArgs {
   int dim, res[3];
   double h[3], detJ;
-  long E;
+  long E, Es;
   int nmat;
   double Ge[24];  // constant element gradient matrix, passed by value
 };
@@ -163,7 +163,7 @@ __global__ void k_moments(MomArgs A, const int* __restrict__ cell_ptr, const int
     }
   }
   moments_to_weights<DIM>(M);
-  for (int q = 0; q < Q; q++) W[(long)q * A.E + e] = M[q];
+  for (int q = 0; q < Q; q++) W[(long)q * A.Es + e] = M[q];
   for (int q = 0; q < NN * DIM; q++) fe[e * NN * DIM + q] = F[q];
   // approxS_e = sum_v Ge[v]^2 / Ke[v][v]
   double sc[3] = {2.0 / A.h[0], 2.0 / A.h[1], DIM == 3 ? 2.0 / A.h[2] : 1.0};
@@ -187,6 +187,7 @@ static MomArgs mom_args(const Geom& g) {
     if (d < g.dim) A.detJ *= 0.5 * A.h[d];
   }
   A.E = g.E;
+  A.Es = g.Es;
   A.nmat = 0;
   return A;
 }
@@ -243,7 +244,7 @@ void launch_node_gather_fe(const Geom& g, const double* fe, const uint8_t* bc, d
 // (xi_c = (xi_child + s)/2, s = -1/+1) give the coarse moments exactly.
 // ---------------------------------------------------------------------------------------
 template <int DIM>
-__global__ void k_restrict_W(int crx, int cry, int crz, long Ec, long Ef, const double* __restrict__ Wf,
+__global__ void k_restrict_W(int crx, int cry, int crz, long Ec, long Ecs, long Efs, const double* __restrict__ Wf,
                              double* __restrict__ Wc) {
   constexpr int Q = DIM == 3 ? 27 : 9;
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,7 +259,7 @@ __global__ void k_restrict_W(int crx, int cry, int crz, long Ec, long Ef, const 
         long fe = (2 * ci + ox) + (long)frx * ((2 * cj + oy) + (long)fry * (DIM == 3 ? (2 * ck + oz) : 0));
         double sx = ox ? 1.0 : -1.0, sy = oy ? 1.0 : -1.0, sz = oz ? 1.0 : -1.0;
         for (int g = 0; g < Q; g++) {
-          double w = Wf[(long)g * Ef + fe];
+          double w = Wf[(long)g * Efs + fe];
           double X = 0.5 * (gpt(g % 3) + sx), Y = 0.5 * (gpt((g / 3) % 3) + sy);
           double px[3] = {1.0, X, X * X}, py[3] = {1.0, Y, Y * Y};
           if (DIM == 3) {
@@ -274,22 +275,22 @@ __global__ void k_restrict_W(int crx, int cry, int crz, long Ec, long Ef, const 
         }
       }
   moments_to_weights<DIM>(M);
-  for (int q = 0; q < Q; q++) Wc[(long)q * Ec + e] = M[q];
+  for (int q = 0; q < Q; q++) Wc[(long)q * Ecs + e] = M[q];
 }
 
 void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, double* Wc, hipStream_t s) {
   if (coarse.dim == 3)
     hipLaunchKernelGGL(k_restrict_W<3>, nblocks(coarse.E, 128), 128, 0, s, coarse.res[0], coarse.res[1],
-                       coarse.res[2], coarse.E, fine.E, Wf, Wc);
+                       coarse.res[2], coarse.E, coarse.Es, fine.Es, Wf, Wc);
   else
     hipLaunchKernelGGL(k_restrict_W<2>, nblocks(coarse.E, 128), 128, 0, s, coarse.res[0], coarse.res[1], 1,
-                       coarse.E, fine.E, Wf, Wc);
+                       coarse.E, coarse.Es, fine.Es, Wf, Wc);
   UWG_HIP(hipGetLastError());
 }
 
 // inverse diagonal of K (identity rows on Dirichlet dofs)
 template <int DIM>
-__global__ void k_diag(int rx, int ry, int rz, long N, long E, double hx, double hy, double hz,
+__global__ void k_diag(int rx, int ry, int rz, long N, long Es, double hx, double hy, double hz,
                        const double* __restrict__ W, const uint8_t* __restrict__ bc, double* __restrict__ dinv) {
   constexpr int Q = DIM == 3 ? 27 : 9;
   long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,7 +307,7 @@ __global__ void k_diag(int rx, int ry, int rz, long N, long E, double hx, double
         long e = ei + (long)rx * (ej + (long)ry * ek);
         int a = (1 - ox) + 2 * (1 - oy) + (DIM == 3 ? 4 * (1 - oz) : 0);
         double Wg[27];
-        for (int g = 0; g < Q; g++) Wg[g] = W[(long)g * E + e];
+        for (int g = 0; g < Q; g++) Wg[g] = W[(long)g * Es + e];
         for (int d = 0; d < DIM; d++) acc[d] += kdiag_entry<DIM>(Wg, a, d, sc);
       }
   uint8_t m = bc[n];
@@ -317,10 +318,10 @@ void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv
   double h[3];
   for (int d = 0; d < 3; d++) h[d] = d < g.dim ? (g.hi[d] - g.lo[d]) / g.res[d] : 1.0;
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_diag<3>, nblocks(g.N, 128), 128, 0, s, g.res[0], g.res[1], g.res[2], g.N, g.E, h[0], h[1],
+    hipLaunchKernelGGL(k_diag<3>, nblocks(g.N, 128), 128, 0, s, g.res[0], g.res[1], g.res[2], g.N, g.Es, h[0], h[1],
                        h[2], W, bc, dinv);
   else
-    hipLaunchKernelGGL(k_diag<2>, nblocks(g.N, 128), 128, 0, s, g.res[0], g.res[1], 1, g.N, g.E, h[0], h[1], h[2],
+    hipLaunchKernelGGL(k_diag<2>, nblocks(g.N, 128), 128, 0, s, g.res[0], g.res[1], 1, g.N, g.Es, h[0], h[1], h[2],
                        W, bc, dinv);
   UWG_HIP(hipGetLastError());
 }
@@ -340,7 +341,7 @@ template <typename T>
 struct KArgs {
   int rx, ry, rz, KZ;
   T sx, sy, sz;
-  long E, nx, nxy;
+  long Ws, nx, nxy;  // Ws: stride of the per-point weight arrays
 };
 
 template <typename T>
@@ -460,11 +461,45 @@ __device__ __forceinline__ void load_level3(const KArgs<T>& A, const T* __restri
   }
 }
 
+// What the apply kernel writes for an owned node (the operator value is kx = (K x)_n, or x_n on
+// a Dirichlet row):
+//   APPLY_Y        y = kx
+//   APPLY_RESIDUAL y = b - kx
+//   APPLY_CHEB     r = b - kx; v = a d + c dinv r; d = v; y = x + v   (a Chebyshev step)
+//   APPLY_CHEB0    the same with d taken as zero (the first step from a nonzero guess)
+// The Chebyshev forms write the new iterate into y, a buffer distinct from x (other tiles
+// still read x), so the smoother alternates two buffers.
+enum ApplyMode { APPLY_Y = 0, APPLY_RESIDUAL = 1, APPLY_CHEB = 2, APPLY_CHEB0 = 3 };
+template <typename T>
+struct SmoothArgs {
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T* d = nullptr;
+  T a = 0, c = 0;
+};
+template <typename T, int MODE>
+__device__ __forceinline__ void apply_epilogue(const SmoothArgs<T>& S, const T* __restrict__ x, T* __restrict__ y,
+                                               long k, T kx) {
+  if (MODE == APPLY_Y) {
+    y[k] = kx;
+  } else {
+    const T r = S.b[k] - kx;
+    if (MODE == APPLY_RESIDUAL) {
+      y[k] = r;
+    } else {
+      const T v = MODE == APPLY_CHEB ? S.a * S.d[k] + S.c * (S.dinv[k] * r) : S.c * (S.dinv[k] * r);
+      S.d[k] = v;
+      y[k] = x[k] + v;
+    }
+  }
+}
+
 // Occupancy: the FP64 instance is held to 256 VGPRs (two waves per SIMD; the cap costs a few
 // bytes of spill), the FP32 instance needs no cap (153 VGPRs, three waves per SIMD).
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
-                                                T* __restrict__ y, const uint8_t* __restrict__ bc) {
+template <typename T, int MODE>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T> A, const T* __restrict__ W,
+                                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                                     const uint8_t* __restrict__ bc, SmoothArgs<T> S) {
   __shared__ T sb[12][256];
   __shared__ T st[12][256];
   const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
@@ -494,7 +529,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T>
     T r[8][3];
     if (colE && ek >= 0 && ek < A.rz) {
       long e = ei + (long)A.rx * (ej + (long)A.ry * ek);
-      elem3<T>(W, A.E, e, u, r, A.sx, A.sy, A.sz);
+      elem3<T>(W, A.Ws, e, u, r, A.sx, A.sy, A.sz);
     } else {
 #pragma unroll
       for (int a = 0; a < 8; a++)
@@ -513,7 +548,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T>
       for (int i = 0; i < 3; i++) {
         T s = st[9 + i][tid - 17] + st[6 + i][tid - 16] + st[3 + i][tid - 1] + st[i][tid] + sb[9 + i][tid - 17] +
               sb[6 + i][tid - 16] + sb[3 + i][tid - 1] + sb[i][tid];
-        y[n * 3 + i] = ((m >> i) & 1) ? x[n * 3 + i] : s;
+        apply_epilogue<T, MODE>(S, x, y, n * 3 + i, ((m >> i) & 1) ? x[n * 3 + i] : s);
       }
     }
     __syncthreads();
@@ -591,9 +626,9 @@ __device__ __forceinline__ void load_level2(const KArgs<T>& A, const T* __restri
   }
 }
 
-template <typename T>
+template <typename T, int MODE>
 __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
-                                               T* __restrict__ y, const uint8_t* __restrict__ bc) {
+                                               T* __restrict__ y, const uint8_t* __restrict__ bc, SmoothArgs<T> S) {
   __shared__ T sb[4][64];
   __shared__ T st[4][64];
   const int tid = threadIdx.x;
@@ -609,7 +644,7 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
     T r[4][2];
     if (colE && ek >= 0 && ek < A.ry) {
       long e = ei + (long)A.rx * ek;
-      elem2<T>(W, A.E, e, u, r, A.sx, A.sy);
+      elem2<T>(W, A.Ws, e, u, r, A.sx, A.sy);
     } else {
 #pragma unroll
       for (int a = 0; a < 4; a++) r[a][0] = r[a][1] = (T)0;
@@ -625,7 +660,7 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
 #pragma unroll
       for (int i = 0; i < 2; i++) {
         T s = st[2 + i][tid - 1] + st[i][tid] + sb[2 + i][tid - 1] + sb[i][tid];
-        y[n * 2 + i] = ((m >> i) & 1) ? x[n * 2 + i] : s;
+        apply_epilogue<T, MODE>(S, x, y, n * 2 + i, ((m >> i) & 1) ? x[n * 2 + i] : s);
       }
     }
     __syncthreads();
@@ -641,13 +676,21 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
 }
 
 static int g_kz3 = 8, g_kz2 = 32;
+static bool g_kz3_fixed = false;
 void set_apply_chunk(int kz3, int kz2) {
-  if (kz3 > 0) g_kz3 = kz3;
+  if (kz3 > 0) {
+    g_kz3 = kz3;
+    g_kz3_fixed = false;
+  } else if (kz3 < 0) {
+    g_kz3 = -kz3;
+    g_kz3_fixed = true;
+  }
   if (kz2 > 0) g_kz2 = kz2;
 }
 
-template <typename T>
-void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s) {
+template <typename T, int MODE>
+static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, const SmoothArgs<T>& S,
+                              hipStream_t s) {
   KArgs<T> A;
   A.rx = g.res[0];
   A.ry = g.res[1];
@@ -655,25 +698,68 @@ void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* 
   A.sx = (T)(2.0 / ((g.hi[0] - g.lo[0]) / g.res[0]));
   A.sy = (T)(2.0 / ((g.hi[1] - g.lo[1]) / g.res[1]));
   A.sz = g.dim == 3 ? (T)(2.0 / ((g.hi[2] - g.lo[2]) / g.res[2])) : (T)1;
-  A.E = g.E;
+  A.Ws = g.Es;
   A.nx = g.res[0] + 1;
   A.nxy = A.nx * (g.res[1] + 1);
   if (g.dim == 3) {
-    // shorter z-chunks on small grids so the launch still covers the 256 CUs
+    // z-chunk: one plane per block on the small multigrid levels (latency-bound, more blocks
+    // win), longer chunks where the grid already fills the GPU
     const long tiles = (long)((g.res[0] + 1 + 14) / 15) * ((g.res[1] + 1 + 14) / 15);
     A.KZ = g_kz3;
-    while (A.KZ > 2 && tiles * ((g.res[2] + A.KZ) / A.KZ) < 1024) A.KZ /= 2;
+    if (!g_kz3_fixed) {
+      if (g.res[2] <= 32)
+        A.KZ = 1;
+      else
+        while (A.KZ > 2 && tiles * ((g.res[2] + A.KZ) / A.KZ) < 512) A.KZ /= 2;
+    }
     dim3 grid((g.res[0] + 1 + 14) / 15, (g.res[1] + 1 + 14) / 15, (g.res[2] + 1 + A.KZ - 1) / A.KZ);
-    hipLaunchKernelGGL(k_apply3<T>, grid, 256, 0, s, A, W, x, y, bc);
+    hipLaunchKernelGGL((k_apply3<T, MODE>), grid, 256, 0, s, A, W, x, y, bc, S);
   } else {
     A.KZ = g_kz2;
     dim3 grid((g.res[0] + 1 + 62) / 63, (g.res[1] + 1 + A.KZ - 1) / A.KZ);
-    hipLaunchKernelGGL(k_apply2<T>, grid, 64, 0, s, A, W, x, y, bc);
+    hipLaunchKernelGGL((k_apply2<T, MODE>), grid, 64, 0, s, A, W, x, y, bc, S);
   }
   UWG_HIP(hipGetLastError());
 }
+
+template <typename T>
+void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s) {
+  launch_apply_mode<T, APPLY_Y>(g, W, x, y, bc, SmoothArgs<T>(), s);
+}
 template void launch_apply_K<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t);
 template void launch_apply_K<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t);
+
+// r = b - K x
+template <typename T>
+void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s) {
+  SmoothArgs<T> S;
+  S.b = b;
+  launch_apply_mode<T, APPLY_RESIDUAL>(g, W, x, r, bc, S, s);
+}
+template void launch_residual_K<double>(const Geom&, const double*, const double*, const double*, double*,
+                                        const uint8_t*, hipStream_t);
+template void launch_residual_K<float>(const Geom&, const float*, const float*, const float*, float*, const uint8_t*,
+                                       hipStream_t);
+
+// one Chebyshev step: r = b - K x; d = a d + c dinv r (a d dropped when first); xn = x + d
+template <typename T>
+void launch_cheb_step_K(const Geom& g, const T* W, const T* x, T* xn, const T* b, const T* dinv, T* d, T a, T c,
+                        bool first, const uint8_t* bc, hipStream_t s) {
+  SmoothArgs<T> S;
+  S.b = b;
+  S.dinv = dinv;
+  S.d = d;
+  S.a = a;
+  S.c = c;
+  if (first)
+    launch_apply_mode<T, APPLY_CHEB0>(g, W, x, xn, bc, S, s);
+  else
+    launch_apply_mode<T, APPLY_CHEB>(g, W, x, xn, bc, S, s);
+}
+template void launch_cheb_step_K<double>(const Geom&, const double*, const double*, double*, const double*,
+                                         const double*, double*, double, double, bool, const uint8_t*, hipStream_t);
+template void launch_cheb_step_K<float>(const Geom&, const float*, const float*, float*, const float*, const float*,
+                                        float*, float, float, bool, const uint8_t*, hipStream_t);
 
 // ---------------------------------------------------------------------------------------
 // G p (node gather) and G^T u (element gather); G_e constant on a regular mesh
@@ -968,6 +1054,15 @@ template <typename T>
 __global__ void k_residual(long n, const T* b, T* r) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) r[i] = b[i] - r[i];
 }
+// first Chebyshev step from a zero guess: d = c dinv b, x = d
+template <typename T>
+__global__ void k_cheb_first(long n, T c, const T* dinv, const T* b, T* d, T* x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T v = c * (dinv[i] * b[i]);
+    d[i] = v;
+    x[i] = v;
+  }
+}
 template <typename T>
 __global__ void k_cheb(long n, T a, T b, const T* dinv, const T* r, T* d, T* x) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -1021,6 +1116,10 @@ template <typename T>
 void vk_cheb_update(long n, T a, T b, const T* dinv, const T* r, T* d, T* x, hipStream_t s) {
   hipLaunchKernelGGL(k_cheb<T>, vgrid(n), 256, 0, s, n, a, b, dinv, r, d, x);
 }
+template <typename T>
+void vk_cheb_first(long n, T c, const T* dinv, const T* b, T* d, T* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_first<T>, vgrid(n), 256, 0, s, n, c, dinv, b, d, x);
+}
 #define INST(T)                                                               \
   template void vk_fill<T>(long, T, T*, hipStream_t);                         \
   template void vk_copy<T>(long, const T*, T*, hipStream_t);                  \
@@ -1029,7 +1128,8 @@ void vk_cheb_update(long n, T a, T b, const T* dinv, const T* r, T* d, T* x, hip
   template void vk_scale<T>(long, T, const T*, T*, hipStream_t);              \
   template void vk_xmy_scale<T>(long, const T*, const T*, T*, hipStream_t);   \
   template void vk_residual<T>(long, const T*, T*, hipStream_t);              \
-  template void vk_cheb_update<T>(long, T, T, const T*, const T*, T*, T*, hipStream_t);
+  template void vk_cheb_update<T>(long, T, T, const T*, const T*, T*, T*, hipStream_t); \
+  template void vk_cheb_first<T>(long, T, const T*, const T*, T*, T*, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

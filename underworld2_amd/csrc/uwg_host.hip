@@ -706,7 +706,12 @@ static void build_levels(uwg_stokes* st, int levels_req) {
     Level& A = st->L[l];
     A.g = gs[l];
     const long Q = A.g.dim == 3 ? 27 : 9;
-    A.W = (l == 0 && W0) ? W0 : dalloc<double>(Q * A.g.E);
+    if (l == 0 && W0) {
+      A.W = W0;
+    } else {
+      A.W = dalloc<double>(Q * A.g.Es);
+      UWG_HIP(hipMemset(A.W, 0, sizeof(double) * Q * A.g.Es));  // padding stays zero
+    }
     A.dinv = dalloc<double>(A.n());
     A.x = dalloc<double>(A.n());
     A.b = dalloc<double>(A.n());
@@ -718,19 +723,26 @@ static void build_levels(uwg_stokes* st, int levels_req) {
   st->fp32_ready = false;
 }
 
+// quadrature weights of a level, packed W[q][E] on the host (the device copy is padded to Es)
+static void copy_W_to_host(const Level& A, double* out) {
+  const long Q = A.g.dim == 3 ? 27 : 9;
+  UWG_HIP(hipMemcpy2D(out, sizeof(double) * A.g.E, A.W, sizeof(double) * A.g.Es, sizeof(double) * A.g.E, Q,
+                      hipMemcpyDeviceToHost));
+}
+
 static void ensure_fp32(uwg_stokes* st) {
   if (st->fp32_ready) return;
   for (auto& A : st->L) {
     const long Q = A.g.dim == 3 ? 27 : 9;
     if (!A.W32) {
-      A.W32 = dalloc<float>(Q * A.g.E);
+      A.W32 = dalloc<float>(Q * A.g.Es);
       A.dinv32 = dalloc<float>(A.n());
       A.x32 = dalloc<float>(A.n());
       A.b32 = dalloc<float>(A.n());
       A.r32 = dalloc<float>(A.n());
       A.d32 = dalloc<float>(A.n());
     }
-    vk_d2f(Q * A.g.E, 1.0, A.W, A.W32, 0);
+    vk_d2f(Q * A.g.Es, 1.0, A.W, A.W32, 0);
     vk_d2f(A.n(), 1.0, A.dinv, A.dinv32, 0);
   }
   st->fp32_ready = true;
@@ -750,7 +762,7 @@ static void build_coarse(uwg_stokes* st) {
     return;
   }
   std::vector<double> W((size_t)Q * g.E);
-  UWG_HIP(hipMemcpy(W.data(), A.W, sizeof(double) * Q * g.E, hipMemcpyDeviceToHost));
+  copy_W_to_host(A, W.data());
   std::vector<uint8_t> bc(g.N);
   UWG_HIP(hipMemcpy(bc.data(), A.bc, g.N, hipMemcpyDeviceToHost));
   std::vector<double> K((size_t)n * n, 0.0);
@@ -868,7 +880,7 @@ static void level_feature(uwg_stokes* st, int level, std::vector<float>& out, do
   const Geom& g = A.g;
   const int Q = g.dim == 3 ? 27 : 9;
   std::vector<double> W((size_t)Q * g.E);
-  UWG_HIP(hipMemcpy(W.data(), A.W, sizeof(double) * Q * g.E, hipMemcpyDeviceToHost));
+  copy_W_to_host(A, W.data());
   double vol = 1.0;
   for (int d = 0; d < g.dim; d++) vol *= (g.hi[d] - g.lo[d]) / g.res[d];
   std::vector<double> le(g.E);
@@ -949,30 +961,39 @@ static int g_smooth_degree = 2;
 static int g_verbose = 0;
 
 template <typename T>
+// Chebyshev(degree) smoothing of K x = b with Jacobi scaling, eigenvalue window
+// [0.12, 1.1] * lmax.  Each step after the first is one fused kernel (apply, residual and
+// update); the iterate alternates between x and the level's r buffer, arranged so that the
+// last step writes x.
 static void chebyshev(Level& A, const T* b, T* x, int degree, bool zero_guess) {
   const long n = A.n();
   const double emax = 1.1 * A.lmax, emin = 0.12 * A.lmax;
   const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
   const double sigma = theta / delta;
   double rho = 1.0 / sigma;
-  T* r = LV<T>::r(A);
+  T* tmp = LV<T>::r(A);
   T* d = LV<T>::d(A);
+  const T* dinv = LV<T>::dinv(A);
+  const int steps = zero_guess ? degree - 1 : degree;  // fused steps
+  T* cur = x;
   if (zero_guess) {
-    vk_copy<T>(n, b, r, 0);
-    vk_fill<T>(n, (T)0, x, 0);
-  } else {
-    launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
-    vk_residual<T>(n, b, r, 0);
+    // x0 = 0: d = dinv b / theta, x1 = d; written where the alternation ends in x
+    cur = (steps % 2 == 1) ? tmp : x;
+    vk_cheb_first<T>(n, (T)(1.0 / theta), dinv, b, d, cur, 0);
   }
-  vk_fill<T>(n, (T)0, d, 0);
-  vk_cheb_update<T>(n, (T)0, (T)(1.0 / theta), LV<T>::dinv(A), r, d, x, 0);
-  for (int k = 1; k < degree; k++) {
-    launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
-    vk_residual<T>(n, b, r, 0);
-    double rho_new = 1.0 / (2.0 * sigma - rho);
-    vk_cheb_update<T>(n, (T)(rho_new * rho), (T)(2.0 * rho_new / delta), LV<T>::dinv(A), r, d, x, 0);
-    rho = rho_new;
+  for (int k = 0; k < steps; k++) {
+    T* nxt = (cur == x) ? tmp : x;
+    if (!zero_guess && k == 0) {
+      launch_cheb_step_K<T>(A.g, LV<T>::W(A), cur, nxt, b, dinv, d, (T)0, (T)(1.0 / theta), true, A.bc, 0);
+    } else {
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      launch_cheb_step_K<T>(A.g, LV<T>::W(A), cur, nxt, b, dinv, d, (T)(rho_new * rho), (T)(2.0 * rho_new / delta),
+                            false, A.bc, 0);
+      rho = rho_new;
+    }
+    cur = nxt;
   }
+  if (cur != x) vk_copy<T>(n, cur, x, 0);
 }
 
 template <typename T>
@@ -1015,8 +1036,7 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   Level& C = st->L[l + 1];
   chebyshev<T>(A, b, x, g_smooth_degree, true);
   T* r = LV<T>::r(A);
-  launch_apply_K<T>(A.g, LV<T>::W(A), x, r, A.bc, 0);
-  vk_residual<T>(n, b, r, 0);
+  launch_residual_K<T>(A.g, LV<T>::W(A), x, b, r, A.bc, 0);
   launch_restrict2<T>(A.g, C.g, r, LV<T>::b(C), C.bc, A.bc, 0);
   const bool capture = st->capture_level == l + 1 && st->captured < st->capture_max &&
                        (st->capture_seen++ % st->capture_every) == 0;
@@ -1033,15 +1053,13 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
     if (capture && st->capture_kind == 1) {
       // residual the coarse cycle leaves behind: its error K_C^{-1} r is what an additive
       // correction has to supply
-      launch_apply_K<T>(C.g, LV<T>::W(C), LV<T>::x(C), LV<T>::r(C), C.bc, 0);
-      vk_residual<T>(C.n(), LV<T>::b(C), LV<T>::r(C), 0);
+      launch_residual_K<T>(C.g, LV<T>::W(C), LV<T>::x(C), LV<T>::b(C), LV<T>::r(C), C.bc, 0);
       capture_vector<T>(st, C.n(), LV<T>::r(C));
     }
     if constexpr (std::is_same<T, float>::value) {
       if (use_nn && st->nn_mode == 1) {
         // learned correction of the coarse cycle's remaining error: x_C += NN(b_C - K_C x_C)
-        launch_apply_K<float>(C.g, C.W32, C.x32, C.r32, C.bc, 0);
-        vk_residual<float>(C.n(), C.b32, C.r32, 0);
+        launch_residual_K<float>(C.g, C.W32, C.x32, C.b32, C.r32, C.bc, 0);
         nn_coarse_correction(st->nn.get(), C.g, C.r32, C.d32, C.bc, 0);
         vk_axpy<float>(C.n(), 1.0f, C.d32, C.x32, 0);
       }
@@ -1357,7 +1375,8 @@ extern "C" int uwg_stokes_assemble_fields(uwg_stokes* st, const double* eta_g, c
     }
   }
   Level& A = st->L[0];
-  UWG_HIP(hipMemcpy(A.W, W.data(), sizeof(double) * Q * g.E, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy2D(A.W, sizeof(double) * g.Es, W.data(), sizeof(double) * g.E, sizeof(double) * g.E, Q,
+                      hipMemcpyHostToDevice));
   UWG_HIP(hipMemcpy(st->fe, fe.data(), sizeof(double) * fe.size(), hipMemcpyHostToDevice));
   finish_assembly(st);
   // approxS from the assembled diagonal: computed with the level-0 element weights
@@ -1814,7 +1833,8 @@ extern "C" int uwg_stokes_get_assembly(uwg_stokes* st, int level, double* W, dou
   API_BEGIN
   Level& A = st->L[level];
   const long Q = A.g.dim == 3 ? 27 : 9;
-  if (W) UWG_HIP(hipMemcpy(W, A.W, sizeof(double) * Q * A.g.E, hipMemcpyDeviceToHost));
+  (void)Q;
+  if (W) copy_W_to_host(A, W);
   if (level == 0) {
     const int NN = 1 << st->g.dim;
     if (fe) UWG_HIP(hipMemcpy(fe, st->fe, sizeof(double) * st->g.E * NN * st->g.dim, hipMemcpyDeviceToHost));
@@ -1863,10 +1883,12 @@ extern "C" int uwg_stream_copy(long n, int reps, double* ms) {
   API_END
 }
 
-extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int reps, double* ms) {
+extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int level, int reps, double* ms) {
   if (!st->assembled) return arg_error("system not assembled");
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
+  if (reps < 1) return arg_error("reps must be positive");
   API_BEGIN
-  Level& A = st->L[0];
+  Level& A = st->L[level];
   hipEvent_t e0, e1;
   UWG_HIP(hipEventCreate(&e0));
   UWG_HIP(hipEventCreate(&e1));

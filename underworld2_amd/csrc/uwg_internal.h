@@ -35,6 +35,9 @@ struct Geom {
   double lo[3], hi[3];
   long N, E;       // nodes, elements
   int nn[3];       // nodes per dim
+  long Es;         // stride between the per-point arrays of the quadrature weights W[q][Es]:
+                   // E padded so that the 3^dim arrays never sit a power of two apart
+                   // (identical strides alias the same cache sets and memory channels)
 };
 
 inline Geom make_geom(int dim, const int* res, const double* lo, const double* hi) {
@@ -48,6 +51,7 @@ inline Geom make_geom(int dim, const int* res, const double* lo, const double* h
   }
   g.N = (long)g.nn[0] * g.nn[1] * g.nn[2];
   g.E = (long)g.res[0] * g.res[1] * g.res[2];
+  g.Es = ((g.E + 63) / 64) * 64 + 64;
   return g;
 }
 
@@ -83,6 +87,11 @@ void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, d
 void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
 template <typename T>
 void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
+template <typename T>
+void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s);
+template <typename T>
+void launch_cheb_step_K(const Geom& g, const T* W, const T* x, T* xn, const T* b, const T* dinv, T* d, T a, T c,
+                        bool first, const uint8_t* bc, hipStream_t s);
 template <typename T>
 void launch_apply_G(const Geom& g, const double* Ge, const T* p, const uint8_t* bc, T* y, hipStream_t s);
 template <typename T>
@@ -137,6 +146,8 @@ int vk_reduce_blocks();
 // chebyshev helper: d = a*d + b*(dinv .* r) ; x += d  (fused)
 template <typename T>
 void vk_cheb_update(long n, T a, T b, const T* dinv, const T* r, T* d, T* x, hipStream_t s);
+template <typename T>
+void vk_cheb_first(long n, T c, const T* dinv, const T* b, T* d, T* x, hipStream_t s);
 // r = b - r  (r holds A x on input)
 template <typename T> void vk_residual(long n, const T* b, T* r, hipStream_t s);
 // dense coarse solve y = Ainv x for a small dense inverse (double)
