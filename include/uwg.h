@@ -153,6 +153,10 @@ int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
  * correction of the coarse cycle's remaining error.  FP32 cycles only (precision 1). */
 int uwg_stokes_load_nn(uwg_stokes *st, int level, int nparams, const float *params, int channels, int depth);
 int uwg_stokes_set_nn_mode(uwg_stokes *st, int mode);
+/* fallback monitor: while the network is in use the FP64 outer residual must fall by at least
+ * min_drop (fraction) over every `window` iterations, else multigrid alone takes over
+ * (defaults 10 and 0.1) */
+int uwg_stokes_set_nn_monitor(uwg_stokes *st, int window, double min_drop);
 /* record up to max_samples vectors of `level` (every `every`-th visit) during multigrid
  * cycles: kind 0 the right-hand side restricted to the level, kind 1 the residual that the
  * cycle below the level leaves (b - K x after it); retrieved as float[count][N_level*dim] */

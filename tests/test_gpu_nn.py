@@ -117,24 +117,47 @@ def test_capture_and_level_targets():
     assert solver.get_stats().converged
 
 
-def test_fallback_with_useless_model():
-    """A network that makes the cycle worse is dropped by the FP64 residual monitor and the
-    solve still reaches its tolerance on plain multigrid."""
+def _mixed_solver(st, model=None):
     import underworld2_amd as uw
-    from underworld2_amd import learned
-    st, v, p, _ = _small_problem(32)
-    net = _net((9, 9, 9), 8, 3, seed=1)
-    bad = learned.LearnedCoarseCorrection(net.get_params() * 50.0, (8, 8, 8), 8, 3, mode="replace")
     solver = uw.systems.Solver(st)
     solver.set_method("block")
     solver.set_precision("mixed")
     solver.set_outer_rtol(1e-6)
-    solver.set_learned(bad)
+    solver.set_learned(model)
+    return solver
+
+
+def test_useless_model_is_damped():
+    """A network whose output is meaningless is scaled down by the residual-minimising step
+    on its level: the solve still reaches the FP64 tolerance."""
+    from underworld2_amd import learned
+    st, v, p, _ = _small_problem(32)
+    net = _net((9, 9, 9), 8, 3, seed=1)
+    for mode in ("replace", "add"):
+        bad = learned.LearnedCoarseCorrection(net.get_params() * 50.0, (8, 8, 8), 8, 3, mode=mode)
+        solver = _mixed_solver(st, bad)
+        solver.solve()
+        s = solver.get_stats()
+        print(f"{mode}: its={s.outer_its} fallback_used={s.fallback_used}", flush=True)
+        assert s.converged and s.res_momentum < 1e-5 * s.norm_f
+
+
+def test_fallback_monitor_switches_to_multigrid():
+    """The monitor condition forced from the host (a required 99.99 % drop per 10 iterations):
+    the network is switched off, plain multigrid finishes, and the answer meets the tolerance."""
+    import underworld2_amd._lib as L
+    from underworld2_amd import learned
+    st, v, p, _ = _small_problem(32)
+    net = _net((9, 9, 9), 8, 3, seed=1)
+    model = learned.LearnedCoarseCorrection(net.get_params(), (8, 8, 8), 8, 3, mode="add")
+    solver = _mixed_solver(st, model)
+    L.check(L.lib().uwg_stokes_set_nn_monitor(st._h, 10, 0.9999))
     solver.solve()
     s = solver.get_stats()
-    print(f"fallback_used={s.fallback_used} its={s.outer_its}", flush=True)
     assert s.converged and s.fallback_used == 1
     assert s.res_momentum < 1e-5 * s.norm_f
+    # the monitor is reset for the next solve on the handle
+    L.check(L.lib().uwg_stokes_set_nn_monitor(st._h, 10, 0.1))
 
 
 def test_trained_model_gain_on_unseen_problems():

@@ -85,6 +85,7 @@ SIGNATURES = {
     "uwg_stokes_get_rhs": [_vp, _dp],
     "uwg_stokes_load_nn": [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int],
     "uwg_stokes_set_nn_mode": [_vp, ctypes.c_int],
+    "uwg_stokes_set_nn_monitor": [_vp, ctypes.c_int, ctypes.c_double],
     "uwg_stokes_capture": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "uwg_stokes_get_captured": [_vp, _fp, _ip],
     "uwg_stokes_level_solve": [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, _ip],

@@ -344,9 +344,8 @@ struct KArgs {
   long Ws, nx, nxy;  // Ws: stride of the per-point weight arrays
 };
 
-template <typename T>
-__device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, const T (&u)[8][3], T (&r)[8][3],
-                                      T sx, T sy, T sz) {
+template <typename T, typename WF>
+__device__ __forceinline__ void elem3(WF wq, const T (&u)[8][3], T (&r)[8][3], T sx, T sy, T sz) {
   const T q8 = (T)0.125;
   T cX[3], cY[3], cZ[3], cXY[3], cXZ[3], cYZ[3], cXYZ[3];
 #pragma unroll
@@ -390,7 +389,7 @@ __device__ __forceinline__ void elem3(const T* __restrict__ W, long E, long e, c
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) {
         const T X = (T)gpt(gx), Y = (T)gpt(gy);
-        const T w = W[(long)(gx + 3 * gy + 9 * gz) * E + e];
+        const T w = wq(gx + 3 * gy + 9 * gz);
         const T w2 = w + w;
         const T G00 = GX[0][gy], G10 = GX[1][gy], G20 = GX[2][gy];
         const T G01 = GY[0][gx], G11 = GY[1][gx], G21 = GY[2][gx];
@@ -469,13 +468,22 @@ __device__ __forceinline__ void load_level3(const KArgs<T>& A, const T* __restri
 //   APPLY_CHEB0    the same with d taken as zero (the first step from a nonzero guess)
 // The Chebyshev forms write the new iterate into y, a buffer distinct from x (other tiles
 // still read x), so the smoother alternates two buffers.
-enum ApplyMode { APPLY_Y = 0, APPLY_RESIDUAL = 1, APPLY_CHEB = 2, APPLY_CHEB0 = 3 };
+//   APPLY_STOKES   y = kx + (G p)_n on free rows, and q_e = (G^T u)_e for the elements the
+//                  tile owns: the coupled operator [K G; G^T 0] in one pass
+enum ApplyMode { APPLY_Y = 0, APPLY_RESIDUAL = 1, APPLY_CHEB = 2, APPLY_CHEB0 = 3, APPLY_STOKES = 4 };
+// the element gradient matrix G_e, identical for every element of a regular mesh
+struct GeConst {
+  double v[24];
+};
 template <typename T>
 struct SmoothArgs {
   const T* b = nullptr;
   const T* dinv = nullptr;
   T* d = nullptr;
   T a = 0, c = 0;
+  const T* p = nullptr;  // APPLY_STOKES: pressure per element (input)
+  T* q = nullptr;        //               G^T u per element (output)
+  GeConst Ge;
 };
 template <typename T, int MODE>
 __device__ __forceinline__ void apply_epilogue(const SmoothArgs<T>& S, const T* __restrict__ x, T* __restrict__ y,
@@ -528,8 +536,17 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T>
     load_level3(A, x, bc, ei, ej, ek + 1, u, 4);
     T r[8][3];
     if (colE && ek >= 0 && ek < A.rz) {
-      long e = ei + (long)A.rx * (ej + (long)A.ry * ek);
-      elem3<T>(W, A.Ws, e, u, r, A.sx, A.sy, A.sz);
+      const long e = ei + (long)A.rx * (ej + (long)A.ry * ek);
+      elem3<T>([&](int q) { return W[(long)q * A.Ws + e]; }, u, r, A.sx, A.sy, A.sz);
+      if (MODE == APPLY_STOKES && lx >= 1 && ly >= 1 && ek >= k0) {
+        // G^T u for the element this tile owns (u is zero on Dirichlet dofs)
+        T acc = 0;
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) acc += (T)S.Ge.v[a * 3 + d] * u[a][d];
+        S.q[e] = acc;
+      }
     } else {
 #pragma unroll
       for (int a = 0; a < 8; a++)
@@ -544,11 +561,31 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T>
     if (ownN && ek >= k0) {
       long n = ei + A.nx * ej + A.nxy * ek;
       uint8_t m = bc[n];
+      T gp[3] = {0, 0, 0};
+      if (MODE == APPLY_STOKES) {
+        // (G p)_n: the elements around node n in (z, y, x) order, node n being corner a of each
+#pragma unroll
+        for (int oz = 0; oz < 2; oz++)
+#pragma unroll
+          for (int oy = 0; oy < 2; oy++)
+#pragma unroll
+            for (int ox = 0; ox < 2; ox++) {
+              const int gi = ei - 1 + ox, gj = ej - 1 + oy, gk = ek - 1 + oz;
+              if (gi < 0 || gi >= A.rx || gj < 0 || gj >= A.ry || gk < 0 || gk >= A.rz) continue;
+              const T pe = S.p[gi + (long)A.rx * (gj + (long)A.ry * gk)];
+              const int a = (1 - ox) + 2 * (1 - oy) + 4 * (1 - oz);
+#pragma unroll
+              for (int d = 0; d < 3; d++) gp[d] += (T)S.Ge.v[a * 3 + d] * pe;
+            }
+      }
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         T s = st[9 + i][tid - 17] + st[6 + i][tid - 16] + st[3 + i][tid - 1] + st[i][tid] + sb[9 + i][tid - 17] +
               sb[6 + i][tid - 16] + sb[3 + i][tid - 1] + sb[i][tid];
-        apply_epilogue<T, MODE>(S, x, y, n * 3 + i, ((m >> i) & 1) ? x[n * 3 + i] : s);
+        if (MODE == APPLY_STOKES)
+          y[n * 3 + i] = ((m >> i) & 1) ? x[n * 3 + i] : s + gp[i];
+        else
+          apply_epilogue<T, MODE>(S, x, y, n * 3 + i, ((m >> i) & 1) ? x[n * 3 + i] : s);
       }
     }
     __syncthreads();
@@ -705,12 +742,13 @@ static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const
     // z-chunk: one plane per block on the small multigrid levels (latency-bound, more blocks
     // win), longer chunks where the grid already fills the GPU
     const long tiles = (long)((g.res[0] + 1 + 14) / 15) * ((g.res[1] + 1 + 14) / 15);
-    A.KZ = g_kz3;
+    // measured at 128^3: 16-plane chunks suit the FP32 instance, 8-plane the FP64 one
+    A.KZ = (g_kz3_fixed || sizeof(T) == 8) ? g_kz3 : 2 * g_kz3;
     if (!g_kz3_fixed) {
       if (g.res[2] <= 32)
         A.KZ = 1;
       else
-        while (A.KZ > 2 && tiles * ((g.res[2] + A.KZ) / A.KZ) < 512) A.KZ /= 2;
+        while (A.KZ > 2 && tiles * ((g.res[2] + A.KZ) / A.KZ) < 400) A.KZ /= 2;
     }
     dim3 grid((g.res[0] + 1 + 14) / 15, (g.res[1] + 1 + 14) / 15, (g.res[2] + 1 + A.KZ - 1) / A.KZ);
     hipLaunchKernelGGL((k_apply3<T, MODE>), grid, 256, 0, s, A, W, x, y, bc, S);
@@ -728,6 +766,29 @@ void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* 
 }
 template void launch_apply_K<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t);
 template void launch_apply_K<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t);
+
+// [y; q] = [K G; G^T 0] [x; p] on the free dofs (identity rows on Dirichlet dofs): one fused
+// pass in 3D; in 2D the separate operator kernels
+template <typename T>
+void launch_apply_stokes(const Geom& g, const T* W, const double* Ge, const T* x, const T* p, T* y, T* q, T* gp_tmp,
+                         const uint8_t* bc, hipStream_t s) {
+  if (g.dim == 3) {
+    SmoothArgs<T> S;
+    S.p = p;
+    S.q = q;
+    for (int k = 0; k < 24; k++) S.Ge.v[k] = Ge[k];
+    launch_apply_mode<T, APPLY_STOKES>(g, W, x, y, bc, S, s);
+  } else {
+    launch_apply_K<T>(g, W, x, y, bc, s);
+    launch_apply_G<T>(g, Ge, p, bc, gp_tmp, s);
+    vk_axpy<T>((long)g.N * g.dim, (T)1, gp_tmp, y, s);
+    launch_apply_GT<T>(g, Ge, x, bc, q, s);
+  }
+}
+template void launch_apply_stokes<double>(const Geom&, const double*, const double*, const double*, const double*,
+                                          double*, double*, double*, const uint8_t*, hipStream_t);
+template void launch_apply_stokes<float>(const Geom&, const float*, const double*, const float*, const float*,
+                                         float*, float*, float*, const uint8_t*, hipStream_t);
 
 // r = b - K x
 template <typename T>
@@ -765,9 +826,6 @@ template void launch_cheb_step_K<float>(const Geom&, const float*, const float*,
 // G p (node gather) and G^T u (element gather); G_e constant on a regular mesh
 // (_GradientStiffnessMatrixTerm_AssembleElement, GradientStiffnessMatrixTerm.c:129-221)
 // ---------------------------------------------------------------------------------------
-struct GeConst {
-  double v[24];
-};
 
 template <typename T, int DIM>
 __global__ void k_apply_G(int rx, int ry, int rz, long N, GeConst Ge, const T* __restrict__ p,

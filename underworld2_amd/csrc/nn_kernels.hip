@@ -698,6 +698,41 @@ void nn_set_feature(NNModel* m, const float* feat_host, double eta_ref) {
   UWG_HIP(hipMemcpy(m->feat, feat_host, sizeof(float) * m->grid[0].N(), hipMemcpyHostToDevice));
 }
 
+// Residual-minimising step length for a correction d of K x = b with residual r:
+// omega = (r . Kd) / (Kd . Kd), clamped to [0, 2] (a prediction pointing the wrong way, or a
+// non-finite one, is dropped); then x = (accumulate ? x : 0) + omega d.  Single block, FP64 sums, deterministic.
+__global__ __launch_bounds__(256) void k_line_search(long n, const float* __restrict__ r, const float* __restrict__ kd,
+                                                     const float* __restrict__ d, float* __restrict__ x,
+                                                     int accumulate) {
+  __shared__ double s1[256], s2[256];
+  double a1 = 0.0, a2 = 0.0;
+  for (long t = threadIdx.x; t < n; t += 256) {
+    a1 += (double)r[t] * kd[t];
+    a2 += (double)kd[t] * kd[t];
+  }
+  s1[threadIdx.x] = a1;
+  s2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+      s2[threadIdx.x] += s2[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  double om = s2[0] > 0.0 ? s1[0] / s2[0] : 0.0;
+  if (!(om > 0.0)) om = 0.0;  // also catches a non-finite prediction
+  if (om > 2.0) om = 2.0;
+  const float w = (float)om;
+  for (long t = threadIdx.x; t < n; t += 256) x[t] = (accumulate ? x[t] : 0.0f) + w * d[t];
+}
+
+void nn_line_search(long n, const float* r, const float* kd, const float* d, float* x, bool accumulate,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_line_search, 1, 256, 0, s, n, r, kd, d, x, accumulate ? 1 : 0);
+  UWG_HIP(hipGetLastError());
+}
+
 void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s) {
   const long N = m->grid[0].N();
   hipLaunchKernelGGL(k_rms, 1, 256, 0, s, N * 3, r, m->scale);

@@ -154,6 +154,8 @@ struct uwg_stokes {
   KSpaceT<float>* kf = nullptr;  // FP32 inner Krylov space of the mixed-precision solver
   // learned-correction support
   bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
+  int nn_window = 10;            // fallback monitor: over nn_window iterations the residual must
+  double nn_min_drop = 0.1;      // fall by at least this fraction while the network is in use
   int nn_mode = 0;               // 0: replace the coarse cycle, 1: add to it
   int capture_level = -1;        // record coarse vectors seen by the cycle
   int capture_kind = 0;          // 0: right-hand side reaching the level, 1: residual left by its cycle
@@ -1045,9 +1047,13 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   if constexpr (std::is_same<T, float>::value)
     use_nn = st->nn && !st->nn_disabled && l + 1 == st->nn_level;
   if (use_nn && st->nn_mode == 0) {
-    // the learned network stands in for the coarse-grid cycle below this level
-    if constexpr (std::is_same<T, float>::value)
-      nn_coarse_correction(st->nn.get(), C.g, LV<T>::b(C), LV<T>::x(C), C.bc, 0);
+    // the learned network stands in for the coarse-grid cycle below this level; its output is
+    // scaled by the residual-minimising step on this level
+    if constexpr (std::is_same<T, float>::value) {
+      nn_coarse_correction(st->nn.get(), C.g, C.b32, C.d32, C.bc, 0);
+      launch_apply_K<float>(C.g, C.W32, C.d32, C.r32, C.bc, 0);
+      nn_line_search(C.n(), C.b32, C.r32, C.d32, C.x32, false, 0);
+    }
   } else {
     vcycle<T>(st, l + 1, LV<T>::b(C), LV<T>::x(C));
     if (capture && st->capture_kind == 1) {
@@ -1059,9 +1065,11 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
     if constexpr (std::is_same<T, float>::value) {
       if (use_nn && st->nn_mode == 1) {
         // learned correction of the coarse cycle's remaining error: x_C += NN(b_C - K_C x_C)
+        // (b_C is not needed after the cycle: it receives K_C d for the step length)
         launch_residual_K<float>(C.g, C.W32, C.x32, C.b32, C.r32, C.bc, 0);
         nn_coarse_correction(st->nn.get(), C.g, C.r32, C.d32, C.bc, 0);
-        vk_axpy<float>(C.n(), 1.0f, C.d32, C.x32, 0);
+        launch_apply_K<float>(C.g, C.W32, C.d32, C.b32, C.bc, 0);
+        nn_line_search(C.n(), C.r32, C.b32, C.d32, C.x32, true, 0);
       }
     }
   }
@@ -1578,10 +1586,7 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   launch_neg_recip<double>(E, st->approxS, diagS, 0);
   double* gp = wsbuf<double>(st, WS_BLK_GP, n);
   OpFn Aop = [&](const double* xx, double* yy) {
-    launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
-    launch_apply_G(st->g, st->Ge, xx + n, A.bc, gp, 0);
-    vk_axpy<double>(n, 1.0, gp, yy, 0);
-    launch_apply_GT(st->g, st->Ge, xx, A.bc, yy + n, 0);
+    launch_apply_stokes<double>(A.g, A.W, st->Ge, xx, xx + n, yy, yy + n, gp, A.bc, 0);
   };
   OpFn Mop = [&](const double* rr, double* zz) {
     st->vcycles++;
@@ -1605,7 +1610,8 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   std::vector<double> hist;
   std::function<void(int, double)> mon = [&](int k, double rel) {
     hist.push_back(rel);
-    if (o.learned && st->nn && !st->nn_disabled && k >= 10 && rel > 0.9 * hist[k - 10]) {
+    const int wdw = st->nn_window;
+    if (o.learned && st->nn && !st->nn_disabled && k >= wdw && rel > (1.0 - st->nn_min_drop) * hist[k - wdw]) {
       st->nn_disabled = true;
       S->fallback_used = 1;
       if (g_verbose) fprintf(stderr, " learned correction disabled at iteration %d (rel %.3e)\n", k, rel);
@@ -1933,6 +1939,13 @@ extern "C" int uwg_stokes_load_nn(uwg_stokes* st, int level, int nparams, const 
 extern "C" int uwg_stokes_set_nn_mode(uwg_stokes* st, int mode) {
   if (mode != 0 && mode != 1) return arg_error("mode must be 0 (replace) or 1 (add)");
   st->nn_mode = mode;
+  return 0;
+}
+
+extern "C" int uwg_stokes_set_nn_monitor(uwg_stokes* st, int window, double min_drop) {
+  if (window < 1 || !(min_drop >= 0.0 && min_drop < 1.0)) return arg_error("window >= 1 and 0 <= min_drop < 1");
+  st->nn_window = window;
+  st->nn_min_drop = min_drop;
   return 0;
 }
 

@@ -12,6 +12,9 @@ void nn_destroy(NNModel* m);
 // model acts on; outputs are scaled by rms(residual) / eta_ref
 void nn_set_feature(NNModel* m, const float* feat_host, double eta_ref);
 // x = NN(feature, r) on the nodes of level g (FP32); Dirichlet dofs of x are zero
+// x = (accumulate ? x : 0) + omega d with omega = (r . Kd)/(Kd . Kd) clamped to [0, 2]
+void nn_line_search(long n, const float* r, const float* kd, const float* d, float* x, bool accumulate,
+                    hipStream_t s);
 void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s);
 long nn_param_count(NNModel* m);
 
