@@ -82,7 +82,8 @@ int uwg_swarm_set_integration(uwg_swarm *s, const double *xi_slot, const double 
 int uwg_advect_rk2(uwg_swarm *s, uwg_field *vel, double dt, int update_owners, long *outside);
 /* SwarmAdvector.get_max_dt (SwarmAdvector.c:370-385) */
 int uwg_advect_max_dt(uwg_field *vel, double *dt);
-/* interpolate a nodal field at points (FeVariable_InterpolateValueAt, FeVariable.c:389-403) */
+/* interpolate a nodal field of 1-9 components at points, v[n][ncomp]
+ * (FeVariable_InterpolateValueAt, FeVariable.c:389-403) */
 int uwg_field_interpolate(uwg_field *f, long n, const double *x, double *v, int *ok);
 
 /* ---- Stokes: uw.systems.Stokes(vel, p, fn_viscosity, fn_bodyforce, voronoi_swarm, conditions)
@@ -94,6 +95,20 @@ int uwg_stokes_destroy(uwg_stokes *st);
 int uwg_stokes_set_materials(uwg_stokes *st, int nmat, const double *eta, const double *force);
 /* per-particle function values (general fn_viscosity / fn_bodyforce evaluated at particles) */
 int uwg_stokes_set_particle_values(uwg_stokes *st, long P, const double *eta, const double *force);
+/* a swarm variable (SwarmVariable.data, [P][ncomp] doubles, ncomp 1-3) copied to the device
+ * under a slot 0-7 for uwg_stokes_eval_particle_functions (SwarmVariableFn.cpp) */
+int uwg_swarm_set_var(uwg_swarm *s, int slot, int ncomp, const double *data);
+/* per-particle viscosity and body force evaluated on the device from a compiled function
+ * graph (PY/function/_function.py, branching.py, math.py; LU/Underworld/Function/src
+ * Binary.cpp, Relational.cpp, Map.cpp, Conditional.cpp, FeVariableFn.cpp): prog[nins][5] =
+ * (op, dst, a, b, n) register instructions, consts[nconst], nodal fields[nfields] (1-3
+ * components, interpolated at each particle); out_eta / out_force name the result registers
+ * (-1: not produced).  Replaces uwg_stokes_set_particle_values without a host round trip. */
+int uwg_stokes_eval_particle_functions(uwg_stokes *st, uwg_swarm *s, int nins, const int *prog, int nconst,
+                                       const double *consts, int nfields, uwg_field *const *fields, int out_eta,
+                                       int out_force);
+/* the per-particle viscosity [P] and body force [P][dim] last set or evaluated */
+int uwg_stokes_get_particle_values(uwg_stokes *st, long P, double *eta, double *force);
 /* element assembly from the swarm's integration points: K (A6), f (A7), approxS (A9)
  * (SystemLinearEquations_MatrixSetup/VectorSetup, SystemLinearEquations.c:559-595) */
 int uwg_stokes_assemble(uwg_stokes *st, uwg_swarm *s);

@@ -74,6 +74,10 @@ SIGNATURES = {
     "uwg_stokes_destroy": [_vp],
     "uwg_stokes_set_materials": [_vp, ctypes.c_int, _dp, _dp],
     "uwg_stokes_set_particle_values": [_vp, ctypes.c_long, _dp, _dp],
+    "uwg_swarm_set_var": [_vp, ctypes.c_int, ctypes.c_int, _dp],
+    "uwg_stokes_get_particle_values": [_vp, ctypes.c_long, _dp, _dp],
+    "uwg_stokes_eval_particle_functions": [_vp, _vp, ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_int,
+                                           ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int],
     "uwg_stokes_assemble": [_vp, _vp],
     "uwg_stokes_assemble_fields": [_vp, _dp, _dp],
     "uwg_solve_opts_default": [ctypes.POINTER(SolveOpts)],

@@ -21,6 +21,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
 UNITS = {
     "swarm_kernels.hip": ["-ffp-contract=off"],
     "relocate.hip": ["-ffp-contract=off"],
+    "fn_kernels.hip": ["-ffp-contract=off"],
     "fem_kernels.hip": ["-ffp-contract=fast"],
     "nn_kernels.hip": ["-ffp-contract=fast"],
     "uwg_host.hip": ["-ffp-contract=off"],

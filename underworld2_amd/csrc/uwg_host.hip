@@ -83,6 +83,10 @@ struct uwg_swarm {
   long integration_state = -1;
   RelocWork* reloc = nullptr;
   long reloc_P = -1;
+  // swarm variables uploaded for device function evaluation: [P][var_nc] doubles per slot
+  double* var[8] = {};
+  int var_nc[8] = {};
+  long var_rows[8] = {};
 };
 
 namespace uwg {
@@ -197,6 +201,11 @@ static T* wsbuf(uwg_stokes* st, WsSlot slot, long n) {
 static int arg_error(const char* m) {
   set_error(m);
   return UWG_ERR_ARG;
+}
+static int arg_error(const std::string& m) { return arg_error(m.c_str()); }
+static int domain_error(const char* m) {
+  set_error(m);
+  return UWG_ERR_DOMAIN;
 }
 
 extern "C" const char* uwg_last_error(void) { return g_err.c_str(); }
@@ -349,6 +358,7 @@ extern "C" int uwg_swarm_destroy(uwg_swarm* s) {
                   (void*)s->xi_slot, (void*)s->w_slot, (void*)s->slot_cell})
     dfree(p);
   reloc_destroy(s->reloc);
+  for (int v = 0; v < 8; v++) dfree(s->var[v]);
   delete s;
   return 0;
 }
@@ -632,15 +642,16 @@ extern "C" int uwg_advect_max_dt(uwg_field* vel, double* dt) {
 }
 
 extern "C" int uwg_field_interpolate(uwg_field* f, long n, const double* x, double* v, int* ok) {
-  if (f->on_elements || f->ncomp != f->mesh->g.dim) return arg_error("nodal vector field required");
+  if (f->on_elements) return arg_error("nodal field required");
+  if (f->ncomp < 1 || f->ncomp > 9) return arg_error("fields of 1 to 9 components can be interpolated");
   API_BEGIN
-  int dim = f->mesh->g.dim;
+  const int dim = f->mesh->g.dim, nc = f->ncomp;
   double* dx = dalloc<double>(n * dim);
-  double* dv = dalloc<double>(n * dim);
+  double* dv = dalloc<double>(n * nc);
   int* dok = dalloc<int>(n);
   UWG_HIP(hipMemcpy(dx, x, sizeof(double) * n * dim, hipMemcpyHostToDevice));
-  launch_interpolate(f->mesh->g, n, f->d, dx, dv, dok, 0);
-  UWG_HIP(hipMemcpy(v, dv, sizeof(double) * n * dim, hipMemcpyDeviceToHost));
+  launch_field_interpolate(f->mesh->g, n, f->d, nc, dx, dv, dok, 0);
+  UWG_HIP(hipMemcpy(v, dv, sizeof(double) * n * nc, hipMemcpyDeviceToHost));
   UWG_HIP(hipMemcpy(ok, dok, sizeof(int) * n, hipMemcpyDeviceToHost));
   dfree(dx);
   dfree(dv);
@@ -1313,6 +1324,123 @@ extern "C" int uwg_stokes_set_particle_values(uwg_stokes* st, long P, const doub
   st->np_vals = P;
   st->mode = 1;
   st->assembled_state = -1;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_swarm_set_var(uwg_swarm* s, int slot, int ncomp, const double* data) {
+  if (slot < 0 || slot >= 8) return arg_error("swarm variable slot must be 0..7");
+  if (ncomp < 1 || ncomp > 3) return arg_error("swarm variables of 1 to 3 components");
+  API_BEGIN
+  if (s->var_rows[slot] != s->P || s->var_nc[slot] != ncomp) {
+    dfree(s->var[slot]);
+    s->var[slot] = dalloc<double>(s->P * ncomp);
+    s->var_rows[slot] = s->P;
+    s->var_nc[slot] = ncomp;
+  }
+  UWG_HIP(hipMemcpy(s->var[slot], data, sizeof(double) * s->P * ncomp, hipMemcpyHostToDevice));
+  return 0;
+  API_END
+}
+
+// Per-particle viscosity and body force from a compiled function program (fn_kernels.hip):
+// prog[nins][5] = (op, dst, a, b, n).  Checked here so that every index the kernel follows
+// stays inside its arrays.
+extern "C" int uwg_stokes_eval_particle_functions(uwg_stokes* st, uwg_swarm* s, int nins, const int* prog, int nconst,
+                                                  const double* consts, int nfields, uwg_field* const* fields,
+                                                  int out_eta, int out_force) {
+  const int dim = st->g.dim;
+  if (nins < 1 || nins > 4096) return arg_error("function program must hold 1..4096 instructions");
+  if (nfields < 0 || nfields > 8) return arg_error("at most 8 mesh fields per program");
+  if (nconst < 0) return arg_error("bad constant count");
+  for (int f = 0; f < nfields; f++)
+    if (!fields[f] || fields[f]->on_elements || fields[f]->ncomp < 1 || fields[f]->ncomp > 3 ||
+        fields[f]->mesh != s->mesh)
+      return arg_error("program fields must be nodal fields of 1-3 components on the swarm's mesh");
+  int nc_reg[48];
+  for (int r = 0; r < 48; r++) nc_reg[r] = 0;
+  for (int i = 0; i < nins; i++) {
+    const int op = prog[5 * i], dst = prog[5 * i + 1], a = prog[5 * i + 2], b = prog[5 * i + 3], n = prog[5 * i + 4];
+    auto reg_ok = [&](int r) { return r >= 0 && r < 48 && nc_reg[r] > 0; };
+    if (dst < 0 || dst >= 48 || n < 1 || n > 3) return arg_error("function program: bad destination");
+    bool ok = true;
+    switch (op) {
+      case 0: ok = a >= 0 && a + n <= nconst; break;                                   // CONST
+      case 1: ok = n == dim; break;                                                    // COORD
+      case 2: ok = a >= 0 && a < nfields && fields[a]->ncomp == n; break;              // FIELD
+      case 3: ok = a >= 0 && a < 8 && s->var[a] && s->var_rows[a] == s->P && s->var_nc[a] == n; break;
+      case 4: ok = s->has_mat && n == 1; break;                                        // MAT
+      case 10: case 11: case 12: case 13: case 14: case 15: case 16: case 17: case 18: case 19: case 20:
+        ok = reg_ok(a) && reg_ok(b) && (nc_reg[a] == n || nc_reg[a] == 1) && (nc_reg[b] == n || nc_reg[b] == 1);
+        break;
+      case 30: case 31: case 32: case 33: case 34: case 35: case 36: case 37:
+        ok = reg_ok(a) && nc_reg[a] == n;
+        break;
+      case 40: ok = reg_ok(a) && reg_ok(b) && n == 1 && (nc_reg[b] == nc_reg[a] || nc_reg[b] == 1); break;
+      case 41: ok = reg_ok(a) && n == 1 && b >= 0 && b < nc_reg[a]; break;             // INDEX
+      case 42: ok = reg_ok(a) && reg_ok(b) && nc_reg[a] + nc_reg[b] == n; break;       // APPEND
+      case 51: ok = reg_ok(a) && b >= 0 && b < nconst && n == 1; break;                // EQK
+      case 52: ok = reg_ok(a) && reg_ok(b) && nc_reg[dst] == n && nc_reg[a] == n; break; // SELECT
+      case 53: ok = reg_ok(a); break;                                                  // CHECK
+      case 54: ok = reg_ok(a) && nc_reg[a] == n; break;                                // MOVE
+      default: ok = false;
+    }
+    if (!ok) return arg_error("function program: invalid instruction " + std::to_string(i));
+    if (op != 53) nc_reg[dst] = n;
+  }
+  if (out_eta >= 0 && (out_eta >= 48 || nc_reg[out_eta] != 1)) return arg_error("viscosity must be a scalar");
+  if (out_force >= 0 && (out_force >= 48 || (nc_reg[out_force] != dim && nc_reg[out_force] != 1)))
+    return arg_error("fn_bodyforce must be a vector of the mesh dimension");
+  API_BEGIN
+  const long P = s->P;
+  if (st->np_vals != P || !st->eta_p) {
+    dfree(st->eta_p);
+    dfree(st->force_p);
+    st->eta_p = dalloc<double>(P);
+    st->force_p = dalloc<double>(P * dim);
+    st->np_vals = P;
+  }
+  int* dprog = dalloc<int>(5L * nins);
+  double* dconst = dalloc<double>(std::max(nconst, 1));
+  int* derr = dalloc<int>(2);
+  UWG_HIP(hipMemcpy(dprog, prog, sizeof(int) * 5 * nins, hipMemcpyHostToDevice));
+  if (nconst) UWG_HIP(hipMemcpy(dconst, consts, sizeof(double) * nconst, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemset(derr, 0, sizeof(int) * 2));
+  const double* fptr[8] = {};
+  int fnc[8] = {};
+  for (int f = 0; f < nfields; f++) {
+    fptr[f] = fields[f]->d;
+    fnc[f] = fields[f]->ncomp;
+  }
+  const double* vptr[8];
+  int vnc[8];
+  for (int v = 0; v < 8; v++) {
+    vptr[v] = s->var[v];
+    vnc[v] = s->var_nc[v];
+  }
+  launch_fn_eval(st->g, P, s->coords, s->mat, dprog, nins, dconst, fptr, fnc, nfields, vptr, vnc, 8, out_eta,
+                 out_force, st->eta_p, st->force_p, derr, 0);
+  int herr[2];
+  UWG_HIP(hipMemcpy(herr, derr, sizeof(int) * 2, hipMemcpyDeviceToHost));
+  dfree(dprog);
+  dfree(dconst);
+  dfree(derr);
+  if (herr[0] == 1) return domain_error("a mesh field was evaluated outside the mesh");
+  if (herr[0] == 2) return arg_error("branching.map / conditional: no mapping or clause for some particles");
+  if (herr[0] == 3) return arg_error("function program: unknown operation");
+  if (herr[1]) return arg_error("viscosity must be positive everywhere");
+  st->mode = 1;
+  st->assembled_state = -1;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_stokes_get_particle_values(uwg_stokes* st, long P, double* eta, double* force) {
+  if (st->mode != 1 || !st->eta_p) return arg_error("no per-particle values");
+  if (P != st->np_vals) return arg_error("particle count mismatch");
+  API_BEGIN
+  if (eta) UWG_HIP(hipMemcpy(eta, st->eta_p, sizeof(double) * P, hipMemcpyDeviceToHost));
+  if (force) UWG_HIP(hipMemcpy(force, st->force_p, sizeof(double) * P * st->g.dim, hipMemcpyDeviceToHost));
   return 0;
   API_END
 }

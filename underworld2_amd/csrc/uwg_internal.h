@@ -109,6 +109,13 @@ void set_apply_chunk(int kz3, int kz2);
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap);
 bool dvc_wave_supported(int dim, int num);
 long dvc_wave_bound_shorts(int cap);
+// fn_kernels.hip: per-particle function programs and general field interpolation
+void launch_fn_eval(const Geom& g, long P, const double* coords, const int* mat, const void* prog, int nins,
+                    const double* consts, const double* const* fields, const int* field_nc, int nfields,
+                    const double* const* vars, const int* var_nc, int nvars, int out_eta, int out_force, double* eta,
+                    double* force, int* err, hipStream_t s);
+void launch_field_interpolate(const Geom& g, long P, const double* f, int nc, const double* x, double* v, int* ok,
+                              hipStream_t s);
 void launch_dvc_wave(int dim, int num, long E, const int* cell_ptr, double* xi_slot, double* w_slot, short* gbound,
                      int nblk, int cap, int* flag, hipStream_t s);
 void launch_dvc_list(int dim, int num, long E0, long nE, const int* elist, const int* cell_ptr,

@@ -312,3 +312,191 @@ class _Misc:
 
 
 misc = _Misc()
+
+
+# ------------------------------------------------------------------------------------------
+# device programs: the graph compiled for per-particle evaluation on the GPU
+# (the instruction set of fn_kernels.hip)
+# ------------------------------------------------------------------------------------------
+class NotCompilable(Exception):
+    """The graph holds a node the device evaluator does not provide."""
+
+
+_BIN = {np.add: 10, np.subtract: 11, np.multiply: 12, np.divide: 13, np.power: 14, np.less: 15,
+        np.less_equal: 16, np.greater: 17, np.greater_equal: 18, np.logical_and: 19, np.logical_or: 20}
+_UN = {np.negative: 30, np.exp: 31, np.log: 32, np.sqrt: 33, np.abs: 34, np.sin: 35, np.cos: 36}
+_NREG = 48
+
+
+class DeviceProgram:
+    """Register program for a set of functions evaluated at every particle of a swarm."""
+
+    def __init__(self, dim):
+        self.dim = dim
+        self.ins = []
+        self.consts = []
+        self.fields = []      # MeshVariable objects, by field index
+        self.swarmvars = []   # SwarmVariable objects, by slot
+        self.uses_material = None
+        self._memo = {}
+        self._nreg = 0
+
+    # registers and constants
+    def _reg(self):
+        r = self._nreg
+        self._nreg += 1
+        if r >= _NREG:
+            raise NotCompilable("function graph too large for the device evaluator")
+        return r
+
+    def _emit(self, op, dst, a=-1, b=-1, n=1):
+        self.ins.append((op, dst, a, b, n))
+        return dst
+
+    def _const(self, values):
+        values = [float(v) for v in np.atleast_1d(np.asarray(values, dtype=np.float64)).ravel()]
+        if not 1 <= len(values) <= 3:
+            raise NotCompilable("constants of 1-3 components only")
+        off = len(self.consts)
+        self.consts.extend(values)
+        r = self._reg()
+        self._emit(0, r, off, -1, len(values))
+        return r, len(values)
+
+    def compile(self, f):
+        """(register, components) holding f at the particle."""
+        key = id(f)
+        if key in self._memo:
+            return self._memo[key]
+        out = self._compile(f)
+        self._memo[key] = out
+        return out
+
+    def _compile(self, f):
+        from .mesh import MeshVariable
+        from .swarm import SwarmVariable
+        if isinstance(f, _Wrap):
+            return self.compile(f.obj)
+        if isinstance(f, Constant):
+            return self._const(f.value.astype(np.float64))
+        if isinstance(f, _Input):
+            r = self._reg()
+            return self._emit(1, r, n=self.dim), self.dim
+        if isinstance(f, MeshVariable):
+            if f._on_elements or not 1 <= f.nodeDofCount <= 3:
+                raise NotCompilable("only nodal mesh variables of 1-3 components")
+            if len(self.fields) >= 8 and f not in self.fields:
+                raise NotCompilable("at most 8 mesh variables")
+            if f not in self.fields:
+                self.fields.append(f)
+            r = self._reg()
+            return self._emit(2, r, self.fields.index(f), -1, f.nodeDofCount), f.nodeDofCount
+        if isinstance(f, SwarmVariable):
+            if not 1 <= f.count <= 3:
+                raise NotCompilable("only swarm variables of 1-3 components")
+            if f not in self.swarmvars:
+                if len(self.swarmvars) >= 8:
+                    raise NotCompilable("at most 8 swarm variables")
+                self.swarmvars.append(f)
+            r = self._reg()
+            return self._emit(3, r, self.swarmvars.index(f), -1, f.count), f.count
+        if isinstance(f, _Binary):
+            if f.op not in _BIN:
+                raise NotCompilable(f"binary operation {f.op}")
+            ra, na = self.compile(f.a)
+            rb, nb = self.compile(f.b)
+            if na != nb and 1 not in (na, nb):
+                raise NotCompilable("operands of incompatible shapes")
+            r = self._reg()
+            n = max(na, nb)
+            return self._emit(_BIN[f.op], r, ra, rb, n), n
+        if isinstance(f, _Unary):
+            if f.op not in _UN:
+                raise NotCompilable(f"unary operation {f.op}")
+            ra, na = self.compile(f.a)
+            r = self._reg()
+            return self._emit(_UN[f.op], r, ra, -1, na), na
+        if isinstance(f, _Index):
+            ra, na = self.compile(f.a)
+            if not 0 <= f.i < na:
+                raise NotCompilable("component index out of range")
+            r = self._reg()
+            return self._emit(41, r, ra, f.i, 1), 1
+        if isinstance(f, _Stack):
+            ra, na = self.compile(f.items[0])
+            for item in f.items[1:]:
+                rb, nb = self.compile(item)
+                if na + nb > 3:
+                    raise NotCompilable("vectors of more than 3 components")
+                r = self._reg()
+                self._emit(42, r, ra, rb, na + nb)
+                ra, na = r, na + nb
+            return ra, na
+        if type(f).__name__ == "_Dot":
+            ra, na = self.compile(f.a)
+            rb, nb = self.compile(f.b)
+            if na != nb and nb != 1:
+                raise NotCompilable("dot of incompatible shapes")
+            r = self._reg()
+            return self._emit(40, r, ra, rb, 1), 1
+        if isinstance(f, _Map):
+            rk, nk = self.compile(f.key)
+            vals = [(k, self.compile(v)) for k, v in f.mapping.items()]
+            dflt = self.compile(f.default) if f.default is not None else None
+            n = max([nv for _, (_, nv) in vals] + ([dflt[1]] if dflt else []))
+            if any(nv != n for _, (_, nv) in vals) or (dflt and dflt[1] != n):
+                raise NotCompilable("map values of different shapes")
+            res, _ = self._const([0.0] * n)
+            found, _ = self._const([0.0])
+            for k, (rv, _) in vals:
+                kc = len(self.consts)
+                self.consts.append(float(k))
+                m = self._reg()
+                self._emit(51, m, rk, kc, 1)
+                self._emit(52, res, rv, m, n)
+                f2 = self._reg()
+                self._emit(20, f2, found, m, 1)
+                found = f2
+            if dflt is not None:
+                nf = self._reg()
+                self._emit(37, nf, found, -1, 1)
+                self._emit(52, res, dflt[0], nf, n)
+            else:
+                self._emit(53, 0, found, -1, 1)
+            return res, n
+        if isinstance(f, _Conditional):
+            clauses = [(self.compile(c), self.compile(v)) for c, v in f.clauses]
+            n = max(nv for _, (_, nv) in clauses)
+            if any(nv != n for _, (_, nv) in clauses) or any(nc != 1 for (_, nc), _ in clauses):
+                raise NotCompilable("conditional clauses of different shapes")
+            res, _ = self._const([0.0] * n)
+            done, _ = self._const([0.0])
+            for (rc, _), (rv, _) in clauses:
+                nd = self._reg()
+                self._emit(37, nd, done, -1, 1)
+                take = self._reg()
+                self._emit(19, take, rc, nd, 1)
+                self._emit(52, res, rv, take, n)
+                d2 = self._reg()
+                self._emit(20, d2, done, rc, 1)
+                done = d2
+            self._emit(53, 0, done, -1, 1)
+            return res, n
+        raise NotCompilable(f"{type(f).__name__} is evaluated on the host")
+
+    def packed(self):
+        return (np.ascontiguousarray(np.array(self.ins, dtype=np.int32).reshape(-1, 5)),
+                np.ascontiguousarray(np.array(self.consts if self.consts else [0.0], dtype=np.float64)))
+
+
+def compile_particle_program(fn_viscosity, fn_bodyforce, dim):
+    """Compile viscosity (scalar) and body force (dim-vector) for device evaluation.
+    Returns (program, eta_register, force_register); raises NotCompilable."""
+    prog = DeviceProgram(dim)
+    re, ne = prog.compile(fn_viscosity)
+    if ne != 1:
+        raise NotCompilable("viscosity must be a scalar")
+    rf, nf = prog.compile(fn_bodyforce)
+    if nf not in (1, dim):
+        raise NotCompilable("fn_bodyforce must be a vector of the mesh dimension")
+    return prog, re, rf

@@ -14,6 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
+from . import function as _fn
 
 
 class IndexSet:
@@ -127,8 +128,9 @@ class FeMesh_Cartesian:
         return v.value
 
 
-class MeshVariable:
-    """Nodal (or dQ0 element) field, ``MeshVariable(mesh, nodeDofCount)``."""
+class MeshVariable(_fn.Function):
+    """Nodal (or dQ0 element) field, ``MeshVariable(mesh, nodeDofCount)``; as a function it
+    is interpolated at the evaluation points (FeVariableFn.cpp)."""
 
     def __init__(self, mesh, nodeDofCount, dataType="double", **kwargs):
         if dataType != "double":
@@ -147,6 +149,7 @@ class MeshVariable:
         self._host = np.zeros((n, self.nodeDofCount))
         self._dev_newer = False
         self._host_touched = False
+        self.version = 0  # bumps whenever the values may have changed
 
     def __del__(self):
         try:
@@ -162,6 +165,7 @@ class MeshVariable:
             L.check(L.lib().uwg_field_get(self._h, L.dptr(self._host)), "MeshVariable.data")
             self._dev_newer = False
         self._host_touched = True
+        self.version += 1
         return self._host
 
     @data.setter
@@ -177,12 +181,15 @@ class MeshVariable:
     def _mark_device_written(self):
         self._dev_newer = True
         self._host_touched = False
+        self.version += 1
 
-    def evaluate(self, coords):
-        """Interpolate a nodal vector field at points (FeVariable_InterpolateValueAt)."""
+    def _interpolate(self, coords):
+        """Interpolate the nodal field at points (FeVariable_InterpolateValueAt)."""
+        if self._on_elements:
+            raise NotImplementedError("evaluation of sub-mesh (element) variables is not provided by this engine")
         self._push()
         x = L.f64(coords).reshape(-1, self._base.dim)
-        v = np.zeros_like(x)
+        v = np.zeros((x.shape[0], self.nodeDofCount))
         ok = np.zeros(x.shape[0], dtype=np.int32)
         L.check(L.lib().uwg_field_interpolate(self._h, x.shape[0], L.dptr(x), L.dptr(v), L.iptr(ok)), "evaluate")
         if not ok.all():
@@ -194,4 +201,4 @@ class MeshVariable:
         return {("meshvar", id(self))}
 
     def _fn_eval(self, ctx):
-        return self.evaluate(ctx.coords)
+        return self._interpolate(ctx.coords)

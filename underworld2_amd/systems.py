@@ -170,7 +170,20 @@ class Stokes:
                         "materials")
                 self._values_key = ("table", id(var), var.version, sw.stateId)
         else:
+            try:
+                prog, r_eta, r_force = _fn.compile_particle_program(self._fn_viscosity, self._fn_bodyforce, d)
+            except _fn.NotCompilable:
+                prog = None
+            if prog is not None:
+                self._eval_on_device(sw, prog, r_eta, r_force)
+                L.check(lib.uwg_stokes_assemble(self._h, sw._h), "assemble")
+                return
+            # host evaluation (graphs the device program does not cover); a dependence on a mesh
+            # variable always re-evaluates
             key = ("particles", sw.stateId) + tuple(v.version for v in sw._variables)
+            deps = self._fn_viscosity._fn_inputs() | self._fn_bodyforce._fn_inputs()
+            if any(t[0] == "meshvar" for t in deps):
+                key = key + (object(),)
             if key != self._values_key:
                 eta = np.asarray(self._fn_viscosity.evaluate(sw), dtype=np.float64).reshape(-1)
                 force = np.asarray(self._fn_bodyforce.evaluate(sw), dtype=np.float64).reshape(eta.shape[0], -1)
@@ -180,6 +193,29 @@ class Stokes:
                                                            L.dptr(L.f64(force))), "particle values")
                 self._values_key = ("particles", sw.stateId) + tuple(v.version for v in sw._variables)
         L.check(lib.uwg_stokes_assemble(self._h, sw._h), "assemble")
+
+    def _eval_on_device(self, sw, prog, r_eta, r_force):
+        """Per-particle viscosity and body force evaluated by the device program (swarm
+        variables uploaded when they changed, mesh variables read where they live)."""
+        lib = L.lib()
+        key = (("device", id(prog.__class__), sw.stateId, tuple(prog.ins), tuple(prog.consts))
+               + tuple(v.version for v in prog.swarmvars) + tuple(f.version for f in prog.fields))
+        if key == self._values_key:
+            return
+        for slot, v in enumerate(prog.swarmvars):
+            raw = v._data if v._data.shape[0] == sw.particleLocalCount else v.data
+            vals = np.ascontiguousarray(raw, dtype=np.float64)
+            L.check(lib.uwg_swarm_set_var(sw._h, slot, v.count, L.dptr(vals)), "swarm variable upload")
+        for f in prog.fields:
+            f._push()
+        ins, consts = prog.packed()
+        nf = len(prog.fields)
+        handles = (ctypes.c_void_p * max(1, nf))(*[f._h for f in prog.fields])
+        L.check(lib.uwg_stokes_eval_particle_functions(self._h, sw._h, ins.shape[0], L.iptr(ins), consts.size,
+                                                       L.dptr(consts), nf, handles, r_eta, r_force),
+                "function evaluation")
+        self._values_key = (("device", id(prog.__class__), sw.stateId, tuple(prog.ins), tuple(prog.consts))
+                            + tuple(v.version for v in prog.swarmvars) + tuple(f.version for f in prog.fields))
 
     # --- analysis -------------------------------------------------------------------------
     @property
