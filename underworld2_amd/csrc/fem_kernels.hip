@@ -468,22 +468,13 @@ __device__ __forceinline__ void load_level3(const KArgs<T>& A, const T* __restri
 //   APPLY_CHEB0    the same with d taken as zero (the first step from a nonzero guess)
 // The Chebyshev forms write the new iterate into y, a buffer distinct from x (other tiles
 // still read x), so the smoother alternates two buffers.
-//   APPLY_STOKES   y = kx + (G p)_n on free rows, and q_e = (G^T u)_e for the elements the
-//                  tile owns: the coupled operator [K G; G^T 0] in one pass
-enum ApplyMode { APPLY_Y = 0, APPLY_RESIDUAL = 1, APPLY_CHEB = 2, APPLY_CHEB0 = 3, APPLY_STOKES = 4 };
-// the element gradient matrix G_e, identical for every element of a regular mesh
-struct GeConst {
-  double v[24];
-};
+enum ApplyMode { APPLY_Y = 0, APPLY_RESIDUAL = 1, APPLY_CHEB = 2, APPLY_CHEB0 = 3 };
 template <typename T>
 struct SmoothArgs {
   const T* b = nullptr;
   const T* dinv = nullptr;
   T* d = nullptr;
   T a = 0, c = 0;
-  const T* p = nullptr;  // APPLY_STOKES: pressure per element (input)
-  T* q = nullptr;        //               G^T u per element (output)
-  GeConst Ge;
 };
 template <typename T, int MODE>
 __device__ __forceinline__ void apply_epilogue(const SmoothArgs<T>& S, const T* __restrict__ x, T* __restrict__ y,
@@ -538,15 +529,6 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T>
     if (colE && ek >= 0 && ek < A.rz) {
       const long e = ei + (long)A.rx * (ej + (long)A.ry * ek);
       elem3<T>([&](int q) { return W[(long)q * A.Ws + e]; }, u, r, A.sx, A.sy, A.sz);
-      if (MODE == APPLY_STOKES && lx >= 1 && ly >= 1 && ek >= k0) {
-        // G^T u for the element this tile owns (u is zero on Dirichlet dofs)
-        T acc = 0;
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-          for (int d = 0; d < 3; d++) acc += (T)S.Ge.v[a * 3 + d] * u[a][d];
-        S.q[e] = acc;
-      }
     } else {
 #pragma unroll
       for (int a = 0; a < 8; a++)
@@ -561,31 +543,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T>
     if (ownN && ek >= k0) {
       long n = ei + A.nx * ej + A.nxy * ek;
       uint8_t m = bc[n];
-      T gp[3] = {0, 0, 0};
-      if (MODE == APPLY_STOKES) {
-        // (G p)_n: the elements around node n in (z, y, x) order, node n being corner a of each
-#pragma unroll
-        for (int oz = 0; oz < 2; oz++)
-#pragma unroll
-          for (int oy = 0; oy < 2; oy++)
-#pragma unroll
-            for (int ox = 0; ox < 2; ox++) {
-              const int gi = ei - 1 + ox, gj = ej - 1 + oy, gk = ek - 1 + oz;
-              if (gi < 0 || gi >= A.rx || gj < 0 || gj >= A.ry || gk < 0 || gk >= A.rz) continue;
-              const T pe = S.p[gi + (long)A.rx * (gj + (long)A.ry * gk)];
-              const int a = (1 - ox) + 2 * (1 - oy) + 4 * (1 - oz);
-#pragma unroll
-              for (int d = 0; d < 3; d++) gp[d] += (T)S.Ge.v[a * 3 + d] * pe;
-            }
-      }
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         T s = st[9 + i][tid - 17] + st[6 + i][tid - 16] + st[3 + i][tid - 1] + st[i][tid] + sb[9 + i][tid - 17] +
               sb[6 + i][tid - 16] + sb[3 + i][tid - 1] + sb[i][tid];
-        if (MODE == APPLY_STOKES)
-          y[n * 3 + i] = ((m >> i) & 1) ? x[n * 3 + i] : s + gp[i];
-        else
-          apply_epilogue<T, MODE>(S, x, y, n * 3 + i, ((m >> i) & 1) ? x[n * 3 + i] : s);
+        apply_epilogue<T, MODE>(S, x, y, n * 3 + i, ((m >> i) & 1) ? x[n * 3 + i] : s);
       }
     }
     __syncthreads();
@@ -767,29 +729,6 @@ void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* 
 template void launch_apply_K<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t);
 template void launch_apply_K<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t);
 
-// [y; q] = [K G; G^T 0] [x; p] on the free dofs (identity rows on Dirichlet dofs): one fused
-// pass in 3D; in 2D the separate operator kernels
-template <typename T>
-void launch_apply_stokes(const Geom& g, const T* W, const double* Ge, const T* x, const T* p, T* y, T* q, T* gp_tmp,
-                         const uint8_t* bc, hipStream_t s) {
-  if (g.dim == 3) {
-    SmoothArgs<T> S;
-    S.p = p;
-    S.q = q;
-    for (int k = 0; k < 24; k++) S.Ge.v[k] = Ge[k];
-    launch_apply_mode<T, APPLY_STOKES>(g, W, x, y, bc, S, s);
-  } else {
-    launch_apply_K<T>(g, W, x, y, bc, s);
-    launch_apply_G<T>(g, Ge, p, bc, gp_tmp, s);
-    vk_axpy<T>((long)g.N * g.dim, (T)1, gp_tmp, y, s);
-    launch_apply_GT<T>(g, Ge, x, bc, q, s);
-  }
-}
-template void launch_apply_stokes<double>(const Geom&, const double*, const double*, const double*, const double*,
-                                          double*, double*, double*, const uint8_t*, hipStream_t);
-template void launch_apply_stokes<float>(const Geom&, const float*, const double*, const float*, const float*,
-                                         float*, float*, float*, const uint8_t*, hipStream_t);
-
 // r = b - K x
 template <typename T>
 void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s) {
@@ -826,6 +765,10 @@ template void launch_cheb_step_K<float>(const Geom&, const float*, const float*,
 // G p (node gather) and G^T u (element gather); G_e constant on a regular mesh
 // (_GradientStiffnessMatrixTerm_AssembleElement, GradientStiffnessMatrixTerm.c:129-221)
 // ---------------------------------------------------------------------------------------
+
+struct GeConst {
+  double v[24];
+};
 
 template <typename T, int DIM>
 __global__ void k_apply_G(int rx, int ry, int rz, long N, GeConst Ge, const T* __restrict__ p,

@@ -1714,7 +1714,10 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   launch_neg_recip<double>(E, st->approxS, diagS, 0);
   double* gp = wsbuf<double>(st, WS_BLK_GP, n);
   OpFn Aop = [&](const double* xx, double* yy) {
-    launch_apply_stokes<double>(A.g, A.W, st->Ge, xx, xx + n, yy, yy + n, gp, A.bc, 0);
+    launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
+    launch_apply_G(st->g, st->Ge, xx + n, A.bc, gp, 0);
+    vk_axpy<double>(n, 1.0, gp, yy, 0);
+    launch_apply_GT(st->g, st->Ge, xx, A.bc, yy + n, 0);
   };
   OpFn Mop = [&](const double* rr, double* zz) {
     st->vcycles++;

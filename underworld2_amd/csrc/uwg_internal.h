@@ -88,9 +88,6 @@ void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv
 template <typename T>
 void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
 template <typename T>
-void launch_apply_stokes(const Geom& g, const T* W, const double* Ge, const T* x, const T* p, T* y, T* q, T* gp_tmp,
-                         const uint8_t* bc, hipStream_t s);
-template <typename T>
 void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s);
 template <typename T>
 void launch_cheb_step_K(const Geom& g, const T* W, const T* x, T* xn, const T* b, const T* dinv, T* d, T a, T c,
