@@ -37,6 +37,12 @@ int uwg_device_info(char *buf, int n);
 /* ---- mesh: FeMesh_Cartesian('Q1/dQ0', res, min, max)  PY/mesh/_mesh.py:1046-1190,
  *      CartesianGenerator_CalcGeom LU/StgDomain/Mesh/src/CartesianGenerator.c:2001-2039 */
 int uwg_mesh_create(int dim, const int *res, const double *minc, const double *maxc, uwg_mesh **out);
+/* FeMesh_Cartesian('Q2/dPc1', res, min, max) (PY/mesh/_mesh.py:1105-1190): the same element
+ * grid with (2 res + 1)^dim vertices (C2Generator_GenElementVertexInc
+ * LU/StgFEM/Discretisation/src/C2Generator.c:147-260, element-local vertices lexicographic on
+ * the 3^dim sub-grid) and a dPc1 pressure submesh of dim+1 values per element
+ * (Inner2DGenerator.c:258-283); swarms live on the element grid */
+int uwg_mesh_create_q2(int dim, const int *res, const double *minc, const double *maxc, uwg_mesh **out);
 int uwg_mesh_destroy(uwg_mesh *m);
 /* node coordinates [N][dim] (MeshVariable mesh.data) */
 int uwg_mesh_coords(uwg_mesh *m, double *out);
@@ -44,7 +50,8 @@ int uwg_mesh_coords(uwg_mesh *m, double *out);
 int uwg_mesh_min_separation(uwg_mesh *m, double *out);
 
 /* ---- fields: MeshVariable(mesh, nodeDofCount) PY/mesh/_meshvariable.py; nodal [N][ncomp]
- *      or, with on_elements=1, per-element [E][ncomp] (the dQ0 pressure submesh) */
+ *      (N = vertices of the mesh), with on_elements=1 per element [E][ncomp] (the dQ0
+ *      pressure submesh), with on_elements=2 [E][dim+1][ncomp] (the dPc1 submesh) */
 int uwg_field_create(uwg_mesh *m, int ncomp, int on_elements, uwg_field **out);
 int uwg_field_destroy(uwg_field *f);
 int uwg_field_set(uwg_field *f, const double *host);
@@ -116,6 +123,8 @@ int uwg_stokes_assemble(uwg_stokes *st, uwg_swarm *s);
  * PY/swarm/_integration_swarm.py:156-190, 3 points per dim) from function values at the
  * Gauss points: eta_g[E][3^dim], force_g[E][3^dim][dim] (NULL: zero force) */
 int uwg_stokes_assemble_fields(uwg_stokes *st, const double *eta_g, const double *force_g);
+/*      (Q2/dPc1 systems: the 5^dim Gauss points of the Q2 mesh, point index qx + 5 qy + 25 qz,
+ *      PY/swarm/_integration_swarm.py:173-181) */
 
 typedef struct {
   int method;        /* 0: Schur complement (BSSCR auglag driver, auglag-driver-DGTGD.c:64-584)
@@ -155,11 +164,14 @@ int uwg_stokes_solve(uwg_stokes *st, const uwg_solve_opts *o, uwg_solve_stats *s
 int uwg_stokes_residuals(uwg_stokes *st, double *out3);
 
 /* ---- operator hooks (tests and the learned-part trainer) */
-/* which: 0 = K (level), 1 = G, 2 = G^T, 3 = one multigrid V-cycle on K (level 0) */
+/* which: 0 = K (level), 1 = G, 2 = G^T, 3 = one multigrid V-cycle on K (level 0).
+ * Q2/dPc1 systems: level 0 is the Q2 level (pressure vectors [E][dim+1]), levels 1.. are the
+ * Q1 levels of the multigrid hierarchy below it (K only) */
 int uwg_stokes_apply(uwg_stokes *st, int which, int level, const double *x, double *y);
 int uwg_stokes_levels(uwg_stokes *st, int *nlev, int *res_out /*[nlev][3]*/);
 /* per-element quadrature weights W[3^dim][E] of a level, element force fe[E][2^dim*dim],
- * approxS[E] */
+ * approxS[E]; Q2/dPc1 level 0: W[E][5^dim], fe[E][3^dim][dim], approxS blocks
+ * [E][dim+1][dim+1] (UzawaPreconditionerTerm.c:124-209) */
 int uwg_stokes_get_assembly(uwg_stokes *st, int level, double *W, double *fe, double *approxS);
 int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
 /* ---- learned coarse correction (no counterpart in the reference: the preconditioner

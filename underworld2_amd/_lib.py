@@ -46,6 +46,7 @@ SIGNATURES = {
     "uwg_last_error": [],
     "uwg_device_info": [ctypes.c_char_p, ctypes.c_int],
     "uwg_mesh_create": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
+    "uwg_mesh_create_q2": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
     "uwg_mesh_destroy": [_vp],
     "uwg_mesh_coords": [_vp, _dp],
     "uwg_mesh_min_separation": [_vp, _dp],

@@ -24,6 +24,7 @@ UNITS = {
     "fn_kernels.hip": ["-ffp-contract=off"],
     "fem_kernels.hip": ["-ffp-contract=fast"],
     "nn_kernels.hip": ["-ffp-contract=fast"],
+    "q2_kernels.hip": ["-ffp-contract=fast"],
     "uwg_host.hip": ["-ffp-contract=off"],
 }
 

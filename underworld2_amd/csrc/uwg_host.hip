@@ -40,6 +40,7 @@ enum WsSlot {
   WS_BLK_B, WS_BLK_X, WS_BLK_DS, WS_BLK_Q, WS_BLK_GP,
   WS_SCH_USTAR, WS_SCH_HHAT, WS_SCH_T, WS_SCH_Z, WS_SCH_DS,
   WS_MIX_B, WS_MIX_X, WS_MIX_R, WS_MIX_GP, WS_MIX_R32, WS_MIX_D32, WS_MIX_DS32, WS_MIX_Q32, WS_MIX_GP32,
+  WS_Q2_ETAG, WS_Q2_FORCEG, WS_Q2_BAD,
   WS_COUNT
 };
 static double now() {
@@ -54,7 +55,9 @@ using namespace uwg;
 // handles
 // ---------------------------------------------------------------------------------------
 struct uwg_mesh {
-  Geom g;
+  Geom g;          // element grid (the Q1 vertex grid for order 1)
+  Geom gn;         // vertex grid of the velocity discretisation: g for Q1, 2 res for Q2
+  int order = 1;   // 1: Q1/dQ0, 2: Q2/dPc1
   double min_sep;
 };
 
@@ -112,6 +115,27 @@ struct Coarse {
   double* Ainv = nullptr;  // device, n x n
 };
 
+// Q2/dPc1 finest level: quadratic velocity on the vertex grid gn, d+1 pressure values per
+// element.  Its multigrid correction is the Q1 hierarchy of the same element grid (L[0],
+// Galerkin operator of the embedded trilinear space).
+struct Q2Level {
+  Geom gn;
+  uint8_t* bc = nullptr;
+  double* W = nullptr;      // [E][5^dim]
+  float* W32 = nullptr;
+  double* dinv = nullptr;
+  float* dinv32 = nullptr;
+  double lmax = 0.0;
+  double *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
+  float *x32 = nullptr, *b32 = nullptr, *r32 = nullptr, *d32 = nullptr;
+  double* Gr = nullptr;     // G_e [27*dim or 9*dim][dim+1]
+  double* S = nullptr;      // approxS blocks [E][dim+1][dim+1]
+  double* Sinv = nullptr;   // their inverses
+  float* Sinv32 = nullptr;
+  double* fe = nullptr;     // [E][3^dim][dim]
+  long n() const { return gn.N * gn.dim; }
+};
+
 }  // namespace uwg
 
 struct uwg_stokes {
@@ -119,6 +143,8 @@ struct uwg_stokes {
   uwg_field* p;
   Geom g;
   std::vector<Level> L;
+  Q2Level* q2 = nullptr;      // set for a Q2/dPc1 system
+  long nu = 0, np = 0;        // velocity dofs (all vertices x dim) and pressure dofs
   Coarse coarse;
   bool hierarchy_built = false;
   bool assembled = false;
@@ -258,7 +284,24 @@ extern "C" int uwg_mesh_create(int dim, const int* res, const double* minc, cons
     if (e > 4096 && m->g.E > 100000) break;  // regular spacing: remaining elements repeat
   }
   m->min_sep = ms;
+  m->gn = m->g;
   *out = m;
+  return 0;
+  API_END
+}
+
+// Q2 mesh (C2Generator): (2 res + 1)^dim vertices on the same element grid; the minimum
+// vertex separation of a quadratic element is half the element spacing
+extern "C" int uwg_mesh_create_q2(int dim, const int* res, const double* minc, const double* maxc, uwg_mesh** out) {
+  int rc = uwg_mesh_create(dim, res, minc, maxc, out);
+  if (rc) return rc;
+  API_BEGIN
+  uwg_mesh* m = *out;
+  int r2[3];
+  for (int d = 0; d < 3; d++) r2[d] = d < dim ? 2 * res[d] : 1;
+  m->gn = make_geom(dim, r2, minc, maxc);
+  m->order = 2;
+  m->min_sep *= 0.5;
   return 0;
   API_END
 }
@@ -270,9 +313,9 @@ extern "C" int uwg_mesh_destroy(uwg_mesh* m) {
 
 extern "C" int uwg_mesh_coords(uwg_mesh* m, double* out) {
   API_BEGIN
-  double* d = dalloc<double>(m->g.N * m->g.dim);
-  launch_mesh_coords(m->g, d, 0);
-  UWG_HIP(hipMemcpy(out, d, sizeof(double) * m->g.N * m->g.dim, hipMemcpyDeviceToHost));
+  double* d = dalloc<double>(m->gn.N * m->gn.dim);
+  launch_mesh_coords(m->gn, d, 0);
+  UWG_HIP(hipMemcpy(out, d, sizeof(double) * m->gn.N * m->gn.dim, hipMemcpyDeviceToHost));
   dfree(d);
   return 0;
   API_END
@@ -285,12 +328,14 @@ extern "C" int uwg_mesh_min_separation(uwg_mesh* m, double* out) {
 
 extern "C" int uwg_field_create(uwg_mesh* m, int ncomp, int on_elements, uwg_field** out) {
   if (ncomp < 1) return arg_error("ncomp must be >= 1");
+  if (on_elements < 0 || on_elements > 2) return arg_error("on_elements must be 0 (vertices), 1 (dQ0) or 2 (dPc1)");
   API_BEGIN
   uwg_field* f = new uwg_field();
   f->mesh = m;
   f->ncomp = ncomp;
   f->on_elements = on_elements;
-  f->n = (on_elements ? m->g.E : m->g.N) * ncomp;
+  const long nent = on_elements == 0 ? m->gn.N : (on_elements == 1 ? m->g.E : m->g.E * (m->g.dim + 1));
+  f->n = nent * ncomp;
   f->d = dalloc<double>(f->n);
   UWG_HIP(hipMemset(f->d, 0, sizeof(double) * f->n));
   *out = f;
@@ -607,6 +652,7 @@ extern "C" int uwg_swarm_set_integration(uwg_swarm* s, const double* xi_slot, co
 extern "C" int uwg_advect_rk2(uwg_swarm* s, uwg_field* vel, double dt, int update_owners, long* outside) {
   if (vel->mesh != s->mesh || vel->on_elements || vel->ncomp != s->mesh->g.dim)
     return arg_error("velocity field must be a nodal vector field on the swarm's mesh");
+  if (vel->mesh->order != 1) return arg_error("swarm advection through a Q2 velocity field is not provided");
   API_BEGIN
   int* cnt = dalloc<int>(2);
   UWG_HIP(hipMemset(cnt, 0, sizeof(int) * 2));
@@ -627,7 +673,8 @@ extern "C" int uwg_advect_rk2(uwg_swarm* s, uwg_field* vel, double dt, int updat
 
 extern "C" int uwg_advect_max_dt(uwg_field* vel, double* dt) {
   API_BEGIN
-  const Geom& g = vel->mesh->g;
+  if (vel->on_elements || vel->ncomp != vel->mesh->g.dim) return arg_error("nodal velocity field required");
+  const Geom& g = vel->mesh->gn;
   const int nblk = 512;
   double* part = dalloc<double>(nblk);
   launch_node_speed_max(g, vel->d, part, nblk, 0);
@@ -650,7 +697,10 @@ extern "C" int uwg_field_interpolate(uwg_field* f, long n, const double* x, doub
   double* dv = dalloc<double>(n * nc);
   int* dok = dalloc<int>(n);
   UWG_HIP(hipMemcpy(dx, x, sizeof(double) * n * dim, hipMemcpyHostToDevice));
-  launch_field_interpolate(f->mesh->g, n, f->d, nc, dx, dv, dok, 0);
+  if (f->mesh->order == 2)
+    launch_q2_interpolate(f->mesh->g, n, f->d, nc, dx, dv, dok, 0);
+  else
+    launch_field_interpolate(f->mesh->g, n, f->d, nc, dx, dv, dok, 0);
   UWG_HIP(hipMemcpy(v, dv, sizeof(double) * n * nc, hipMemcpyDeviceToHost));
   UWG_HIP(hipMemcpy(ok, dok, sizeof(int) * n, hipMemcpyDeviceToHost));
   dfree(dx);
@@ -757,6 +807,22 @@ static void ensure_fp32(uwg_stokes* st) {
     }
     vk_d2f(Q * A.g.Es, 1.0, A.W, A.W32, 0);
     vk_d2f(A.n(), 1.0, A.dinv, A.dinv32, 0);
+  }
+  if (st->q2) {
+    Q2Level& A = *st->q2;
+    const long nq = st->g.E * (st->g.dim == 3 ? 125 : 25), n = A.n(), ns = st->np * (st->g.dim + 1);
+    if (!A.W32) {
+      A.W32 = dalloc<float>(nq);
+      A.dinv32 = dalloc<float>(n);
+      A.Sinv32 = dalloc<float>(ns);
+      for (float** v : {&A.x32, &A.b32, &A.r32, &A.d32}) {
+        *v = dalloc<float>(n);
+        vk_fill<float>(n, 0.0f, *v, 0);
+      }
+    }
+    vk_d2f(nq, 1.0, A.W, A.W32, 0);
+    vk_d2f(n, 1.0, A.dinv, A.dinv32, 0);
+    vk_d2f(ns, 1.0, A.Sinv, A.Sinv32, 0);
   }
   st->fp32_ready = true;
 }
@@ -1089,6 +1155,128 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Operators of the finest level (Q1/dQ0 or Q2/dPc1) and its multigrid cycle
+// ---------------------------------------------------------------------------------------
+namespace {
+template <typename T> struct Q2V;
+template <> struct Q2V<double> {
+  static const double* W(Q2Level& A) { return A.W; }
+  static const double* dinv(Q2Level& A) { return A.dinv; }
+  static const double* Sinv(Q2Level& A) { return A.Sinv; }
+  static double* x(Q2Level& A) { return A.x; }
+  static double* b(Q2Level& A) { return A.b; }
+  static double* r(Q2Level& A) { return A.r; }
+  static double* d(Q2Level& A) { return A.d; }
+};
+template <> struct Q2V<float> {
+  static const float* W(Q2Level& A) { return A.W32; }
+  static const float* dinv(Q2Level& A) { return A.dinv32; }
+  static const float* Sinv(Q2Level& A) { return A.Sinv32; }
+  static float* x(Q2Level& A) { return A.x32; }
+  static float* b(Q2Level& A) { return A.b32; }
+  static float* r(Q2Level& A) { return A.r32; }
+  static float* d(Q2Level& A) { return A.d32; }
+};
+}  // namespace
+
+// K on the finest level; bc selects the Dirichlet rows/columns (identity rows)
+template <typename T>
+static void top_K(uwg_stokes* st, const T* x, T* y, const uint8_t* bc) {
+  if (st->q2)
+    launch_q2_apply<T>(st->g, Q2V<T>::W(*st->q2), x, y, bc, 0);
+  else
+    launch_apply_K<T>(st->g, LV<T>::W(st->L[0]), x, y, bc, 0);
+}
+template <typename T>
+static void top_G(uwg_stokes* st, const T* p, T* y, const uint8_t* bc) {
+  if (st->q2)
+    launch_q2_apply_G<T>(st->g, st->q2->Gr, p, bc, y, 0);
+  else
+    launch_apply_G<T>(st->g, st->Ge, p, bc, y, 0);
+}
+template <typename T>
+static void top_GT(uwg_stokes* st, const T* u, T* q, const uint8_t* bc) {
+  if (st->q2)
+    launch_q2_apply_GT<T>(st->g, st->q2->Gr, u, bc, q, 0);
+  else
+    launch_apply_GT<T>(st->g, st->Ge, u, bc, q, 0);
+}
+static const uint8_t* top_bc(uwg_stokes* st) { return st->q2 ? st->q2->bc : st->L[0].bc; }
+static const Geom& top_vgeom(uwg_stokes* st) { return st->q2 ? st->q2->gn : st->g; }
+// Schur preconditioner z = -approxS^-1 r: diagonal (dQ0, diagS = -1/approxS) or element
+// blocks (dPc1)
+template <typename T>
+static void top_schur(uwg_stokes* st, const T* diagS, const T* r, T* z) {
+  if (st->q2)
+    launch_q2_schur_prec<T>(st->g.E, st->g.dim + 1, Q2V<T>::Sinv(*st->q2), r, z, 0);
+  else
+    vk_xmy_scale<T>(st->np, diagS, r, z, 0);
+}
+
+// Chebyshev smoothing on the Q2 level: same window and recurrence as the Q1 smoother
+template <typename T>
+static void q2_chebyshev(uwg_stokes* st, const T* b, T* x, int degree, bool zero_guess) {
+  Q2Level& A = *st->q2;
+  const long n = A.n();
+  const double emax = 1.1 * A.lmax, emin = 0.12 * A.lmax;
+  const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  T* r = Q2V<T>::r(A);
+  T* d = Q2V<T>::d(A);
+  const T* dinv = Q2V<T>::dinv(A);
+  int k0 = 0;
+  if (zero_guess) {
+    vk_cheb_first<T>(n, (T)(1.0 / theta), dinv, b, d, x, 0);
+    k0 = 1;
+  }
+  for (int k = k0; k < degree; k++) {
+    top_K<T>(st, x, r, A.bc);
+    vk_residual<T>(n, b, r, 0);
+    if (!zero_guess && k == 0) {
+      vk_cheb_update<T>(n, (T)0, (T)(1.0 / theta), dinv, r, d, x, 0);
+    } else {
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      vk_cheb_update<T>(n, (T)(rho_new * rho), (T)(2.0 * rho_new / delta), dinv, r, d, x, 0);
+      rho = rho_new;
+    }
+  }
+}
+
+// V-cycle of the Q2 level: smooth, restrict the residual to the embedded Q1 space (the Q2
+// vertices are the vertices of the once-refined Q1 grid, and the Q1 -> Q2 interpolation is the
+// same averaging as Q1 grid refinement), the Q1 cycle, prolong, smooth
+template <typename T>
+static void q2_vcycle(uwg_stokes* st, const T* b, T* x) {
+  Q2Level& A = *st->q2;
+  Level& C = st->L[0];
+  q2_chebyshev<T>(st, b, x, g_smooth_degree, true);
+  T* r = Q2V<T>::r(A);
+  top_K<T>(st, x, r, A.bc);
+  vk_residual<T>(A.n(), b, r, 0);
+  launch_restrict2<T>(A.gn, C.g, r, LV<T>::b(C), C.bc, A.bc, 0);
+  vcycle<T>(st, 0, LV<T>::b(C), LV<T>::x(C));
+  launch_prolong_add<T>(A.gn, C.g, LV<T>::x(C), x, A.bc, 0);
+  q2_chebyshev<T>(st, b, x, g_smooth_degree, false);
+}
+
+template <typename T>
+static void top_vcycle(uwg_stokes* st, const T* b, T* x) {
+  if (st->q2)
+    q2_vcycle<T>(st, b, x);
+  else
+    vcycle<T>(st, 0, b, x);
+}
+template <typename T>
+static T* top_buf_b(uwg_stokes* st) {
+  return st->q2 ? Q2V<T>::b(*st->q2) : LV<T>::b(st->L[0]);
+}
+template <typename T>
+static T* top_buf_x(uwg_stokes* st) {
+  return st->q2 ? Q2V<T>::x(*st->q2) : LV<T>::x(st->L[0]);
+}
+
+// ---------------------------------------------------------------------------------------
 // FGMRES (right preconditioned, classical Gram-Schmidt), FP64 vectors on the device.
 // Converged when |r| <= max(rtol |r0|, atol) with r0 the initial residual
 // (KSPConvergedDefault with zero initial guess).
@@ -1245,26 +1433,62 @@ static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const 
 // ---------------------------------------------------------------------------------------
 extern "C" int uwg_stokes_create(uwg_field* vel, uwg_field* p, const unsigned char* bcmask, uwg_stokes** out) {
   if (vel->on_elements || vel->ncomp != vel->mesh->g.dim) return arg_error("velocityField must be a nodal vector field");
-  if (!p->on_elements || p->ncomp != 1 || p->mesh != vel->mesh)
-    return arg_error("pressureField must be a scalar field on the dQ0 submesh");
+  const bool q2 = vel->mesh->order == 2;
+  if (p->ncomp != 1 || p->mesh != vel->mesh || p->on_elements != (q2 ? 2 : 1))
+    return arg_error(q2 ? "pressureField must be a scalar field on the dPc1 submesh of the Q2 mesh"
+                        : "pressureField must be a scalar field on the dQ0 submesh");
   API_BEGIN
   uwg_stokes* st = new uwg_stokes();
   st->vel = vel;
   st->p = p;
   st->g = vel->mesh->g;
-  st->bc0 = dalloc<uint8_t>(st->g.N);
-  if (bcmask)
-    UWG_HIP(hipMemcpy(st->bc0, bcmask, st->g.N, hipMemcpyHostToDevice));
-  else
-    launch_bc_mask_from_walls(st->g, st->bc0, st->g.dim == 3 ? 0x3f : 0x0f, 0);
+  const Geom& g = st->g;
+  const int walls = g.dim == 3 ? 0x3f : 0x0f;
+  st->bc0 = dalloc<uint8_t>(g.N);
+  if (q2) {
+    st->q2 = new Q2Level();
+    Q2Level& A = *st->q2;
+    A.gn = vel->mesh->gn;
+    A.bc = dalloc<uint8_t>(A.gn.N);
+    if (bcmask)
+      UWG_HIP(hipMemcpy(A.bc, bcmask, A.gn.N, hipMemcpyHostToDevice));
+    else
+      launch_bc_mask_from_walls(A.gn, A.bc, walls, 0);
+    // the embedded Q1 space: a Q1 vertex is the Q2 vertex (2i, 2j, 2k)
+    launch_inject_bc(A.gn, g, A.bc, st->bc0, 0);
+    const int NQ = g.dim == 3 ? 125 : 25, NEN = g.dim == 3 ? 27 : 9, NP = g.dim + 1;
+    const long n = A.n();
+    A.W = dalloc<double>(g.E * NQ);
+    A.dinv = dalloc<double>(n);
+    for (double** v : {&A.x, &A.b, &A.r, &A.d}) {
+      *v = dalloc<double>(n);
+      vk_fill<double>(n, 0.0, *v, 0);
+    }
+    std::vector<double> Gr;
+    q2_element_G(g, Gr);
+    A.Gr = dalloc<double>((long)Gr.size());
+    UWG_HIP(hipMemcpy(A.Gr, Gr.data(), sizeof(double) * Gr.size(), hipMemcpyHostToDevice));
+    A.S = dalloc<double>(g.E * NP * NP);
+    A.Sinv = dalloc<double>(g.E * NP * NP);
+    A.fe = dalloc<double>(g.E * NEN * g.dim);
+    st->nu = n;
+    st->np = g.E * NP;
+  } else {
+    if (bcmask)
+      UWG_HIP(hipMemcpy(st->bc0, bcmask, g.N, hipMemcpyHostToDevice));
+    else
+      launch_bc_mask_from_walls(g, st->bc0, walls, 0);
+    st->nu = g.N * g.dim;
+    st->np = g.E;
+  }
   build_levels(st, 0);
-  compute_Ge(st->g, st->Ge);
-  const int NN = 1 << st->g.dim;
-  st->fe = dalloc<double>(st->g.E * NN * st->g.dim);
-  st->f = dalloc<double>(st->g.N * st->g.dim);
-  st->ubc = dalloc<double>(st->g.N * st->g.dim);
-  st->approxS = dalloc<double>(st->g.E);
-  st->h = dalloc<double>(st->g.E);
+  compute_Ge(g, st->Ge);
+  const int NN = 1 << g.dim;
+  st->fe = dalloc<double>(g.E * NN * g.dim);
+  st->f = dalloc<double>(st->nu);
+  st->ubc = dalloc<double>(st->nu);
+  st->approxS = dalloc<double>(g.E);
+  st->h = dalloc<double>(st->np);
   st->red_partial = dalloc<double>(8 * vk_reduce_blocks());
   st->red_out = dalloc<double>(8);
   st->dcoef = dalloc<double>(256);
@@ -1281,6 +1505,14 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
                     (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
       if (p != (void*)st->bc0) dfree(p);
   dfree(st->bc0);
+  if (st->q2) {
+    Q2Level& A = *st->q2;
+    for (void* p : {(void*)A.bc, (void*)A.W, (void*)A.W32, (void*)A.dinv, (void*)A.dinv32, (void*)A.x, (void*)A.b,
+                    (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32, (void*)A.Gr,
+                    (void*)A.S, (void*)A.Sinv, (void*)A.Sinv32, (void*)A.fe})
+      dfree(p);
+    delete st->q2;
+  }
   delete st->kin;
   delete st->kout;
   delete st->kb;
@@ -1355,8 +1587,8 @@ extern "C" int uwg_stokes_eval_particle_functions(uwg_stokes* st, uwg_swarm* s, 
   if (nconst < 0) return arg_error("bad constant count");
   for (int f = 0; f < nfields; f++)
     if (!fields[f] || fields[f]->on_elements || fields[f]->ncomp < 1 || fields[f]->ncomp > 3 ||
-        fields[f]->mesh != s->mesh)
-      return arg_error("program fields must be nodal fields of 1-3 components on the swarm's mesh");
+        fields[f]->mesh != s->mesh || fields[f]->mesh->order != 1)
+      return arg_error("program fields must be nodal fields of 1-3 components on the swarm's (Q1) mesh");
   int nc_reg[48];
   for (int r = 0; r < 48; r++) nc_reg[r] = 0;
   for (int i = 0; i < nins; i++) {
@@ -1453,6 +1685,45 @@ static void finish_assembly(uwg_stokes* st) {
   st->assembled = true;
 }
 
+// power iteration for lambda_max(D^-1 K) on the Q2 level
+static void q2_estimate_lmax(uwg_stokes* st) {
+  Q2Level& A = *st->q2;
+  const long n = A.n();
+  launch_fill_hash(n, A.x, 0);
+  double lam = 0.0;
+  for (int it = 0; it < 12; it++) {
+    top_K<double>(st, A.x, A.r, A.bc);
+    vk_xmy_scale<double>(n, A.dinv, A.r, A.d, 0);
+    const double nd = dev_dot<double>(st, n, A.d, A.d);
+    const double nx2 = dev_dot<double>(st, n, A.x, A.x);
+    lam = sqrt(nd / nx2);
+    vk_scale<double>(n, 1.0 / sqrt(nd), A.d, A.x, 0);
+  }
+  A.lmax = lam;
+  vk_fill<double>(n, 0.0, A.d, 0);
+}
+
+// Q2/dPc1: body force, approxS blocks, the Q1 weights of the embedded space (the multigrid
+// hierarchy below the Q2 level), Jacobi diagonal and spectral bound of the Q2 level
+static void finish_assembly_q2(uwg_stokes* st) {
+  Q2Level& A = *st->q2;
+  const Geom& g = st->g;
+  launch_q2_gather_f(g, A.fe, A.bc, st->f, 0);
+  vk_fill<double>(st->np, 0.0, st->h, 0);
+  int* bad = wsbuf<int>(st, WS_Q2_BAD, 1);
+  UWG_HIP(hipMemsetAsync(bad, 0, sizeof(int), 0));
+  launch_q2_approxS(g, A.W, A.Gr, A.S, A.Sinv, bad, 0);
+  int hb = 0;
+  UWG_HIP(hipMemcpy(&hb, bad, sizeof(int), hipMemcpyDeviceToHost));
+  if (hb) throw std::runtime_error("singular approxS element block (non-positive viscosity or empty element?)");
+  launch_q2_project_W(g, A.W, st->L[0].W, 0);
+  setup_hierarchy(st);
+  launch_q2_diag(g, A.W, A.bc, A.dinv, 0);
+  q2_estimate_lmax(st);
+  st->fp32_ready = false;
+  st->assembled = true;
+}
+
 extern "C" int uwg_stokes_assemble(uwg_stokes* st, uwg_swarm* s) {
   if (s->mesh->g.E != st->g.E) return arg_error("swarm and Stokes system live on different meshes");
   if (st->mode == 0 && st->nmat == 0) return arg_error("no material tables / particle values set");
@@ -1465,9 +1736,16 @@ extern "C" int uwg_stokes_assemble(uwg_stokes* st, uwg_swarm* s) {
   }
   st->swarm = s;
   Level& A = st->L[0];
-  launch_moments(st->g, s->cell_ptr, s->cell_pid, s->xi_slot, s->w_slot, st->mode == 0 ? s->mat : nullptr, st->eta_tab,
-                 st->force_tab, st->eta_p, st->force_p, A.W, st->fe, st->approxS, st->Ge, st->nmat, 0);
-  finish_assembly(st);
+  if (st->q2) {
+    launch_q2_weights(st->g, s->cell_ptr, s->cell_pid, s->xi_slot, s->w_slot, st->mode == 0 ? s->mat : nullptr,
+                      st->nmat, st->eta_tab, st->force_tab, st->eta_p, st->force_p, nullptr, nullptr, st->q2->W,
+                      st->q2->fe, 0);
+    finish_assembly_q2(st);
+  } else {
+    launch_moments(st->g, s->cell_ptr, s->cell_pid, s->xi_slot, s->w_slot, st->mode == 0 ? s->mat : nullptr,
+                   st->eta_tab, st->force_tab, st->eta_p, st->force_p, A.W, st->fe, st->approxS, st->Ge, st->nmat, 0);
+    finish_assembly(st);
+  }
   st->assembled_state = s->state;
   return 0;
   API_END
@@ -1475,6 +1753,24 @@ extern "C" int uwg_stokes_assemble(uwg_stokes* st, uwg_swarm* s) {
 
 extern "C" int uwg_stokes_assemble_fields(uwg_stokes* st, const double* eta_g, const double* force_g) {
   API_BEGIN
+  if (st->q2) {
+    // Q2 mesh: the 5^dim Gauss points of every element (GaussIntegrationSwarm default for Q2,
+    // PY/swarm/_integration_swarm.py:173-181), point index qx + 5 qy + 25 qz
+    const Geom& g = st->g;
+    const long nq = g.E * (g.dim == 3 ? 125 : 25);
+    double* de = wsbuf<double>(st, WS_Q2_ETAG, nq);
+    double* df = wsbuf<double>(st, WS_Q2_FORCEG, nq * g.dim);
+    UWG_HIP(hipMemcpy(de, eta_g, sizeof(double) * nq, hipMemcpyHostToDevice));
+    if (force_g)
+      UWG_HIP(hipMemcpy(df, force_g, sizeof(double) * nq * g.dim, hipMemcpyHostToDevice));
+    else
+      vk_fill<double>(nq * g.dim, 0.0, df, 0);
+    launch_q2_weights(g, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, de, df,
+                      st->q2->W, st->q2->fe, 0);
+    finish_assembly_q2(st);
+    st->assembled_state = -2;
+    return 0;
+  }
   // Gauss-point viscosities -> W_g = eta_g * w_g * detJ ; f_e = sum_g w_g detJ f(g) N_a(g)
   // (the GaussIntegrationSwarm path of the reference when no voronoi swarm is given)
   const Geom& g = st->g;
@@ -1565,55 +1861,55 @@ extern "C" void uwg_solve_opts_default(uwg_solve_opts* o) {
 
 // lift Dirichlet values: rhs_u = f - K ubc (free dofs), rhs_p = h - G^T ubc
 static void lifted_rhs(uwg_stokes* st, double* fu, double* hp, bool* has_bc) {
-  Level& A = st->L[0];
-  const long n = A.n();
-  const Geom& g = st->g;
+  const long n = st->nu, np = st->np;
+  const Geom& gv = top_vgeom(st);
+  const uint8_t* bc = top_bc(st);
   // ubc = vel on constrained dofs, 0 elsewhere
   int* flag = wsbuf<int>(st, WS_UBC_FLAG, 1);
   UWG_HIP(hipMemsetAsync(flag, 0, sizeof(int), 0));
-  launch_bc_split(g, A.bc, st->vel->d, st->ubc, flag, 0);
+  launch_bc_split(gv, bc, st->vel->d, st->ubc, flag, 0);
   int any = 0;
   UWG_HIP(hipMemcpy(&any, flag, sizeof(int), hipMemcpyDeviceToHost));
   *has_bc = any != 0;
   vk_copy<double>(n, st->f, fu, 0);
-  vk_copy<double>(g.E, st->h, hp, 0);
+  vk_copy<double>(np, st->h, hp, 0);
   if (!any) return;
   // K applied to ubc with BC dofs treated as free (an all-free mask), BC rows then zeroed
-  uint8_t* free_mask = wsbuf<uint8_t>(st, WS_FREEMASK, g.N);
-  UWG_HIP(hipMemsetAsync(free_mask, 0, g.N, 0));
+  uint8_t* free_mask = wsbuf<uint8_t>(st, WS_FREEMASK, gv.N);
+  UWG_HIP(hipMemsetAsync(free_mask, 0, gv.N, 0));
   double* t = wsbuf<double>(st, WS_LIFT_T, n);
-  launch_apply_K<double>(g, A.W, st->ubc, t, free_mask, 0);
-  launch_zero_bc(g, A.bc, t, 0);
+  top_K<double>(st, st->ubc, t, free_mask);
+  launch_zero_bc(gv, bc, t, 0);
   vk_axpy<double>(n, -1.0, t, fu, 0);
-  double* q = wsbuf<double>(st, WS_LIFT_Q, g.E);
-  launch_apply_GT(g, st->Ge, st->ubc, free_mask, q, 0);
-  vk_axpy<double>(g.E, -1.0, q, hp, 0);
+  double* q = wsbuf<double>(st, WS_LIFT_Q, np);
+  top_GT<double>(st, st->ubc, q, free_mask);
+  vk_axpy<double>(np, -1.0, q, hp, 0);
 }
 
 static void true_residuals(uwg_stokes* st, const double* u, const double* p, double* out3) {
   // u includes Dirichlet values; residual on free dofs of f - K u - G p and h - G^T u
-  Level& A = st->L[0];
-  const long n = A.n();
-  const Geom& g = st->g;
+  const long n = st->nu, np = st->np;
+  const Geom& gv = top_vgeom(st);
+  const uint8_t* bc = top_bc(st);
   bool has_bc;
   double* fu = wsbuf<double>(st, WS_RES_FU, n);
-  double* hp = wsbuf<double>(st, WS_RES_HP, g.E);
+  double* hp = wsbuf<double>(st, WS_RES_HP, np);
   lifted_rhs(st, fu, hp, &has_bc);
   // uf = u with BC dofs zeroed (K with identity BC rows maps those to themselves)
   double* uf = wsbuf<double>(st, WS_RES_UF, n);
   vk_copy<double>(n, u, uf, 0);
-  launch_zero_bc(g, A.bc, uf, 0);
+  launch_zero_bc(gv, bc, uf, 0);
   double* t = wsbuf<double>(st, WS_RES_T, n);
   double* gp = wsbuf<double>(st, WS_RES_GP, n);
-  launch_apply_K<double>(g, A.W, uf, t, A.bc, 0);
-  launch_apply_G(g, st->Ge, p, A.bc, gp, 0);
+  top_K<double>(st, uf, t, bc);
+  top_G<double>(st, p, gp, bc);
   vk_axpy<double>(n, 1.0, gp, t, 0);
   vk_axpby<double>(n, 1.0, fu, -1.0, t, 0);  // t = fu - K uf - G p  (BC rows: fu=0, K uf = 0)
-  double* q = wsbuf<double>(st, WS_RES_Q, g.E);
-  launch_apply_GT(g, st->Ge, uf, A.bc, q, 0);
-  vk_axpby<double>(g.E, 1.0, hp, -1.0, q, 0);
+  double* q = wsbuf<double>(st, WS_RES_Q, np);
+  top_GT<double>(st, uf, q, bc);
+  vk_axpby<double>(np, 1.0, hp, -1.0, q, 0);
   out3[0] = sqrt(dev_dot(st, n, t, t));
-  out3[1] = sqrt(dev_dot(st, g.E, q, q));
+  out3[1] = sqrt(dev_dot(st, np, q, q));
   out3[2] = sqrt(dev_dot(st, n, fu, fu));
 }
 
@@ -1630,8 +1926,8 @@ extern "C" int uwg_stokes_residuals(uwg_stokes* st, double* out3) {
 //   u = K^-1 (f - G p)
 static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
                        const double* hp, double* u, double* p) {
-  Level& A = st->L[0];
-  const long n = A.n(), E = st->g.E;
+  const long n = st->nu, E = st->np;
+  const uint8_t* bc = top_bc(st);
   if (!st->kin) st->kin = new KrylovSpace();
   if (!st->kout) st->kout = new KrylovSpace();
   KrylovSpace& Kin = *st->kin;
@@ -1639,16 +1935,19 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   Kin.init(n, o.restart);
   Kout.init(E, o.restart);
   int status = 0;
-  OpFn Kop = [&](const double* x, double* y) { launch_apply_K<double>(A.g, A.W, x, y, A.bc, 0); };
+  OpFn Kop = [&](const double* x, double* y) { top_K<double>(st, x, y, bc); };
   OpFn Mv = [&](const double* x, double* y) {
     st->vcycles++;
     if (o.precision != 0) {
-      vk_d2f(n, 1.0, x, A.b32, 0);
-      vcycle<float>(st, 0, A.b32, A.x32);
-      vk_f2d(n, 1.0, A.x32, y, 0);
+      float* b32 = top_buf_b<float>(st);
+      float* x32 = top_buf_x<float>(st);
+      vk_d2f(n, 1.0, x, b32, 0);
+      top_vcycle<float>(st, b32, x32);
+      vk_f2d(n, 1.0, x32, y, 0);
     } else {
-      vk_copy<double>(n, x, A.b, 0);
-      vcycle<double>(st, 0, A.b, y);
+      double* b64 = top_buf_b<double>(st);
+      vk_copy<double>(n, x, b64, 0);
+      top_vcycle<double>(st, b64, y);
     }
   };
   auto inner = [&](const double* rhs, double* x, int* its) {
@@ -1665,20 +1964,20 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   int its = 0;
   inner(fu, ustar, &its);
   S->velocity_presolve_its = its;
-  launch_apply_GT(st->g, st->Ge, ustar, A.bc, hhat, 0);
+  top_GT<double>(st, ustar, hhat, bc);
   vk_axpby<double>(E, 1.0, hp, -1.0, hhat, 0);  // h^ = h - G^T u*
   int pss_its = 0;
   OpFn Sop = [&](const double* v, double* y) {
-    launch_apply_G(st->g, st->Ge, v, A.bc, t, 0);
+    top_G<double>(st, v, t, bc);
     int ii = 0;
     inner(t, z, &ii);
     pss_its += ii;
-    launch_apply_GT(st->g, st->Ge, z, A.bc, y, 0);
+    top_GT<double>(st, z, y, bc);
     vk_scale<double>(E, -1.0, y, y, 0);  // S v = -G^T K^-1 G v
   };
   // preconditioner: -approxS^-1 (approxS approximates G^T diag(K)^-1 G > 0)
-  launch_neg_recip<double>(E, st->approxS, diagS, 0);
-  OpFn Sprec = [&](const double* x, double* y) { vk_xmy_scale<double>(E, diagS, x, y, 0); };
+  if (!st->q2) launch_neg_recip<double>(E, st->approxS, diagS, 0);
+  OpFn Sprec = [&](const double* x, double* y) { top_schur<double>(st, diagS, x, y); };
   vk_fill<double>(E, 0.0, p, 0);
   double rn;
   int oits = 0;
@@ -1687,7 +1986,7 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   S->pressure_its = oits;
   S->velocity_pressuresolve_its = pss_its;
   // back solve
-  launch_apply_G(st->g, st->Ge, p, A.bc, t, 0);
+  top_G<double>(st, p, t, bc);
   vk_axpby<double>(n, 1.0, fu, -1.0, t, 0);
   inner(t, u, &its);
   S->velocity_backsolve_its = its;
@@ -1699,8 +1998,8 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
 // preconditioner  z_u = V-cycle(r_u), z_p = -approxS^-1 (r_p - G^T z_u).
 static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
                        const double* hp, double* u, double* p) {
-  Level& A = st->L[0];
-  const long n = A.n(), E = st->g.E, nt = n + E;
+  const long n = st->nu, E = st->np, nt = n + E;
+  const uint8_t* bc = top_bc(st);
   if (!st->kb) st->kb = new KrylovSpace();
   KrylovSpace& Kb = *st->kb;
   Kb.init(nt, o.restart);
@@ -1711,27 +2010,30 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   vk_copy<double>(n, fu, b, 0);
   vk_copy<double>(E, hp, b + n, 0);
   vk_fill<double>(nt, 0.0, x, 0);
-  launch_neg_recip<double>(E, st->approxS, diagS, 0);
+  if (!st->q2) launch_neg_recip<double>(E, st->approxS, diagS, 0);
   double* gp = wsbuf<double>(st, WS_BLK_GP, n);
   OpFn Aop = [&](const double* xx, double* yy) {
-    launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
-    launch_apply_G(st->g, st->Ge, xx + n, A.bc, gp, 0);
+    top_K<double>(st, xx, yy, bc);
+    top_G<double>(st, xx + n, gp, bc);
     vk_axpy<double>(n, 1.0, gp, yy, 0);
-    launch_apply_GT(st->g, st->Ge, xx, A.bc, yy + n, 0);
+    top_GT<double>(st, xx, yy + n, bc);
   };
   OpFn Mop = [&](const double* rr, double* zz) {
     st->vcycles++;
     if (o.precision != 0) {
-      vk_d2f(n, 1.0, rr, A.b32, 0);
-      vcycle<float>(st, 0, A.b32, A.x32);
-      vk_f2d(n, 1.0, A.x32, zz, 0);
+      float* b32 = top_buf_b<float>(st);
+      float* x32 = top_buf_x<float>(st);
+      vk_d2f(n, 1.0, rr, b32, 0);
+      top_vcycle<float>(st, b32, x32);
+      vk_f2d(n, 1.0, x32, zz, 0);
     } else {
-      vk_copy<double>(n, rr, A.b, 0);
-      vcycle<double>(st, 0, A.b, zz);
+      double* b64 = top_buf_b<double>(st);
+      vk_copy<double>(n, rr, b64, 0);
+      top_vcycle<double>(st, b64, zz);
     }
-    launch_apply_GT(st->g, st->Ge, zz, A.bc, q, 0);
+    top_GT<double>(st, zz, q, bc);
     vk_axpby<double>(E, 1.0, rr + n, -1.0, q, 0);  // q = r_p - G^T z_u
-    vk_xmy_scale<double>(E, diagS, q, zz + n, 0);
+    top_schur<double>(st, diagS, q, zz + n);
   };
   int its = 0;
   double rn;
@@ -1765,8 +2067,8 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
 // The stopping test is always the FP64 true residual.
 static int solve_block_mixed(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
                              const double* hp, double* u, double* p) {
-  Level& A = st->L[0];
-  const long n = A.n(), E = st->g.E, nt = n + E;
+  const long n = st->nu, E = st->np, nt = n + E;
+  const uint8_t* bc = top_bc(st);
   if (!st->kf) st->kf = new KSpaceT<float>();
   KSpaceT<float>& Kf = *st->kf;
   Kf.init(nt, o.restart);
@@ -1782,25 +2084,25 @@ static int solve_block_mixed(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_
   vk_copy<double>(n, fu, b, 0);
   vk_copy<double>(E, hp, b + n, 0);
   vk_fill<double>(nt, 0.0, x, 0);
-  launch_neg_recip<float>(E, st->approxS, diagS32, 0);
+  if (!st->q2) launch_neg_recip<float>(E, st->approxS, diagS32, 0);
   auto A64 = [&](const double* xx, double* yy) {
-    launch_apply_K<double>(A.g, A.W, xx, yy, A.bc, 0);
-    launch_apply_G<double>(st->g, st->Ge, xx + n, A.bc, gp, 0);
+    top_K<double>(st, xx, yy, bc);
+    top_G<double>(st, xx + n, gp, bc);
     vk_axpy<double>(n, 1.0, gp, yy, 0);
-    launch_apply_GT<double>(st->g, st->Ge, xx, A.bc, yy + n, 0);
+    top_GT<double>(st, xx, yy + n, bc);
   };
   OpFn32 A32 = [&](const float* xx, float* yy) {
-    launch_apply_K<float>(A.g, A.W32, xx, yy, A.bc, 0);
-    launch_apply_G<float>(st->g, st->Ge, xx + n, A.bc, gp32, 0);
+    top_K<float>(st, xx, yy, bc);
+    top_G<float>(st, xx + n, gp32, bc);
     vk_axpy<float>(n, 1.0f, gp32, yy, 0);
-    launch_apply_GT<float>(st->g, st->Ge, xx, A.bc, yy + n, 0);
+    top_GT<float>(st, xx, yy + n, bc);
   };
   OpFn32 M32 = [&](const float* rr, float* zz) {
     st->vcycles++;
-    vcycle<float>(st, 0, rr, zz);
-    launch_apply_GT<float>(st->g, st->Ge, zz, A.bc, q32, 0);
+    top_vcycle<float>(st, rr, zz);
+    top_GT<float>(st, zz, q32, bc);
     vk_axpby<float>(E, 1.0f, rr + n, -1.0f, q32, 0);  // q = r_p - G^T z_u
-    vk_xmy_scale<float>(E, diagS32, q32, zz + n, 0);
+    top_schur<float>(st, diagS32, q32, zz + n);
   };
   const double bn = sqrt(dev_dot<double>(st, nt, b, b));
   int total = 0, sweeps = 0, status = 0;
@@ -1868,6 +2170,7 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
     if (st->assembled) setup_hierarchy(st);
   }
   if (o.learned && !st->nn) return arg_error("no learned model loaded");
+  if (o.learned && st->q2) return arg_error("the learned correction is provided for Q1/dQ0 systems");
   if (st->swarm && st->assembled_state != st->swarm->state && st->assembled_state != -2) {
     int rc = uwg_stokes_assemble(st, st->swarm);
     if (rc) return rc;
@@ -1879,8 +2182,7 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   UWG_HIP(hipDeviceSynchronize());
   double t1 = now();
   st->vcycles = 0;
-  Level& A = st->L[0];
-  const long n = A.n(), E = st->g.E;
+  const long n = st->nu, E = st->np;
   bool has_bc;
   double* fu = wsbuf<double>(st, WS_FU, n);
   double* hp = wsbuf<double>(st, WS_HP, E);
@@ -1928,6 +2230,35 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
 // ---------------------------------------------------------------------------------------
 extern "C" int uwg_stokes_apply(uwg_stokes* st, int which, int level, const double* x, double* y) {
   if (!st->assembled) return arg_error("system not assembled");
+  if (st->q2) {
+    // level 0 is the Q2/dPc1 level, level l >= 1 the Q1 level l-1 below it
+    if (level < 0 || level > (int)st->L.size()) return arg_error("bad level");
+    if (level == 0) {
+      if (which < 0 || which > 3) return arg_error("which must be 0..3");
+      API_BEGIN
+      const long nin = which == 1 ? st->np : st->nu, nout = which == 2 ? st->np : st->nu;
+      double* dx = dalloc<double>(nin);
+      double* dy = dalloc<double>(nout);
+      UWG_HIP(hipMemcpy(dx, x, sizeof(double) * nin, hipMemcpyHostToDevice));
+      if (which == 0)
+        top_K<double>(st, dx, dy, st->q2->bc);
+      else if (which == 1)
+        top_G<double>(st, dx, dy, st->q2->bc);
+      else if (which == 2)
+        top_GT<double>(st, dx, dy, st->q2->bc);
+      else {
+        vk_copy<double>(nin, dx, st->q2->b, 0);
+        q2_vcycle<double>(st, st->q2->b, dy);
+      }
+      UWG_HIP(hipMemcpy(y, dy, sizeof(double) * nout, hipMemcpyDeviceToHost));
+      dfree(dx);
+      dfree(dy);
+      return 0;
+      API_END
+    }
+    if (which != 0) return arg_error("only K (which 0) is available on the Q1 levels of a Q2 system");
+    level -= 1;
+  }
   if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
   API_BEGIN
   Level& A = st->L[level];
@@ -1957,15 +2288,36 @@ extern "C" int uwg_stokes_apply(uwg_stokes* st, int which, int level, const doub
 }
 
 extern "C" int uwg_stokes_levels(uwg_stokes* st, int* nlev, int* res_out) {
-  *nlev = (int)st->L.size();
-  if (res_out)
+  const int off = st->q2 ? 1 : 0;
+  *nlev = (int)st->L.size() + off;
+  if (res_out) {
+    if (off)
+      for (int d = 0; d < 3; d++) res_out[d] = st->g.res[d];
     for (size_t l = 0; l < st->L.size(); l++)
-      for (int d = 0; d < 3; d++) res_out[l * 3 + d] = st->L[l].g.res[d];
+      for (int d = 0; d < 3; d++) res_out[(l + off) * 3 + d] = st->L[l].g.res[d];
+  }
   return 0;
 }
 
 extern "C" int uwg_stokes_get_assembly(uwg_stokes* st, int level, double* W, double* fe, double* approxS) {
   if (!st->assembled) return arg_error("system not assembled");
+  if (st->q2) {
+    if (level < 0 || level > (int)st->L.size()) return arg_error("bad level");
+    if (level == 0) {
+      // W [E][5^dim], fe [E][3^dim][dim], approxS blocks [E][dim+1][dim+1]
+      API_BEGIN
+      const Geom& g = st->g;
+      const long nq = g.E * (g.dim == 3 ? 125 : 25), nf = g.E * (g.dim == 3 ? 27 : 9) * g.dim;
+      const long ns = g.E * (g.dim + 1) * (g.dim + 1);
+      if (W) UWG_HIP(hipMemcpy(W, st->q2->W, sizeof(double) * nq, hipMemcpyDeviceToHost));
+      if (fe) UWG_HIP(hipMemcpy(fe, st->q2->fe, sizeof(double) * nf, hipMemcpyDeviceToHost));
+      if (approxS) UWG_HIP(hipMemcpy(approxS, st->q2->S, sizeof(double) * ns, hipMemcpyDeviceToHost));
+      return 0;
+      API_END
+    }
+    if (fe || approxS) return arg_error("fe and approxS exist on level 0 only");
+    level -= 1;
+  }
   if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
   API_BEGIN
   Level& A = st->L[level];
@@ -1983,7 +2335,7 @@ extern "C" int uwg_stokes_get_assembly(uwg_stokes* st, int level, double* W, dou
 
 extern "C" int uwg_stokes_get_rhs(uwg_stokes* st, double* f) {
   API_BEGIN
-  UWG_HIP(hipMemcpy(f, st->f, sizeof(double) * st->L[0].n(), hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(f, st->f, sizeof(double) * st->nu, hipMemcpyDeviceToHost));
   return 0;
   API_END
 }
@@ -2022,8 +2374,39 @@ extern "C" int uwg_stream_copy(long n, int reps, double* ms) {
 
 extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int level, int reps, double* ms) {
   if (!st->assembled) return arg_error("system not assembled");
-  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
   if (reps < 1) return arg_error("reps must be positive");
+  if (st->q2) {
+    if (level < 0 || level > (int)st->L.size()) return arg_error("bad level");
+    if (level == 0) {
+      API_BEGIN
+      Q2Level& A = *st->q2;
+      hipEvent_t e0, e1;
+      UWG_HIP(hipEventCreate(&e0));
+      UWG_HIP(hipEventCreate(&e1));
+      if (precision == 1) {
+        ensure_fp32(st);
+        top_K<float>(st, A.x32, A.r32, A.bc);
+        UWG_HIP(hipEventRecord(e0, 0));
+        for (int i = 0; i < reps; i++) top_K<float>(st, A.x32, A.r32, A.bc);
+        UWG_HIP(hipEventRecord(e1, 0));
+      } else {
+        top_K<double>(st, A.x, A.r, A.bc);
+        UWG_HIP(hipEventRecord(e0, 0));
+        for (int i = 0; i < reps; i++) top_K<double>(st, A.x, A.r, A.bc);
+        UWG_HIP(hipEventRecord(e1, 0));
+      }
+      UWG_HIP(hipEventSynchronize(e1));
+      float t;
+      UWG_HIP(hipEventElapsedTime(&t, e0, e1));
+      *ms = t / reps;
+      UWG_HIP(hipEventDestroy(e0));
+      UWG_HIP(hipEventDestroy(e1));
+      return 0;
+      API_END
+    }
+    level -= 1;
+  }
+  if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
   API_BEGIN
   Level& A = st->L[level];
   hipEvent_t e0, e1;
@@ -2052,6 +2435,7 @@ extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int level, i
 }
 
 extern "C" int uwg_stokes_load_nn(uwg_stokes* st, int level, int nparams, const float* params, int channels, int depth) {
+  if (st->q2) return arg_error("the learned correction is provided for Q1/dQ0 systems");
   if (level < 1 || level >= (int)st->L.size()) return arg_error("learned correction level out of range");
   API_BEGIN
   st->nn.reset(nn_create(st->L[level].g, channels, depth, params, nparams));

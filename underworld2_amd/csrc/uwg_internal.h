@@ -132,6 +132,29 @@ void launch_bc_split(const Geom& g, const uint8_t* bc, const double* v, double* 
 void launch_zero_bc(const Geom& g, const uint8_t* bc, double* v, hipStream_t s);
 template <typename T> void launch_neg_recip(long n, const double* sv, T* out, hipStream_t s);
 
+// q2_kernels.hip: the Q2/dPc1 element pair.  g is the element grid; vertex vectors live on
+// the (2 res + 1)^dim grid, pressure vectors are [E][dim+1], weights W are [E][5^dim].
+void launch_q2_weights(const Geom& g, const int* cell_ptr, const int* cell_pid, const double* xi_slot,
+                       const double* w_slot, const int* mat, int nmat, const double* eta_tab, const double* force_tab,
+                       const double* eta_p, const double* force_p, const double* eta_g, const double* force_g,
+                       double* W, double* fe, hipStream_t s);
+void launch_q2_gather_f(const Geom& g, const double* fe, const uint8_t* bc, double* f, hipStream_t s);
+template <typename T>
+void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
+void launch_q2_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
+template <typename T>
+void launch_q2_apply_G(const Geom& g, const double* Gr, const T* p, const uint8_t* bc, T* y, hipStream_t s);
+template <typename T>
+void launch_q2_apply_GT(const Geom& g, const double* Gr, const T* u, const uint8_t* bc, T* q, hipStream_t s);
+void launch_q2_approxS(const Geom& g, const double* W, const double* Gr, double* S, double* Sinv, int* bad,
+                       hipStream_t s);
+template <typename T>
+void launch_q2_schur_prec(long E, int np, const T* Sinv, const T* r, T* z, hipStream_t s);
+void launch_q2_project_W(const Geom& g, const double* W, double* W1, hipStream_t s);
+void q2_element_G(const Geom& g, std::vector<double>& Gr);
+void launch_q2_interpolate(const Geom& g, long P, const double* f, int nc, const double* x, double* v, int* ok,
+                           hipStream_t s);
+
 // vector kernels (T = double or float)
 template <typename T> void vk_fill(long n, T v, T* x, hipStream_t s);
 template <typename T> void vk_copy(long n, const T* x, T* y, hipStream_t s);

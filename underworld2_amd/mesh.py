@@ -5,6 +5,10 @@ Q1 velocity elements with a piecewise-constant (dQ0) pressure sub-mesh.  Node nu
 lexicographic with x fastest (CartesianGenerator.c:2001-2039); element-local nodes are
 ordered (-,-,-), (+,-,-), (-,+,-), (+,+,-), then +z.
 
+``FeMesh_Cartesian("Q2/dPc1", ...)`` has (2 res + 1)^dim vertices on the same element grid
+(C2Generator.c:147-260) and a dPc1 sub-mesh of dim+1 pressure vertices per element, stored
+element-major, at the element-local points of Inner2DGenerator.c:286-353.
+
 ``MeshVariable.data`` is a host numpy array of shape [nodes][nodeDofCount] (or
 [elements][count] on the dQ0 sub-mesh).  The authoritative copy lives on the GPU; the host
 view is refreshed on access and pushed back before the next device operation.
@@ -38,35 +42,56 @@ class IndexSet:
         return int(i) in set(self._idx.tolist())
 
 
-class _SubMesh:
-    """The dQ0 pressure mesh: one value per element."""
+_DPC1_LOCAL = {2: ((-0.5, -0.5), (0.5, -0.5), (0.0, 0.5)),
+               3: ((-0.5, -0.5, -0.5), (0.25, 0.25, 0.25), (0.5, -0.25, -0.5), (-0.25, 0.5, 0.5))}
 
-    def __init__(self, parent):
+
+class _SubMesh:
+    """The pressure mesh: one value per element (dQ0) or dim+1 per element (dPc1)."""
+
+    def __init__(self, parent, kind):
         self.parent = parent
-        self.elementType = "DQ0"
+        self.elementType = kind
         self.elementRes = parent.elementRes
         self.dim = parent.dim
+        self._per_element = 1 if kind == "DQ0" else parent.dim + 1
 
     @property
     def nodesGlobal(self):
-        return self.parent.elementsGlobal
+        return self.parent.elementsGlobal * self._per_element
+
+    @property
+    def data(self):
+        """Pressure vertex coordinates: element centres (dQ0) or the dPc1 inner vertices."""
+        p = self.parent
+        res = np.array(p.elementRes)
+        lo, hi = np.array(p.minCoord), np.array(p.maxCoord)
+        h = (hi - lo) / res
+        idx = np.indices(tuple(res[::-1])).reshape(p.dim, -1)[::-1]
+        cen = np.stack([lo[d] + (idx[d] + 0.5) * h[d] for d in range(p.dim)], -1)
+        if self._per_element == 1:
+            return cen
+        loc = np.array(_DPC1_LOCAL[p.dim])
+        return (cen[:, None, :] + loc[None, :, :] * (0.5 * h)[None, None, :]).reshape(-1, p.dim)
 
 
 class FeMesh_Cartesian:
     """Regular Cartesian finite-element mesh (PY/mesh/_mesh.py:1046-1190).
 
-    Only ``elementType="Q1/dQ0"`` is provided by this engine.
+    The element pairs ``"Q1/dQ0"`` and ``"Q2/dPc1"`` are provided by this engine.
     """
 
     def __init__(self, elementType="Q1/dQ0", elementRes=(4, 4), minCoord=(0.0, 0.0), maxCoord=(1.0, 1.0),
                  periodic=None, partitioned=True, **kwargs):
-        if not isinstance(elementType, str) or elementType.upper() not in ("Q1/DQ0", "Q1"):
-            raise ValueError("Only the 'Q1/dQ0' element type is provided by this engine.")
+        et = elementType.upper().replace("-", "/").replace(",", "/") if isinstance(elementType, str) else None
+        if et not in ("Q1/DQ0", "Q1", "Q2/DPC1", "Q2"):
+            raise ValueError("Only the 'Q1/dQ0' and 'Q2/dPc1' element types are provided by this engine.")
         if periodic is not None and any(periodic):
             raise ValueError("Periodic meshes are not supported by this engine.")
         if len(elementRes) not in (2, 3) or len(minCoord) != len(elementRes) or len(maxCoord) != len(elementRes):
             raise ValueError("elementRes, minCoord and maxCoord must have the same length (2 or 3).")
-        self.elementType = "Q1"
+        self._order = 2 if et.startswith("Q2") else 1
+        self.elementType = "Q2" if self._order == 2 else "Q1"
         self.dim = len(elementRes)
         self.elementRes = tuple(int(r) for r in elementRes)
         self.minCoord = tuple(float(c) for c in minCoord)
@@ -75,12 +100,13 @@ class FeMesh_Cartesian:
         res = np.array(self.elementRes, dtype=np.int32)
         lo = np.array(self.minCoord)
         hi = np.array(self.maxCoord)
-        L.check(L.lib().uwg_mesh_create(self.dim, L.iptr(res), L.dptr(lo), L.dptr(hi), ctypes.byref(h)),
-                "FeMesh_Cartesian")
+        create = L.lib().uwg_mesh_create_q2 if self._order == 2 else L.lib().uwg_mesh_create
+        L.check(create(self.dim, L.iptr(res), L.dptr(lo), L.dptr(hi), ctypes.byref(h)), "FeMesh_Cartesian")
         self._h = h
-        self.nodesGlobal = int(np.prod([r + 1 for r in self.elementRes]))
+        self._nn = tuple(self._order * r + 1 for r in self.elementRes)
+        self.nodesGlobal = int(np.prod(self._nn))
         self.elementsGlobal = int(np.prod(self.elementRes))
-        self.subMesh = _SubMesh(self)
+        self.subMesh = _SubMesh(self, "DPC1" if self._order == 2 else "DQ0")
         self._data = None
         self._special = None
 
@@ -106,7 +132,7 @@ class FeMesh_Cartesian:
     def specialSets(self):
         """Wall node sets: Min/Max I/J/K _VertexSet, plus AllWalls_VertexSet."""
         if self._special is None:
-            n = [r + 1 for r in self.elementRes]
+            n = list(self._nn)
             idx = np.indices(n[::-1]).reshape(self.dim, -1)[::-1]
             names = "IJK"
             sets = {}
@@ -141,10 +167,10 @@ class MeshVariable(_fn.Function):
         self._base = base
         self.nodeDofCount = int(nodeDofCount)
         self._on_elements = on_elements
-        n = base.elementsGlobal if on_elements else base.nodesGlobal
+        n = mesh.nodesGlobal
+        kind = 0 if not on_elements else (1 if mesh.elementType == "DQ0" else 2)
         h = ctypes.c_void_p()
-        L.check(L.lib().uwg_field_create(base._h, self.nodeDofCount, 1 if on_elements else 0, ctypes.byref(h)),
-                "MeshVariable")
+        L.check(L.lib().uwg_field_create(base._h, self.nodeDofCount, kind, ctypes.byref(h)), "MeshVariable")
         self._h = h
         self._host = np.zeros((n, self.nodeDofCount))
         self._dev_newer = False

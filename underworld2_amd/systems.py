@@ -21,17 +21,23 @@ from .mesh import MeshVariable
 from .swarm import Swarm, SwarmVariable
 
 _GA = np.sqrt(0.6)
+# 5-point Gauss-Legendre rule (the Q2 mesh default, PY/swarm/_integration_swarm.py:173-181)
+_G5A = np.sqrt(5.0 - 2.0 * np.sqrt(10.0 / 7.0)) / 3.0
+_G5B = np.sqrt(5.0 + 2.0 * np.sqrt(10.0 / 7.0)) / 3.0
+_G5X = np.array([-_G5B, -_G5A, 0.0, _G5A, _G5B])
+_G5W = np.array([(322.0 - 13.0 * np.sqrt(70.0)) / 900.0, (322.0 + 13.0 * np.sqrt(70.0)) / 900.0, 128.0 / 225.0,
+                 (322.0 + 13.0 * np.sqrt(70.0)) / 900.0, (322.0 - 13.0 * np.sqrt(70.0)) / 900.0])
 
 
 def _gauss_points(mesh):
-    """Physical coordinates of the 3^dim Gauss points of every element, [E][Q][dim],
-    point index q = qx + 3 qy + 9 qz."""
+    """Physical coordinates of the Gauss points of every element, [E][Q][dim]: 3^dim points
+    (Q1 mesh) or 5^dim (Q2 mesh), point index q = qx + n qy + n^2 qz."""
     d = mesh.dim
     res = np.array(mesh.elementRes)
     lo = np.array(mesh.minCoord)
     hi = np.array(mesh.maxCoord)
     h = (hi - lo) / res
-    x1 = np.array([-_GA, 0.0, _GA])
+    x1 = _G5X if mesh._order == 2 else np.array([-_GA, 0.0, _GA])
     if d == 2:
         ej, ei = np.meshgrid(np.arange(res[1]), np.arange(res[0]), indexing="ij")
         cen = np.stack([lo[0] + (ei.ravel() + 0.5) * h[0], lo[1] + (ej.ravel() + 0.5) * h[1]], 1)
@@ -57,7 +63,7 @@ class Stokes:
         if not isinstance(velocityField, MeshVariable) or velocityField._on_elements:
             raise TypeError("Provided 'velocityField' must be a nodal 'MeshVariable'.")
         if not isinstance(pressureField, MeshVariable) or not pressureField._on_elements:
-            raise TypeError("Provided 'pressureField' must be a MeshVariable on the dQ0 sub-mesh.")
+            raise TypeError("Provided 'pressureField' must be a MeshVariable on the pressure sub-mesh.")
         for name, v in (("fn_viscosity2", _fn_viscosity2), ("fn_director", _fn_director),
                         ("fn_stresshistory", fn_stresshistory or _fn_stresshistory),
                         ("fn_one_on_lambda", fn_one_on_lambda or _fn_one_on_lambda), ("fn_source", fn_source),
@@ -228,11 +234,14 @@ class Stokes:
         return out[0], out[1]
 
     def velocity_rms(self):
-        """sqrt(int v.v / int 1) by 3^d Gauss quadrature (PY/systems/_stokes.py:435-448)."""
+        """sqrt(int v.v / int 1) by Gauss quadrature exact for the velocity space
+        (PY/systems/_stokes.py:435-448)."""
         mesh = self._mesh
         v = self._velocityField.data
         d = mesh.dim
         res = mesh.elementRes
+        if mesh._order == 2:
+            return self._velocity_rms_q2(v)
         nn = [r + 1 for r in res]
         x1 = np.array([-_GA, 0.0, _GA])
         w1 = np.array([5.0, 8.0, 5.0]) / 9.0
@@ -266,6 +275,30 @@ class Stokes:
         detJ = np.prod([hh / 2 for hh in h])
         for i in range(d):
             vol *= mesh.maxCoord[i] - mesh.minCoord[i]
+        return float(np.sqrt(acc * detJ / vol))
+
+    def _velocity_rms_q2(self, v):
+        """5^dim Gauss rule on the quadratic velocity (Biquadratic.c:137-160 basis)."""
+        mesh = self._mesh
+        d = mesh.dim
+        res = mesh.elementRes
+        V = v.reshape(tuple(mesh._nn[::-1]) + (d,))
+        phi = np.stack([0.5 * _G5X * (_G5X - 1.0), 1.0 - _G5X * _G5X, 0.5 * _G5X * (_G5X + 1.0)], 0)  # [3][5]
+        # element-local vertex blocks: V[2k+c, 2j+b, 2i+a]
+        acc = 0.0
+        if d == 2:
+            blk = np.stack([np.stack([V[b:b + 2 * res[1]:2, a:a + 2 * res[0]:2] for a in range(3)], 0)
+                            for b in range(3)], 0)  # [b][a][ey][ex][d]
+            val = np.einsum("bq,ap,baijc->qpijc", phi, phi, blk)  # [qy][qx]
+            acc = np.einsum("q,p,qpijc,qpijc->", _G5W, _G5W, val, val)
+        else:
+            blk = np.stack([np.stack([np.stack([V[c:c + 2 * res[2]:2, b:b + 2 * res[1]:2, a:a + 2 * res[0]:2]
+                                                for a in range(3)], 0) for b in range(3)], 0) for c in range(3)], 0)
+            val = np.einsum("cr,bq,ap,cbaijkd->rqpijkd", phi, phi, phi, blk)
+            acc = np.einsum("r,q,p,rqpijkd,rqpijkd->", _G5W, _G5W, _G5W, val, val)
+        h = [(mesh.maxCoord[i] - mesh.minCoord[i]) / res[i] for i in range(d)]
+        detJ = np.prod([hh / 2 for hh in h])
+        vol = np.prod([mesh.maxCoord[i] - mesh.minCoord[i] for i in range(d)])
         return float(np.sqrt(acc * detJ / vol))
 
 
