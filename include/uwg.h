@@ -218,6 +218,29 @@ int uwg_stream_copy(long n, int reps, double *ms);
 /* time `reps` applies of K on a multigrid level in the given precision; ms per apply */
 int uwg_stokes_time_apply(uwg_stokes *st, int precision, int level, int reps, double *ms);
 
+/* ---- SUPG advection-diffusion: AdvectionDiffusion(phiField, velocityField, fn_diffusivity,
+ *      fn_sourceTerm, method="SUPG", conditions, phiDotField)
+ *      (PY/systems/_advectiondiffusion.py:838-1010; AD/ = LU/Underworld/Utils/AdvectionDiffusion/src/).
+ *      Q1 mesh; phi, phidot nodal scalars, vel the nodal velocity on the same mesh; bcmask: one
+ *      byte per vertex, bit 0 set where phi is fixed (those vertices keep their values) */
+typedef struct uwg_advdiff uwg_advdiff;
+int uwg_advdiff_create(uwg_field *phi, uwg_field *phidot, uwg_field *vel, const unsigned char *bcmask,
+                       uwg_advdiff **out);
+int uwg_advdiff_destroy(uwg_advdiff *a);
+/* diffusivity and source at the 3^dim Gauss points of every element, [E][3^dim], point index
+ * qx + 3 qy + 9 qz (GaussIntegrationSwarm of the Q1 mesh); source may be NULL.  Values are
+ * held until set again (fn_diffusivity / fn_sourceTerm evaluated by the caller) */
+int uwg_advdiff_set_coefficients(uwg_advdiff *a, const double *kappa_g, const double *source_g);
+/* AdvDiffMulticorrector gamma (0.5) and corrector passes (2) (Multicorrector.c:96-106), upwind
+ * parameter function 0 exact / 1 doubly asymptotic / 2 critical (Residual.cpp:150-160, 557-606) */
+int uwg_advdiff_set_scheme(uwg_advdiff *a, double gamma, int iterations, int upwind);
+/* one step: AdvectionDiffusion.integrate(dt) -> _AdvDiffMulticorrector_Solve
+ * (Multicorrector.c:158-200) with the residual of Residual.cpp:223-513 and the lumped mass of
+ * LumpedMassMatrixForceTerm.c:88-129; updates phi and phidot in place */
+int uwg_advdiff_integrate(uwg_advdiff *a, double dt);
+/* get_max_dt(): AdvectionDiffusionSLE_CalculateDt (Timestep.c:36-110) */
+int uwg_advdiff_max_dt(uwg_advdiff *a, double *dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,12 @@ SIGNATURES = {
     "uwg_device_info": [ctypes.c_char_p, ctypes.c_int],
     "uwg_mesh_create": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
     "uwg_mesh_create_q2": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
+    "uwg_advdiff_create": [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(_vp)],
+    "uwg_advdiff_destroy": [_vp],
+    "uwg_advdiff_set_coefficients": [_vp, _dp, _dp],
+    "uwg_advdiff_set_scheme": [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int],
+    "uwg_advdiff_integrate": [_vp, ctypes.c_double],
+    "uwg_advdiff_max_dt": [_vp, _dp],
     "uwg_mesh_destroy": [_vp],
     "uwg_mesh_coords": [_vp, _dp],
     "uwg_mesh_min_separation": [_vp, _dp],

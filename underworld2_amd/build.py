@@ -25,6 +25,7 @@ UNITS = {
     "fem_kernels.hip": ["-ffp-contract=fast"],
     "nn_kernels.hip": ["-ffp-contract=fast"],
     "q2_kernels.hip": ["-ffp-contract=fast"],
+    "advdiff_kernels.hip": ["-ffp-contract=off"],
     "uwg_host.hip": ["-ffp-contract=off"],
 }
 

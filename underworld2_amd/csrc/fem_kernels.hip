@@ -208,6 +208,54 @@ void launch_moments(const Geom& g, const int* cell_ptr, const int* cell_pid, con
   UWG_HIP(hipGetLastError());
 }
 
+// Gauss-swarm assembly (no voronoi swarm, PY/swarm/_integration_swarm.py:173-181): the weights
+// are eta(g) w_g detJ at the 3^dim Gauss points, f_e = sum_g w_g detJ f(g) N_a(g), approxS as
+// above.  eta_g [E][Q], force_g [E][Q][DIM] (null: zero force), point index qx + 3 qy + 9 qz.
+template <int DIM>
+__global__ void k_gauss_assemble(MomArgs A, const double* __restrict__ eta_g, const double* __restrict__ force_g,
+                                 double* __restrict__ W, double* __restrict__ fe, double* __restrict__ approxS) {
+  constexpr int NN = DIM == 3 ? 8 : 4, Q = DIM == 3 ? 27 : 9;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.E) return;
+  const double w1[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  double M[27], F[NN * DIM];
+  for (int q = 0; q < NN * DIM; q++) F[q] = 0.0;
+  for (int q = 0; q < Q; q++) {
+    const double wq = w1[q % 3] * w1[(q / 3) % 3] * (DIM == 3 ? w1[q / 9] : 1.0);
+    M[q] = eta_g[e * Q + q] * wq * A.detJ;
+    if (force_g) {
+      const double X[3] = {gpt(q % 3), gpt((q / 3) % 3), gpt(q / 9)};
+      for (int a = 0; a < NN; a++) {
+        double Na = 1.0;
+        for (int d = 0; d < DIM; d++) Na *= 0.5 * (1.0 + (((a >> d) & 1) ? 1.0 : -1.0) * X[d]);
+        for (int d = 0; d < DIM; d++) F[a * DIM + d] += wq * A.detJ * force_g[(e * Q + q) * DIM + d] * Na;
+      }
+    }
+  }
+  for (int q = 0; q < Q; q++) W[(long)q * A.Es + e] = M[q];
+  for (int q = 0; q < NN * DIM; q++) fe[e * NN * DIM + q] = F[q];
+  double sc[3] = {2.0 / A.h[0], 2.0 / A.h[1], DIM == 3 ? 2.0 / A.h[2] : 1.0};
+  double S = 0.0;
+  for (int a = 0; a < NN; a++)
+    for (int i = 0; i < DIM; i++) {
+      const double kd = kdiag_entry<DIM>(M, a, i, sc);
+      const double g = A.Ge[a * DIM + i];
+      S += g * (g / kd);
+    }
+  approxS[e] = S;
+}
+
+void launch_gauss_assemble(const Geom& g, const double* eta_g, const double* force_g, double* W, double* fe,
+                           double* approxS, const double* Ge, hipStream_t s) {
+  MomArgs A = mom_args(g);
+  for (int q = 0; q < 24; q++) A.Ge[q] = q < (1 << g.dim) * g.dim ? Ge[q] : 0.0;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_gauss_assemble<3>, nblocks(g.E, 128), 128, 0, s, A, eta_g, force_g, W, fe, approxS);
+  else
+    hipLaunchKernelGGL(k_gauss_assemble<2>, nblocks(g.E, 128), 128, 0, s, A, eta_g, force_g, W, fe, approxS);
+  UWG_HIP(hipGetLastError());
+}
+
 // f[n][i] = sum over the elements around node n of fe (fixed element order); 0 on BC dofs
 template <int DIM>
 __global__ void k_gather_fe(int rx, int ry, int rz, long N, const double* __restrict__ fe,

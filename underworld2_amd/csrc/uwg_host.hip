@@ -144,6 +144,7 @@ struct uwg_stokes {
   Geom g;
   std::vector<Level> L;
   Q2Level* q2 = nullptr;      // set for a Q2/dPc1 system
+  std::vector<double> eta_g_prev;  // Gauss-point viscosities of the last Gauss-swarm assembly
   long nu = 0, np = 0;        // velocity dofs (all vertices x dim) and pressure dofs
   Coarse coarse;
   bool hierarchy_built = false;
@@ -1752,93 +1753,39 @@ extern "C" int uwg_stokes_assemble(uwg_stokes* st, uwg_swarm* s) {
 }
 
 extern "C" int uwg_stokes_assemble_fields(uwg_stokes* st, const double* eta_g, const double* force_g) {
+  if (!eta_g) return arg_error("viscosity values are required");
   API_BEGIN
+  // Gauss-swarm path of the reference (no voronoi swarm): function values at the Gauss points of
+  // every element (3^dim on Q1 meshes, 5^dim on Q2 meshes, PY/swarm/_integration_swarm.py:173-181),
+  // point index qx + n qy + n^2 qz.  When the viscosities equal those of the previous assembly
+  // the operators and the multigrid hierarchy are kept and only the body force is re-assembled.
+  const Geom& g = st->g;
+  const long nq = g.E * (st->q2 ? (g.dim == 3 ? 125 : 25) : (g.dim == 3 ? 27 : 9));
+  const bool same_eta = st->assembled && st->assembled_state == -2 && (long)st->eta_g_prev.size() == nq &&
+                        memcmp(st->eta_g_prev.data(), eta_g, sizeof(double) * nq) == 0;
+  double* de = wsbuf<double>(st, WS_Q2_ETAG, nq);
+  double* df = wsbuf<double>(st, WS_Q2_FORCEG, nq * g.dim);
+  UWG_HIP(hipMemcpy(de, eta_g, sizeof(double) * nq, hipMemcpyHostToDevice));
+  if (force_g)
+    UWG_HIP(hipMemcpy(df, force_g, sizeof(double) * nq * g.dim, hipMemcpyHostToDevice));
+  else
+    vk_fill<double>(nq * g.dim, 0.0, df, 0);
   if (st->q2) {
-    // Q2 mesh: the 5^dim Gauss points of every element (GaussIntegrationSwarm default for Q2,
-    // PY/swarm/_integration_swarm.py:173-181), point index qx + 5 qy + 25 qz
-    const Geom& g = st->g;
-    const long nq = g.E * (g.dim == 3 ? 125 : 25);
-    double* de = wsbuf<double>(st, WS_Q2_ETAG, nq);
-    double* df = wsbuf<double>(st, WS_Q2_FORCEG, nq * g.dim);
-    UWG_HIP(hipMemcpy(de, eta_g, sizeof(double) * nq, hipMemcpyHostToDevice));
-    if (force_g)
-      UWG_HIP(hipMemcpy(df, force_g, sizeof(double) * nq * g.dim, hipMemcpyHostToDevice));
-    else
-      vk_fill<double>(nq * g.dim, 0.0, df, 0);
     launch_q2_weights(g, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, de, df,
                       st->q2->W, st->q2->fe, 0);
-    finish_assembly_q2(st);
-    st->assembled_state = -2;
-    return 0;
+    if (same_eta)
+      launch_q2_gather_f(g, st->q2->fe, st->q2->bc, st->f, 0);
+    else
+      finish_assembly_q2(st);
+  } else {
+    Level& A = st->L[0];
+    launch_gauss_assemble(g, de, df, A.W, st->fe, st->approxS, st->Ge, 0);
+    if (same_eta)
+      launch_node_gather_fe(g, st->fe, A.bc, st->f, 0);
+    else
+      finish_assembly(st);
   }
-  // Gauss-point viscosities -> W_g = eta_g * w_g * detJ ; f_e = sum_g w_g detJ f(g) N_a(g)
-  // (the GaussIntegrationSwarm path of the reference when no voronoi swarm is given)
-  const Geom& g = st->g;
-  const int dim = g.dim, Q = dim == 3 ? 27 : 9, NN = 1 << dim;
-  double h[3] = {1, 1, 1}, detJ = 1.0;
-  for (int d = 0; d < dim; d++) {
-    h[d] = (g.hi[d] - g.lo[d]) / g.res[d];
-    detJ *= 0.5 * h[d];
-  }
-  const double w1[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
-  const double ga = sqrt(0.6);
-  std::vector<double> W((size_t)Q * g.E), fe((size_t)g.E * NN * dim, 0.0), S(g.E);
-  long nx = g.nn[0], ny = g.nn[1];
-  for (long e = 0; e < g.E; e++) {
-    long i = e % g.res[0], j = (e / g.res[0]) % g.res[1], k = dim == 3 ? e / ((long)g.res[0] * g.res[1]) : 0;
-    long base = i + nx * (j + ny * k);
-    for (int q = 0; q < Q; q++) {
-      double wq = w1[q % 3] * w1[(q / 3) % 3] * (dim == 3 ? w1[q / 9] : 1.0);
-      W[(size_t)q * g.E + e] = eta_g[e * Q + q] * wq * detJ;
-      if (force_g) {
-        double X[3] = {q % 3 == 0 ? -ga : (q % 3 == 1 ? 0.0 : ga), (q / 3) % 3 == 0 ? -ga : ((q / 3) % 3 == 1 ? 0.0 : ga),
-                       q / 9 == 0 ? -ga : (q / 9 == 1 ? 0.0 : ga)};
-        double N[8];
-        for (int a = 0; a < NN; a++) {
-          N[a] = 1.0;
-          for (int d = 0; d < dim; d++) N[a] *= 0.5 * (1.0 + ((a >> d) & 1 ? 1.0 : -1.0) * X[d]);
-        }
-        double fq[3] = {0, 0, 0};
-        (void)base;
-        for (int d = 0; d < dim; d++) fq[d] = force_g[((size_t)e * Q + q) * dim + d];
-        for (int a = 0; a < NN; a++)
-          for (int d = 0; d < dim; d++) fe[(size_t)e * NN * dim + a * dim + d] += wq * detJ * fq[d] * N[a];
-      }
-    }
-  }
-  Level& A = st->L[0];
-  UWG_HIP(hipMemcpy2D(A.W, sizeof(double) * g.Es, W.data(), sizeof(double) * g.E, sizeof(double) * g.E, Q,
-                      hipMemcpyHostToDevice));
-  UWG_HIP(hipMemcpy(st->fe, fe.data(), sizeof(double) * fe.size(), hipMemcpyHostToDevice));
-  finish_assembly(st);
-  // approxS from the assembled diagonal: computed with the level-0 element weights
-  {
-    std::vector<double> Wh(W);
-    double sc[3] = {2.0 / h[0], 2.0 / h[1], dim == 3 ? 2.0 / h[2] : 1.0};
-    for (long e = 0; e < g.E; e++) {
-      double s = 0.0;
-      for (int a = 0; a < NN; a++)
-        for (int i = 0; i < dim; i++) {
-          double kd = 0.0;
-          for (int q = 0; q < Q; q++) {
-            double X[3] = {q % 3 == 0 ? -ga : (q % 3 == 1 ? 0.0 : ga), (q / 3) % 3 == 0 ? -ga : ((q / 3) % 3 == 1 ? 0.0 : ga),
-                           q / 9 == 0 ? -ga : (q / 9 == 1 ? 0.0 : ga)};
-            double dn[3], s2 = 0.0;
-            for (int jd = 0; jd < dim; jd++) {
-              double v = (dim == 3 ? 0.125 : 0.25) * (((a >> jd) & 1) ? 1.0 : -1.0);
-              for (int od = 0; od < dim; od++)
-                if (od != jd) v *= 1.0 + (((a >> od) & 1) ? 1.0 : -1.0) * X[od];
-              dn[jd] = v * sc[jd];
-              s2 += dn[jd] * dn[jd];
-            }
-            kd += Wh[(size_t)q * g.E + e] * (dn[i] * dn[i] + s2);
-          }
-          s += st->Ge[a * dim + i] * st->Ge[a * dim + i] / kd;
-        }
-      S[e] = s;
-    }
-    UWG_HIP(hipMemcpy(st->approxS, S.data(), sizeof(double) * g.E, hipMemcpyHostToDevice));
-  }
+  if (!same_eta) st->eta_g_prev.assign(eta_g, eta_g + nq);
   st->assembled_state = -2;
   return 0;
   API_END
@@ -2537,6 +2484,168 @@ extern "C" int uwg_stokes_level_mask(uwg_stokes* st, int level, unsigned char* m
   if (level < 0 || level >= (int)st->L.size()) return arg_error("bad level");
   API_BEGIN
   UWG_HIP(hipMemcpy(mask, st->L[level].bc, st->L[level].g.N, hipMemcpyDeviceToHost));
+  return 0;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// SUPG advection-diffusion (AdvectionDiffusion(method="SUPG"), PY/systems/_advectiondiffusion.py
+// :838-1010; AD/ = LU/Underworld/Utils/AdvectionDiffusion/src/)
+// ---------------------------------------------------------------------------------------
+struct uwg_advdiff {
+  uwg_field* phi = nullptr;
+  uwg_field* phidot = nullptr;
+  uwg_field* vel = nullptr;
+  Geom g;
+  uint8_t* bc = nullptr;      // bit 0: Dirichlet vertex
+  double* kap = nullptr;      // [E][3^dim] diffusivity at the Gauss points
+  double* src = nullptr;      // [E][3^dim] source, or null
+  double* res_e = nullptr;    // [E][2^dim]
+  double* me = nullptr;       // [E][2^dim]
+  double* M = nullptr;        // lumped mass [N]
+  double* phi_v = nullptr;    // solution vectors [N] (free entries)
+  double* pdot_v = nullptr;
+  double* partial = nullptr;
+  double kappa_max = 0.0;
+  bool have_coef = false;
+  double gamma = 0.5;         // AdvDiffMulticorrector defaults (Multicorrector.c:101-102)
+  int iterations = 2;
+  int upwind = 0;             // "Exact" (Residual.cpp:150)
+  double min_sep = 0.0, sep[3] = {1.0, 1.0, 1.0};
+};
+
+extern "C" int uwg_advdiff_create(uwg_field* phi, uwg_field* phidot, uwg_field* vel, const unsigned char* bcmask,
+                                  uwg_advdiff** out) {
+  if (!phi || !phidot || !vel) return arg_error("phiField, phiDotField and velocityField are required");
+  if (phi->on_elements || phi->ncomp != 1) return arg_error("phiField must be a nodal scalar field");
+  if (phidot->on_elements || phidot->ncomp != 1 || phidot->mesh != phi->mesh)
+    return arg_error("phiDotField must be a nodal scalar field on the phiField's mesh");
+  if (phi->mesh->order != 1) return arg_error("SUPG advection-diffusion is provided for Q1 meshes");
+  if (vel->on_elements || vel->mesh != phi->mesh || vel->ncomp != phi->mesh->g.dim)
+    return arg_error("velocityField must be a nodal vector field on the phiField's mesh");
+  API_BEGIN
+  uwg_advdiff* a = new uwg_advdiff();
+  a->phi = phi;
+  a->phidot = phidot;
+  a->vel = vel;
+  a->g = phi->mesh->g;
+  const Geom& g = a->g;
+  const int NN = 1 << g.dim, Q = g.dim == 3 ? 27 : 9;
+  a->bc = dalloc<uint8_t>(g.N);
+  if (bcmask)
+    UWG_HIP(hipMemcpy(a->bc, bcmask, g.N, hipMemcpyHostToDevice));
+  else
+    UWG_HIP(hipMemset(a->bc, 0, g.N));
+  a->kap = dalloc<double>(g.E * Q);
+  a->res_e = dalloc<double>(g.E * NN);
+  a->me = dalloc<double>(g.E * NN);
+  a->M = dalloc<double>(g.N);
+  a->phi_v = dalloc<double>(g.N);
+  a->pdot_v = dalloc<double>(g.N);
+  a->partial = dalloc<double>(512);
+  launch_supg_mass(g, a->me, a->M, 0);
+  // minimum vertex separation and its per-axis values (Mesh_HexType_GetMinimumSeparation,
+  // Mesh_HexType.c:177-272, taken from the first element attaining the minimum)
+  std::vector<double> ax[3];
+  for (int d = 0; d < g.dim; d++) {
+    ax[d].resize(g.res[d]);
+    for (int i = 0; i < g.res[d]; i++)
+      ax[d][i] = node_coord_1d(i + 1, g.res[d] + 1, g.lo[d], g.hi[d]) - node_coord_1d(i, g.res[d] + 1, g.lo[d], g.hi[d]);
+  }
+  double best = HUGE_VAL;
+  for (int k = 0; k < (g.dim == 3 ? g.res[2] : 1); k++)
+    for (int j = 0; j < g.res[1]; j++)
+      for (int i = 0; i < g.res[0]; i++) {
+        const double s0 = ax[0][i], s1 = ax[1][j], s2 = g.dim == 3 ? ax[2][k] : HUGE_VAL;
+        double cur = s0;
+        cur = s1 < cur ? s1 : cur;
+        if (g.dim == 3) cur = s2 < cur ? s2 : cur;
+        if (cur < best) {
+          best = cur;
+          a->sep[0] = s0;
+          a->sep[1] = s1;
+          a->sep[2] = g.dim == 3 ? s2 : 1.0;
+        }
+      }
+  a->min_sep = best;
+  *out = a;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_advdiff_destroy(uwg_advdiff* a) {
+  if (!a) return 0;
+  for (void* p : {(void*)a->bc, (void*)a->kap, (void*)a->src, (void*)a->res_e, (void*)a->me, (void*)a->M,
+                  (void*)a->phi_v, (void*)a->pdot_v, (void*)a->partial})
+    dfree(p);
+  delete a;
+  return 0;
+}
+
+extern "C" int uwg_advdiff_set_coefficients(uwg_advdiff* a, const double* kappa_g, const double* source_g) {
+  if (!kappa_g) return arg_error("diffusivity values are required");
+  const Geom& g = a->g;
+  const long nq = g.E * (g.dim == 3 ? 27 : 9);
+  double kmax = -1.0;  // AdvDiffResidualForceTerm_GetMaxDiffusivity starts below any diffusivity
+  for (long q = 0; q < nq; q++) {
+    if (std::isnan(kappa_g[q])) return arg_error("diffusivity is NaN");
+    if (kappa_g[q] > kmax) kmax = kappa_g[q];
+  }
+  API_BEGIN
+  UWG_HIP(hipMemcpy(a->kap, kappa_g, sizeof(double) * nq, hipMemcpyHostToDevice));
+  if (source_g) {
+    if (!a->src) a->src = dalloc<double>(nq);
+    UWG_HIP(hipMemcpy(a->src, source_g, sizeof(double) * nq, hipMemcpyHostToDevice));
+  } else {
+    dfree(a->src);
+    a->src = nullptr;
+  }
+  a->kappa_max = kmax;
+  a->have_coef = true;
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_advdiff_set_scheme(uwg_advdiff* a, double gamma, int iterations, int upwind) {
+  if (!(gamma >= 0.0 && gamma <= 1.0)) return arg_error("gamma must lie in [0, 1]");
+  if (iterations < 1) return arg_error("at least one corrector iteration is required");
+  if (upwind < 0 || upwind > 2) return arg_error("upwind must be 0 (exact), 1 (doubly asymptotic) or 2 (critical)");
+  a->gamma = gamma;
+  a->iterations = iterations;
+  a->upwind = upwind;
+  return 0;
+}
+
+// one time step (AdvDiffMulticorrector_Solve, Multicorrector.c:158-200): predictor on the
+// solution vectors, then the corrector passes; each pass assembles the residual from the nodal
+// fields and refreshes them from the vectors when it ends
+extern "C" int uwg_advdiff_integrate(uwg_advdiff* a, double dt) {
+  if (!a->have_coef) return arg_error("diffusivity / source not set");
+  if (!std::isfinite(dt)) return arg_error("dt must be finite");
+  API_BEGIN
+  const Geom& g = a->g;
+  launch_supg_predict(g.N, a->bc, a->phi->d, a->phidot->d, dt * (1.0 - a->gamma), a->phi_v, a->pdot_v, 0);
+  for (int it = 0; it < a->iterations; it++) {
+    launch_supg_elem(g, a->phi->d, a->phidot->d, a->vel->d, a->kap, a->src, a->upwind, a->res_e, 0);
+    launch_supg_correct(g, a->bc, a->res_e, a->M, dt * a->gamma, a->phi_v, a->pdot_v, a->phi->d, a->phidot->d, 0);
+  }
+  return 0;
+  API_END
+}
+
+// AdvectionDiffusionSLE_CalculateDt (Timestep.c:36-110): courant factor 0.5
+extern "C" int uwg_advdiff_max_dt(uwg_advdiff* a, double* dt) {
+  if (!a->have_coef) return arg_error("diffusivity / source not set");
+  API_BEGIN
+  const int nblk = 512;
+  launch_supg_adv_dt(a->g, a->vel->d, a->sep, a->partial, nblk, 0);
+  std::vector<double> h(nblk);
+  UWG_HIP(hipMemcpy(h.data(), a->partial, sizeof(double) * nblk, hipMemcpyDeviceToHost));
+  double adv = HUGE_VAL;
+  for (double v : h) adv = v < adv ? v : adv;
+  adv = 0.5 * adv;
+  const double dif = 0.5 * a->min_sep * a->min_sep / a->kappa_max;
+  *dt = adv < dif ? adv : dif;
   return 0;
   API_END
 }

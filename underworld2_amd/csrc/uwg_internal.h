@@ -83,6 +83,8 @@ void launch_moments(const Geom& g, const int* cell_ptr, const int* cell_pid, con
                     const double* eta_p, const double* force_p, double* W, double* fe, double* approxS,
                     const double* Ge /*host, 2^d*d values*/, int nmat, hipStream_t s);
 void launch_node_gather_fe(const Geom& g, const double* fe, const uint8_t* bc, double* f, hipStream_t s);
+void launch_gauss_assemble(const Geom& g, const double* eta_g, const double* force_g, double* W, double* fe,
+                           double* approxS, const double* Ge, hipStream_t s);
 void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, double* Wc, hipStream_t s);
 void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
 template <typename T>
@@ -154,6 +156,17 @@ void launch_q2_project_W(const Geom& g, const double* W, double* W1, hipStream_t
 void q2_element_G(const Geom& g, std::vector<double>& Gr);
 void launch_q2_interpolate(const Geom& g, long P, const double* f, int nc, const double* x, double* v, int* ok,
                            hipStream_t s);
+
+// advdiff_kernels.hip: SUPG advection-diffusion of a nodal scalar (Q1 mesh)
+void launch_supg_elem(const Geom& g, const double* phi, const double* phidot, const double* vel, const double* kap,
+                      const double* src, int upwind, double* res_e, hipStream_t s);
+void launch_supg_mass(const Geom& g, double* me, double* M, hipStream_t s);
+void launch_supg_predict(long N, const uint8_t* bc, const double* phi, const double* phidot, double a, double* phi_v,
+                         double* pdot_v, hipStream_t s);
+void launch_supg_correct(const Geom& g, const uint8_t* bc, const double* res_e, const double* M, double dtg,
+                         double* phi_v, double* pdot_v, double* phi, double* phidot, hipStream_t s);
+void launch_supg_adv_dt(const Geom& g, const double* vel, const double* sep, double* partial, int nblk,
+                        hipStream_t s);
 
 // vector kernels (T = double or float)
 template <typename T> void vk_fill(long n, T v, T* x, hipStream_t s);

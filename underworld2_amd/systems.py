@@ -498,3 +498,135 @@ class SwarmAdvector:
             raise RuntimeError(msg)
         self.time += dt
         self.dt = dt
+
+
+class AdvectionDiffusion:
+    """``AdvectionDiffusion(phiField, velocityField, fn_diffusivity, fn_sourceTerm=None,
+    method="SUPG", conditions=[], phiDotField=None)`` (PY/systems/_advectiondiffusion.py:16-140,
+    838-1010): d(phi)/dt + u.grad(phi) = div(k grad(phi)) + H by SUPG with the explicit
+    multicorrector (gamma 1/2, two corrector passes, lumped mass) on the GPU.
+
+    ``fn_diffusivity`` and ``fn_sourceTerm`` are evaluated at the Gauss points of the Q1 mesh
+    at every ``integrate`` (and re-used while the fields they read are unchanged)."""
+
+    def __init__(self, phiField=None, velocityField=None, fn_diffusivity=None, fn_sourceTerm=None, method="SUPG",
+                 conditions=[], phiDotField=None, allow_non_q1=False, gauss_swarm=None, **kwargs):
+        if not isinstance(method, str) or method.upper() not in ("SUPG", "SLCN"):
+            raise ValueError("'method' parameter must be 'SUPG' or 'SLCN'")
+        if method.upper() == "SLCN":
+            raise NotImplementedError("The semi-Lagrangian Crank-Nicolson method is not provided by this engine")
+        if phiDotField is None:
+            raise ValueError("'phiDotField' is required for the 'SUPG' method")
+        if not isinstance(phiField, MeshVariable):
+            raise TypeError("Provided 'phiField' must be of 'MeshVariable' class.")
+        if phiField.nodeDofCount != 1:
+            raise TypeError("Provided 'phiField' must be a scalar")
+        if not isinstance(phiDotField, MeshVariable) or phiDotField.data.shape != phiField.data.shape:
+            raise TypeError("Provided 'phiDotField' is not the same shape as the provided 'phiField'")
+        if not isinstance(velocityField, MeshVariable) or velocityField.nodeDofCount != phiField._base.dim:
+            raise TypeError("Provided 'velocityField' must be the same dimensionality as the phiField's mesh")
+        mesh = phiField._base
+        if phiField._on_elements or mesh._order != 1:
+            raise ValueError("The 'phiField' must be discretised on a Q1 mesh (the SUPG implementation is "
+                             "provided for Q1 elements).")
+        if velocityField._base is not mesh or velocityField._on_elements:
+            raise NotImplementedError("the velocity must live on the phiField's mesh in this engine")
+        if gauss_swarm is not None:
+            raise NotImplementedError("custom Gauss swarms are not provided by this engine")
+        if fn_diffusivity is None:
+            raise ValueError("'fn_diffusivity' is required")
+        if not isinstance(conditions, (list, tuple)):
+            conditions = [conditions]
+        mask = np.zeros(mesh.nodesGlobal, dtype=np.uint8)
+        for cond in conditions:
+            if not isinstance(cond, DirichletCondition):
+                raise TypeError("Provided 'conditions' must be a list of 'DirichletCondition' objects "
+                                "(Neumann fluxes are not provided by this engine).")
+            if cond.variable is phiField:
+                mask |= cond.mask(mesh.nodesGlobal) & np.uint8(1)
+        self._phiField = phiField
+        self._phiDotField = phiDotField
+        self._velocityField = velocityField
+        self._mesh = mesh
+        self._conditions = conditions
+        self.fn_diffusivity = fn_diffusivity
+        self.fn_sourceTerm = fn_sourceTerm
+        h = ctypes.c_void_p()
+        L.check(L.lib().uwg_advdiff_create(phiField._h, phiDotField._h, velocityField._h,
+                                           mask.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), ctypes.byref(h)),
+                "AdvectionDiffusion")
+        self._h = h
+        self._coef_key = None
+        self.method = "SUPG"
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.lib().uwg_advdiff_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def fn_diffusivity(self):
+        return self._fn_diffusivity
+
+    @fn_diffusivity.setter
+    def fn_diffusivity(self, value):
+        self._fn_diffusivity = _fn.convert(value)
+        self._coef_key = None
+
+    @property
+    def fn_sourceTerm(self):
+        return self._fn_sourceTerm
+
+    @fn_sourceTerm.setter
+    def fn_sourceTerm(self, value):
+        self._fn_sourceTerm = None if value is None else _fn.convert(value)
+        self._coef_key = None
+
+    @property
+    def phiField(self):
+        return self._phiField
+
+    @property
+    def velocityField(self):
+        return self._velocityField
+
+    def _prepare(self):
+        deps = self._fn_diffusivity._fn_inputs()
+        if self._fn_sourceTerm is not None:
+            deps = deps | self._fn_sourceTerm._fn_inputs()
+        if ("meshvar", id(self._phiField)) in deps or ("meshvar", id(self._phiDotField)) in deps:
+            raise NotImplementedError("diffusivity / source depending on the advected field are not provided")
+        key = (id(self._fn_diffusivity), id(self._fn_sourceTerm)) + tuple(sorted(deps))
+        varying = any(t[0] in ("meshvar", "swarmvar") for t in deps)
+        if key == self._coef_key and not varying:
+            return
+        pts = _gauss_points(self._mesh)
+        E, Q, d = pts.shape
+        flat = pts.reshape(-1, d)
+        kap = np.ascontiguousarray(np.asarray(self._fn_diffusivity.evaluate(flat), dtype=np.float64).reshape(E, Q))
+        src = None
+        if self._fn_sourceTerm is not None:
+            src = np.ascontiguousarray(np.asarray(self._fn_sourceTerm.evaluate(flat), dtype=np.float64).reshape(E, Q))
+        L.check(L.lib().uwg_advdiff_set_coefficients(self._h, L.dptr(kap), L.dptr(src) if src is not None else None),
+                "AdvectionDiffusion coefficients")
+        self._coef_key = key
+
+    def integrate(self, dt=0.0, **kwargs):
+        """One time step of size dt (_AdvDiffMulticorrector_Solve)."""
+        for f in (self._phiField, self._phiDotField, self._velocityField):
+            f._push()
+        self._prepare()
+        L.check(L.lib().uwg_advdiff_integrate(self._h, float(dt)), "AdvectionDiffusion.integrate")
+        self._phiField._mark_device_written()
+        self._phiDotField._mark_device_written()
+
+    def get_max_dt(self):
+        """0.5 x the smaller of the advective and diffusive limits (AdvectionDiffusionSLE_CalculateDt)."""
+        self._velocityField._push()
+        self._prepare()
+        v = ctypes.c_double()
+        L.check(L.lib().uwg_advdiff_max_dt(self._h, ctypes.byref(v)), "AdvectionDiffusion.get_max_dt")
+        return v.value
