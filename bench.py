@@ -194,6 +194,94 @@ def time_advection(P, steps=10):
 METRIC = "Stokes solves/sec at 128\u00b3 Q1/dQ0, \u03b7-contrast 1e3, rtol 1e-6; learned vs plain"
 
 
+def nusselt_3d(mesh, T):
+    """-int_top dT/dz dA / int_bottom T dA for a nodal Q1 temperature (exact for the trilinear
+    field: on a face the normal derivative is bilinear, its integral the mean of the corners)."""
+    nx, ny, nz = (r + 1 for r in mesh.elementRes)
+    Tg = T.reshape(nz, ny, nx)
+    X = mesh.data
+    x = X[:nx, 0]
+    y = X[::nx, 1][:ny]
+    z = X[::nx * ny, 2][:nz]
+    A = np.outer(np.diff(y), np.diff(x))
+    d = (Tg[-1] - Tg[-2]) / (z[-1] - z[-2])
+    top = np.sum(A * 0.25 * (d[1:, 1:] + d[1:, :-1] + d[:-1, 1:] + d[:-1, :-1]))
+    b = Tg[0]
+    bot = np.sum(A * 0.25 * (b[1:, 1:] + b[1:, :-1] + b[:-1, 1:] + b[:-1, :-1]))
+    return float(-top / bot)
+
+
+def build_convection(res, ppc, seed=0, Ra=1.0e6):
+    """3D thermal convection: unit box, free-slip walls, T = 1 at the bottom and 0 at the top,
+    eta = exp(-ln(1e4) T) and buoyancy Ra T z evaluated at the particles, initial T = 1 - z plus
+    a seeded 1e-2 sinusoidal perturbation, kappa = 1."""
+    import underworld2_amd as uw
+    fn = uw.function
+    mesh = uw.mesh.FeMesh_Cartesian("Q1/dQ0", res, (0.0,) * 3, (1.0,) * 3)
+    vel = uw.mesh.MeshVariable(mesh, 3)
+    p = uw.mesh.MeshVariable(mesh.subMesh, 1)
+    T = uw.mesh.MeshVariable(mesh, 1)
+    Tdot = uw.mesh.MeshVariable(mesh, 1)
+    vel.data[:] = 0.0
+    p.data[:] = 0.0
+    Tdot.data[:] = 0.0
+    X = mesh.data
+    rng = np.random.default_rng(seed)
+    kx, ky = rng.integers(1, 3, 2)
+    phase = rng.uniform(0.0, 2.0 * np.pi)
+    pert = np.cos(kx * np.pi * X[:, 0] + phase) * np.cos(ky * np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
+    T.data[:, 0] = np.clip(1.0 - X[:, 2] + 1.0e-2 * pert, 0.0, 1.0)
+    bot, top = mesh.specialSets["MinK_VertexSet"], mesh.specialSets["MaxK_VertexSet"]
+    T.data[bot.data, 0] = 1.0
+    T.data[top.data, 0] = 0.0
+    swarm = uw.swarm.Swarm(mesh)
+    swarm.populate_using_layout(uw.swarm.layouts.PerCellRandomLayout(swarm, particlesPerCell=ppc, seed=seed))
+    walls = tuple(mesh.specialSets[f"Min{n}_VertexSet"] + mesh.specialSets[f"Max{n}_VertexSet"] for n in "IJK")
+    stokes = uw.systems.Stokes(vel, p, fn_viscosity=fn.math.exp(-np.log(1.0e4) * T),
+                               fn_bodyforce=Ra * T * (0.0, 0.0, 1.0), voronoi_swarm=swarm,
+                               conditions=[uw.conditions.DirichletCondition(vel, walls)])
+    ad = uw.systems.AdvectionDiffusion(phiField=T, phiDotField=Tdot, velocityField=vel, fn_diffusivity=1.0,
+                                       conditions=[uw.conditions.DirichletCondition(T, (bot + top,))])
+    adv = uw.systems.SwarmAdvector(vel, swarm, order=2)
+    return dict(uw=uw, mesh=mesh, vel=vel, p=p, T=T, Tdot=Tdot, swarm=swarm, stokes=stokes, ad=ad, adv=adv)
+
+
+def run_convection(C, steps, mode, rtol=1e-6, log_every=0):
+    """Each step (the loop of the reference's convection examples): Stokes solve from the current
+    T (mode fp64 | mixed | learned); dt = min of the advection-diffusion and swarm limits for
+    the new velocity; SUPG step of T; RK2 swarm advection.  Returns per-step series and timings."""
+    s = make_solver({"uw": C["uw"], "stokes": C["stokes"]}, mode, rtol)
+    if s is None:
+        return None
+    out = dict(mode=mode, t=[], dt=[], nu=[], vrms=[], its=[], s_supg=0.0, s_stokes=0.0, s_adv=0.0)
+    t = 0.0
+    for k in range(steps):
+        t0 = time.perf_counter()
+        s.solve()
+        st = s.get_stats()
+        t1 = time.perf_counter()
+        dt = min(C["ad"].get_max_dt(), C["adv"].get_max_dt())
+        t2 = time.perf_counter()
+        C["ad"].integrate(dt)
+        C["T"].data  # noqa: B018 (sync point for the timing)
+        t3 = time.perf_counter()
+        C["adv"].integrate(dt)
+        C["swarm"].particleCoordinates.data  # noqa: B018
+        t4 = time.perf_counter()
+        t += dt
+        out["t"].append(t)
+        out["dt"].append(dt)
+        out["nu"].append(nusselt_3d(C["mesh"], C["T"].data[:, 0]))
+        out["vrms"].append(C["stokes"].velocity_rms())
+        out["its"].append(int(st.outer_its))
+        out["s_stokes"] += t1 - t0
+        out["s_supg"] += t3 - t2
+        out["s_adv"] += t4 - t3
+        if log_every and (k % log_every == 0 or k == steps - 1):
+            log(f"  {mode} step {k}: t {t:.4e} Nu {out['nu'][-1]:.5f} v_rms {out['vrms'][-1]:.4f} its {st.outer_its}")
+    return out
+
+
 def stream_copy(n=1 << 27, reps=20):
     """Attainable HBM bandwidth: a device copy of n doubles (read + write 16 n bytes)."""
     import underworld2_amd._lib as L
@@ -225,6 +313,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-advection", action="store_true")
     ap.add_argument("--apply-only", action="store_true")
+    ap.add_argument("--convection", action="store_true",
+                    help="run the 3D thermal-convection time series (--res, --ppc, --steps, --modes) instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -232,6 +322,9 @@ def main():
         sys.exit(2)
     import underworld2_amd as uw
     log(uw.device_info())
+    if args.convection:
+        convection_main(args)
+        return
     res = (args.res,) * 3
     P = build_problem(res, args.ppc)
     if args.apply_only:
@@ -275,6 +368,33 @@ def main():
     for plain in ("fp64", "schur"):
         if plain in modes:
             line[f"speedup_vs_{plain}"] = H["solves_per_sec"] / modes[plain]["solves_per_sec"]
+    print(json.dumps(line), flush=True)
+
+
+def convection_main(args):
+    """3D thermal convection (Ra 1e6, eta = exp(-ln(1e4) T)): the same initial state run once per
+    mode; Nu and v_rms series of each mode and their largest relative difference from the
+    FP64 series."""
+    res = (args.res,) * 3
+    runs = {}
+    for m in [x.strip() for x in args.modes.split(",") if x.strip()]:
+        C = build_convection(res, args.ppc)
+        log(f"convection {res} {args.ppc} ppc, mode {m}, {args.steps} steps")
+        r = run_convection(C, args.steps, m, rtol=args.rtol, log_every=max(1, args.steps // 10))
+        if r is not None:
+            runs[m] = r
+    line = {"workload": f"3D thermal convection {args.res}^3 Q1/dQ0, Ra 1e6, eta = exp(-ln(1e4) T), {args.ppc} ppc, "
+                        f"{args.steps} steps of SUPG + Stokes (rtol {args.rtol}) + RK2", "modes": {}}
+    ref = runs.get("fp64")
+    for m, r in runs.items():
+        e = {"final_nu": r["nu"][-1], "final_vrms": r["vrms"][-1], "final_time": r["t"][-1],
+             "mean_stokes_its": float(np.mean(r["its"])), "s_per_step": (r["s_supg"] + r["s_stokes"] + r["s_adv"]) / len(r["t"]),
+             "s_stokes_per_step": r["s_stokes"] / len(r["t"]), "s_supg_per_step": r["s_supg"] / len(r["t"]),
+             "s_rk2_per_step": r["s_adv"] / len(r["t"])}
+        if ref is not None and m != "fp64":
+            e["max_rel_diff_nu_vs_fp64"] = float(np.max(np.abs(np.array(r["nu"]) - ref["nu"]) / np.abs(ref["nu"])))
+            e["max_rel_diff_vrms_vs_fp64"] = float(np.max(np.abs(np.array(r["vrms"]) - ref["vrms"]) / np.abs(ref["vrms"])))
+        line["modes"][m] = e
     print(json.dumps(line), flush=True)
 
 

@@ -142,3 +142,82 @@ def test_config_layered_2d_1024_properties():
     dp = _rel(pm - pm.mean(), pf - pf.mean())
     print(f"mixed vs fp64: velocity {du:.2e}, pressure {dp:.2e}", flush=True)
     assert du < 1e-3 and dp < 1e-3
+
+
+def _oracle_convection(res, ppc, steps, T0, Ra=1.0e6):
+    """The 3D convection loop restated on the CPU: the Stokes solve with particle quadrature
+    (coincident map, DVC, eta and buoyancy from T interpolated at the particles, assembly, exact
+    solve), dt from the new velocity, the SUPG step of T (oracle/supg_oracle.py), then RK2
+    advection and relocation."""
+    from oracle import oracle as O
+    from oracle import supg_oracle as SO
+    mesh = O.Mesh(res, (0.0,) * 3, (1.0,) * 3)
+    sm = SO.Q1Mesh(res, (0.0,) * 3, (1.0,) * 3)
+    X = sm.coords()
+    Tb = (np.abs(X[:, 2]) < 1e-12) | (np.abs(X[:, 2] - 1.0) < 1e-12)
+    sup = SO.SUPG(sm, Tb)
+    kap = np.ones((sm.E, 27))
+    src = np.zeros((sm.E, 27))
+    coords = O.layout_random(mesh, ppc, 0)
+    ptr, pid = O.initial_cells(mesh, ppc)
+    tables = O.CellTables(mesh, ptr, pid)
+    T, Td = T0.copy(), np.zeros(sm.N)
+    vel = np.zeros((sm.N, 3))
+    bc = mesh.freeslip_mask()
+    out = []
+    for _ in range(steps):
+        ptr, pid = tables.csr()
+        xi = O.coincident_map(mesh, ptr, pid, coords)
+        xw, w = O.dvc_all(3, ptr, xi)
+        Tf = np.stack([T, np.zeros_like(T), np.zeros_like(T)], 1)
+        Tp = O.interpolate(mesh, Tf, coords)[0][:, 0]
+        eta = np.exp(-np.log(1.0e4) * Tp)[pid]
+        force = (Ra * Tp)[pid][:, None] * np.array([0.0, 0.0, 1.0])[None, :]
+        Ke, fe, Ge, S = O.assemble_elements(mesh, ptr, xw, w, eta, force)
+        sysm = O.System(mesh, bc, Ke, fe, Ge, S)
+        vel, _ = sysm.solve()
+        dt = min(sup.max_dt(vel, kap), O.max_dt(mesh, vel))
+        T, Td = sup.integrate(T, Td, vel, kap, src, dt)
+        coords, bad, _ = O.advect_rk2(mesh, vel, coords, dt)
+        assert bad == 0
+        tables.update(coords)
+        out.append(dict(dt=dt, T=T.copy(), vel=vel.copy(), coords=coords.copy()))
+    return out
+
+
+def test_config_thermal_convection_small():
+    """The 3D thermal-convection configuration (Ra 1e6, eta = exp(-ln(1e4) T), Stokes + SUPG +
+    RK2 per step) at 8^3 with 8 particles per cell: two FP64 steps against the CPU restatement of
+    the whole loop, then FP64 against mixed precision over eight steps on Nu and v_rms."""
+    import bench
+    res, ppc = (8, 8, 8), 8
+    print(f"thermal convection {res}, {ppc} ppc: GPU FP64 loop against the oracle", flush=True)
+    C = bench.build_convection(res, ppc)
+    T0 = C["T"].data[:, 0].copy()
+    ref = _oracle_convection(res, ppc, 2, T0)
+    s = bench.make_solver({"uw": C["uw"], "stokes": C["stokes"]}, "fp64", 1e-11)
+    for k in range(2):
+        s.solve()
+        dt = min(C["ad"].get_max_dt(), C["adv"].get_max_dt())
+        assert dt == pytest.approx(ref[k]["dt"], rel=1e-7)
+        C["ad"].integrate(dt)
+        C["adv"].integrate(dt)
+        eT = _rel(C["T"].data[:, 0], ref[k]["T"])
+        eu = _rel(C["vel"].data, ref[k]["vel"])
+        ec = float(np.abs(C["swarm"].particleCoordinates.data - ref[k]["coords"]).max())
+        print(f"step {k}: dt {dt:.4e}, T error {eT:.2e}, velocity error {eu:.2e}, max particle offset {ec:.2e}",
+              flush=True)
+        # the GPU solve stops at rtol 1e-11 (the oracle solves exactly); T and the particles inherit
+        # the velocity's error through dt and the advection
+        assert eT < 1e-8 and eu < 1e-8 and ec < 1e-8
+    print("FP64 against mixed precision over 8 steps", flush=True)
+    runs = {}
+    for m in ("fp64", "mixed"):
+        C = bench.build_convection(res, ppc)
+        runs[m] = bench.run_convection(C, 8, m, rtol=1e-8)
+    nu = np.array(runs["fp64"]["nu"]), np.array(runs["mixed"]["nu"])
+    vr = np.array(runs["fp64"]["vrms"]), np.array(runs["mixed"]["vrms"])
+    print(f"Nu {nu[0][-1]:.6f} / {nu[1][-1]:.6f}, v_rms {vr[0][-1]:.4f} / {vr[1][-1]:.4f}", flush=True)
+    assert np.max(np.abs(nu[1] - nu[0]) / nu[0]) < 1e-6
+    assert np.max(np.abs(vr[1] - vr[0]) / vr[0]) < 1e-6
+    assert nu[0][-1] > 1.0 and vr[0][-1] > 0.0

@@ -130,8 +130,24 @@ class LearnedCoarseCorrection:
     def nodes(self):
         return tuple(r + 1 for r in self.level_res)
 
-    def attach(self, stokes):
-        lev = level_index(stokes, self.level_res)
+    def attach(self, stokes, level_res=None):
+        """Bind to the multigrid level of resolution ``level_res`` (default: the trained level;
+        when the system has no such level, the level as many halvings below the finest grid as
+        in training -- the inputs and output are non-dimensional, so the network transfers)."""
+        if level_res is None:
+            res = levels(stokes)
+            if tuple(self.level_res) in res:
+                level_res = self.level_res
+            else:
+                train_fine = int(self.meta.get("res", 0))
+                if train_fine <= 0 or train_fine % self.level_res[0]:
+                    raise ValueError(f"no multigrid level with resolution {self.level_res}; levels are {res}")
+                factor = train_fine // self.level_res[0]
+                want = tuple(r // factor for r in res[0])
+                if any(r % factor for r in res[0]) or want not in res:
+                    raise ValueError(f"no multigrid level {factor}x coarser than {res[0]}; levels are {res}")
+                level_res = want
+        lev = level_index(stokes, level_res)
         L.check(L.lib().uwg_stokes_load_nn(stokes._h, lev, self.params.size, L.fptr(self.params), self.channels,
                                            self.depth), "load learned correction")
         L.check(L.lib().uwg_stokes_set_nn_mode(stokes._h, self.MODES[self.mode]), "learned correction mode")
