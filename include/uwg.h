@@ -218,6 +218,17 @@ int uwg_stream_copy(long n, int reps, double *ms);
 /* time `reps` applies of K on a multigrid level in the given precision; ms per apply */
 int uwg_stokes_time_apply(uwg_stokes *st, int precision, int level, int reps, double *ms);
 
+/* ---- population control: PopulationControl(swarm, deleteThreshold, splitThreshold,
+ *      maxDeletions, maxSplits).repopulate() (PY/swarm/_population_control.py:13-140;
+ *      LU/PICellerator/Utils/src/PCDVC.c:313-1085, non-aggressive path).  Splits particles whose
+ *      DVC weight exceeds splitThreshold * 2^dim (at most maxSplits per cell, the copy placed at
+ *      the centroid of the parent's Voronoi cell and appended), deletes those below
+ *      deleteThreshold * 2^dim (at most maxDeletions per cell, swap-with-last).  src[i]
+ *      (capacity src_cap >= P + E * maxSplits) receives the original index whose data particle i
+ *      carries; *p_new the new particle count */
+int uwg_swarm_population_control(uwg_swarm *s, double delete_threshold, double split_threshold, int max_deletions,
+                                 int max_splits, long src_cap, long *src, long *p_new);
+
 /* ---- SUPG advection-diffusion: AdvectionDiffusion(phiField, velocityField, fn_diffusivity,
  *      fn_sourceTerm, method="SUPG", conditions, phiDotField)
  *      (PY/systems/_advectiondiffusion.py:838-1010; AD/ = LU/Underworld/Utils/AdvectionDiffusion/src/).

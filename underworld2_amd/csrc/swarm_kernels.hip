@@ -668,3 +668,33 @@ void launch_node_speed_max(const Geom& g, const double* vel, double* partial, in
 }
 
 }  // namespace uwg
+
+namespace uwg {
+// particle arrays rebuilt after population control: coords/material of particle i from src[i]
+__global__ void k_gather_particles(long P, int dim, const long* __restrict__ src, const double* __restrict__ coords,
+                                   const int* __restrict__ mat, double* __restrict__ ncoords, int* __restrict__ nmat) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const long s = src[i];
+  for (int d = 0; d < dim; d++) ncoords[i * dim + d] = coords[s * dim + d];
+  nmat[i] = mat[s];
+}
+__global__ void k_scatter_coords(long n, int dim, const int* __restrict__ idx, const double* __restrict__ val,
+                                 double* __restrict__ coords) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int d = 0; d < dim; d++) coords[(long)idx[k] * dim + d] = val[k * dim + d];
+}
+void launch_gather_particles(long P, int dim, const long* src, const double* coords, const int* mat, double* ncoords,
+                             int* nmat, hipStream_t s) {
+  if (P == 0) return;
+  hipLaunchKernelGGL(k_gather_particles, (unsigned)((P + 255) / 256), 256, 0, s, P, dim, src, coords, mat, ncoords,
+                     nmat);
+  UWG_HIP(hipGetLastError());
+}
+void launch_scatter_coords(long n, int dim, const int* idx, const double* val, double* coords, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_scatter_coords, (unsigned)((n + 255) / 256), 256, 0, s, n, dim, idx, val, coords);
+  UWG_HIP(hipGetLastError());
+}
+}  // namespace uwg

@@ -2649,3 +2649,188 @@ extern "C" int uwg_advdiff_max_dt(uwg_advdiff* a, double* dt) {
   return 0;
   API_END
 }
+
+// ---------------------------------------------------------------------------------------
+// population control: PopulationControl(swarm, deleteThreshold, splitThreshold, maxDeletions,
+// maxSplits).repopulate() (PY/swarm/_population_control.py:13-140), the non-aggressive path of
+// _PCDVC_Calculate3D / 2D (LU/PICellerator/Utils/src/PCDVC.c:313-700, 703-1085).
+// The DVC weights of every cell come from the device pass; the edits are the reference's, in
+// its order: cells in order; in a cell, particles whose weight exceeds split*2^dim are split
+// (first maxSplits in table order; the copy is appended to the swarm and to the cell at the
+// centroid of the parent's Voronoi cell, FeMesh_CoordLocalToGlobal), then particles below
+// delete*2^dim are deleted (by descending swarm index, at most maxDeletions) with the swarm's
+// swap-with-last removal (Swarm_DeleteParticle, SwarmClass.c:1016-1082).  src[i] receives the
+// original index whose data particle i now carries (for the caller's per-particle variables).
+// ---------------------------------------------------------------------------------------
+extern "C" int uwg_swarm_population_control(uwg_swarm* s, double delete_threshold, double split_threshold,
+                                            int max_deletions, int max_splits, long src_cap, long* src,
+                                            long* p_new) {
+  if (!(delete_threshold >= 0.0 && delete_threshold <= 1.0) || !(split_threshold >= 0.0 && split_threshold <= 1.0))
+    return arg_error("thresholds must lie in [0, 1]");
+  if (delete_threshold > split_threshold) return arg_error("'deleteThreshold' must be less than 'splitThreshold'");
+  if (max_deletions < 0 || max_splits < 0) return arg_error("maxDeletions and maxSplits must be non-negative");
+  const Geom& g = s->mesh->g;
+  const int dim = g.dim;
+  const long E = g.E;
+  int rc = uwg_swarm_integration_points(s, 1, 15);
+  if (rc) return rc;
+  API_BEGIN
+  const long P0 = s->P;
+  std::vector<int> ptr(E + 1), pid(P0);
+  std::vector<double> w(P0), cen((size_t)P0 * dim);
+  UWG_HIP(hipMemcpy(ptr.data(), s->cell_ptr, sizeof(int) * (E + 1), hipMemcpyDeviceToHost));
+  if (P0) {
+    UWG_HIP(hipMemcpy(pid.data(), s->cell_pid, sizeof(int) * P0, hipMemcpyDeviceToHost));
+    UWG_HIP(hipMemcpy(w.data(), s->w_slot, sizeof(double) * P0, hipMemcpyDeviceToHost));
+    UWG_HIP(hipMemcpy(cen.data(), s->xi_slot, sizeof(double) * P0 * dim, hipMemcpyDeviceToHost));
+  }
+  const double vol = (double)(1 << dim);
+  const double maxW = split_threshold * vol, minW = delete_threshold * vol;
+  // any deletions at all?  (they need the full bookkeeping of current indices)
+  bool any_del = false;
+  if (max_deletions > 0)
+    for (long k = 0; k < P0 && !any_del; k++) any_del = w[k] < minW;
+  std::vector<long> rsrc(P0);                  // data source of the particle at each current index
+  for (long k = 0; k < P0; k++) rsrc[k] = k;
+  std::vector<int> rnew(P0, -1);               // index into newc for particles created here
+  std::vector<double> newc;                    // coordinates of created particles
+  std::vector<std::vector<int>> mod(E);        // tables of edited cells (others: the CSR slice)
+  std::vector<char> edited(E, 0);
+  std::vector<int> cell_of;                    // current index -> cell (deletions only)
+  if (any_del) {
+    cell_of.assign(P0, 0);
+    for (long e = 0; e < E; e++)
+      for (int k = ptr[e]; k < ptr[e + 1]; k++) cell_of[pid[k]] = (int)e;
+  }
+  auto table = [&](long e) -> std::vector<int>& {
+    if (!edited[e]) {
+      mod[e].assign(pid.begin() + ptr[e], pid.begin() + ptr[e + 1]);
+      edited[e] = 1;
+    }
+    return mod[e];
+  };
+  long P = P0;
+  for (long e = 0; e < E; e++) {
+    const int s0 = ptr[e], n0 = ptr[e + 1] - ptr[e];
+    std::vector<int> splits;
+    std::vector<std::pair<long, int>> dels;  // (index on the swarm, position in the cell)
+    for (int i = 0; i < n0; i++) {
+      if (w[s0 + i] > maxW) splits.push_back(i);
+      if (w[s0 + i] < minW) dels.push_back({edited[e] ? (long)mod[e][i] : (long)pid[s0 + i], i});
+    }
+    const int ns = std::min((int)splits.size(), max_splits);
+    const int nd = std::min((int)dels.size(), max_deletions);
+    if (ns == 0 && nd == 0) continue;
+    std::vector<int>& T = table(e);
+    for (int k = 0; k < ns; k++) {
+      const int i = splits[k];
+      const long parent = T[i];
+      double xi[3] = {0.0, 0.0, 0.0}, v[24], N[8], x[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < dim; d++) xi[d] = cen[(size_t)(s0 + i) * dim + d];
+      if (dim == 3) {
+        element_verts<3>(g.res, g.lo, g.hi, e, v);
+        shape<3>(xi, N);
+      } else {
+        element_verts<2>(g.res, g.lo, g.hi, e, v);
+        shape<2>(xi, N);
+      }
+      for (int a = 0; a < (dim == 3 ? 8 : 4); a++)
+        for (int d = 0; d < dim; d++) x[d] += N[a] * v[a * dim + d];
+      rsrc.push_back(rsrc[parent]);
+      rnew.push_back((int)(newc.size() / dim));
+      for (int d = 0; d < dim; d++) newc.push_back(x[d]);
+      if (any_del) cell_of.push_back((int)e);
+      T.push_back((int)P);
+      P++;
+    }
+    if (nd > 0) {
+      std::sort(dels.begin(), dels.end(), [](const std::pair<long, int>& a, const std::pair<long, int>& b) {
+        return a.first > b.first;
+      });
+      for (int k = 0; k < nd; k++) {
+        const long idx = dels[k].first;
+        // remove from its cell: the cell's last entry takes its place
+        std::vector<int>& Tc = table(cell_of[idx]);
+        for (size_t q = 0; q < Tc.size(); q++)
+          if (Tc[q] == idx) {
+            Tc[q] = Tc.back();
+            Tc.pop_back();
+            break;
+          }
+        const long last = P - 1;
+        if (idx != last) {
+          std::vector<int>& Tl = table(cell_of[last]);
+          for (size_t q = 0; q < Tl.size(); q++)
+            if (Tl[q] == last) {
+              Tl[q] = (int)idx;
+              break;
+            }
+          rsrc[idx] = rsrc[last];
+          rnew[idx] = rnew[last];
+          cell_of[idx] = cell_of[last];
+        }
+        rsrc.pop_back();
+        rnew.pop_back();
+        cell_of.pop_back();
+        P--;
+      }
+    }
+  }
+  if (P > src_cap) return arg_error("population control: output capacity too small");
+  for (long k = 0; k < P; k++) src[k] = rsrc[k];
+  *p_new = P;
+  bool changed = P != P0;
+  for (long e = 0; e < E && !changed; e++) changed = edited[e];
+  if (!changed) return 0;
+  // new cell tables, owners
+  std::vector<int> nptr(E + 1, 0), npid(P), nown(P);
+  for (long e = 0; e < E; e++) {
+    const int c = edited[e] ? (int)mod[e].size() : ptr[e + 1] - ptr[e];
+    nptr[e + 1] = nptr[e] + c;
+    for (int q = 0; q < c; q++) {
+      const int i = edited[e] ? mod[e][q] : pid[ptr[e] + q];
+      npid[nptr[e] + q] = i;
+      nown[i] = (int)e;
+    }
+  }
+  // device: gather coordinates and material index by source, then the created coordinates
+  std::vector<long> hsrc(rsrc.begin(), rsrc.begin() + P);
+  std::vector<int> cidx;
+  std::vector<double> cval;
+  for (long k = 0; k < P; k++)
+    if (rnew[k] >= 0) {
+      cidx.push_back((int)k);
+      for (int d = 0; d < dim; d++) cval.push_back(newc[(size_t)rnew[k] * dim + d]);
+    }
+  long* dsrc = dalloc<long>(P);
+  UWG_HIP(hipMemcpy(dsrc, hsrc.data(), sizeof(long) * P, hipMemcpyHostToDevice));
+  double* ncoords = dalloc<double>(P * dim);
+  int* nmat = dalloc<int>(P);
+  launch_gather_particles(P, dim, dsrc, s->coords, s->mat, ncoords, nmat, 0);
+  if (!cidx.empty()) {
+    int* dci = dalloc<int>((long)cidx.size());
+    double* dcv = dalloc<double>((long)cval.size());
+    UWG_HIP(hipMemcpy(dci, cidx.data(), sizeof(int) * cidx.size(), hipMemcpyHostToDevice));
+    UWG_HIP(hipMemcpy(dcv, cval.data(), sizeof(double) * cval.size(), hipMemcpyHostToDevice));
+    launch_scatter_coords((long)cidx.size(), dim, dci, dcv, ncoords, 0);
+    UWG_HIP(hipDeviceSynchronize());
+    dfree(dci);
+    dfree(dcv);
+  }
+  swarm_reserve(s, P);
+  UWG_HIP(hipMemcpy(s->coords, ncoords, sizeof(double) * P * dim, hipMemcpyDeviceToDevice));
+  UWG_HIP(hipMemcpy(s->mat, nmat, sizeof(int) * P, hipMemcpyDeviceToDevice));
+  UWG_HIP(hipMemcpy(s->owner, nown.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(s->cell_ptr, nptr.data(), sizeof(int) * (E + 1), hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(s->cell_pid, npid.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+  UWG_HIP(hipDeviceSynchronize());
+  dfree(dsrc);
+  dfree(ncoords);
+  dfree(nmat);
+  s->P = P;
+  for (int v = 0; v < 8; v++) s->var_rows[v] = 0;  // swarm variables must be uploaded again
+  s->state++;
+  s->has_integration = false;
+  return 0;
+  API_END
+}

@@ -69,6 +69,9 @@ void launch_advect_rk2(const Geom& g, long P, const double* vel, double* coords,
 void launch_interpolate(const Geom& g, long P, const double* vel, const double* x, double* v,
                         int* ok, hipStream_t s);
 void launch_node_speed_max(const Geom& g, const double* vel, double* partial, int nblk, hipStream_t s);
+void launch_gather_particles(long P, int dim, const long* src, const double* coords, const int* mat, double* ncoords,
+                             int* nmat, hipStream_t s);
+void launch_scatter_coords(long n, int dim, const int* idx, const double* val, double* coords, hipStream_t s);
 
 // ---------------------------------------------------------------------------------------
 // launch wrappers: fem_kernels.hip (solver / assembly kernels)

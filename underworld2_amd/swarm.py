@@ -183,7 +183,83 @@ class Swarm:
         return xi, w
 
 
+class PopulationControl:
+    """``PopulationControl(swarm, deleteThreshold=0.006, splitThreshold=0.25, maxDeletions=0,
+    maxSplits=3)`` (PY/swarm/_population_control.py:13-140): per element, particles whose DVC
+    volume fraction exceeds ``splitThreshold`` are split (a copy at the centroid of their Voronoi
+    cell), those below ``deleteThreshold`` deleted.  Every SwarmVariable follows its particles."""
+
+    def __init__(self, swarm, deleteThreshold=0.006, splitThreshold=0.25, maxDeletions=0, maxSplits=3,
+                 aggressive=False, aggressiveThreshold=0.8, particlesPerCell=None, **kwargs):
+        if not isinstance(swarm, Swarm):
+            raise ValueError("Provided 'swarm' must be of class 'Swarm'.")
+        for name, v in (("deleteThreshold", deleteThreshold), ("splitThreshold", splitThreshold)):
+            if not isinstance(v, (int, float)):
+                raise TypeError(f"'{name}' parameter must of type 'float'.")
+            if v < 0.0 or v > 1.0:
+                raise ValueError(f"'{name}' parameter must take values in [0.,1.].")
+        if deleteThreshold > splitThreshold:
+            raise ValueError("'deleteThreshold' must be less than 'splitThreshold'.")
+        for name, v in (("maxDeletions", maxDeletions), ("maxSplits", maxSplits)):
+            if not isinstance(v, int):
+                raise TypeError(f"'{name}' parameter must of type 'int'.")
+            if v < 0:
+                raise ValueError(f"'{name}' parameter must take non-negative values.")
+        if not isinstance(aggressive, bool):
+            raise TypeError("'aggressive' parameter must of type 'bool'.")
+        if aggressive:
+            raise NotImplementedError("aggressive population control is not provided by this engine")
+        self._swarm = swarm
+        self._args = (float(deleteThreshold), float(splitThreshold), int(maxDeletions), int(maxSplits))
+
+    def repopulate(self):
+        """Split / delete particles, then move every swarm variable's rows with its particles."""
+        sw = self._swarm
+        sw._push()
+        P0 = sw.particleLocalCount
+        for v in sw._variables:  # bring every variable to the current particle count first
+            v.data
+        cap = P0 + sw.mesh.elementsGlobal * self._args[3]
+        src = np.zeros(max(cap, 1), dtype=np.int64)
+        pn = ctypes.c_long()
+        L.check(L.lib().uwg_swarm_population_control(sw._h, *self._args, cap,
+                                                      src.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                                      ctypes.byref(pn)), "PopulationControl.repopulate")
+        src = src[:pn.value]
+        if pn.value != P0 or np.any(src != np.arange(pn.value)):
+            for v in sw._variables:
+                v._data = v._data[src].copy()
+                v.version += 1
+            sw._device_moved()
+
+
 class _Layouts:
+    class PerCellGaussLayout:
+        """``PerCellGaussLayout(swarm, gaussPointCount)`` (PY/swarm/layouts.py:53-100): particles
+        at the tensor Gauss-Legendre points of every element, elements in order, x fastest."""
+
+        def __init__(self, swarm, gaussPointCount, **kwargs):
+            if not isinstance(gaussPointCount, int):
+                raise TypeError("'gaussPointCount' object passed in must be of type 'int'")
+            if gaussPointCount not in (1, 2, 3, 4, 5):
+                raise ValueError("'gaussPointCount' object passed in must take a value between \n between 1 "
+                                 "and 5 inclusive.")
+            self.swarm = swarm
+            self.gaussPointCount = gaussPointCount
+
+        def _populate(self, swarm):
+            m = swarm.mesh
+            d = m.dim
+            g, _ = np.polynomial.legendre.leggauss(self.gaussPointCount)
+            res = np.array(m.elementRes)
+            lo, hi = np.array(m.minCoord), np.array(m.maxCoord)
+            h = (hi - lo) / res
+            idx = np.indices(tuple(res[::-1])).reshape(d, -1)[::-1]          # element (i, j, k), x fastest
+            cen = np.stack([lo[k] + (idx[k] + 0.5) * h[k] for k in range(d)], -1)
+            q = np.stack(np.meshgrid(*([g] * d), indexing="ij"), -1).reshape(-1, d)[:, ::-1]  # x fastest
+            pts = (cen[:, None, :] + q[None, :, :] * (0.5 * h)[None, None, :]).reshape(-1, d)
+            swarm.add_particles_with_coordinates(pts)
+
     class PerCellRandomLayout:
         """``PerCellRandomLayout(swarm, particlesPerCell, seed=13)`` (PY/swarm/layouts.py:186-229)."""
 
