@@ -212,6 +212,8 @@ int uwg_nn_eval(uwg_nn *h, int B, float *loss);
 /* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply (a negative kz3 fixes the
  * 3D chunk at -kz3 on every level instead of adapting it to the grid) */
 int uwg_set_apply_chunk(int kz3, int kz2);
+/* 3D Q2 K apply on the matrix cores (1, default) or the vector kernel (0) */
+int uwg_set_q2_mfma(int on);
 /* STREAM-style device copy of n doubles (reads 8n bytes, writes 8n bytes); ms per copy.
  * Measures attainable HBM bandwidth and calibrates byte counters. */
 int uwg_stream_copy(long n, int reps, double *ms);

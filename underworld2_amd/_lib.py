@@ -47,6 +47,7 @@ SIGNATURES = {
     "uwg_device_info": [ctypes.c_char_p, ctypes.c_int],
     "uwg_mesh_create": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
     "uwg_mesh_create_q2": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
+    "uwg_set_q2_mfma": [ctypes.c_int],
     "uwg_swarm_population_control": [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                      ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)],
     "uwg_advdiff_create": [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(_vp)],

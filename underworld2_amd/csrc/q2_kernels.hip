@@ -22,6 +22,7 @@
 // fixed order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -243,6 +244,172 @@ void launch_q2_gather_f(const Geom& g, const double* fe, const uint8_t* bc, doub
 }
 
 // ---------------------------------------------------------------------------------------
+// 3D K apply on the matrix cores.  Both contractions of the element operator are GEMMs with
+// one constant matrix D[a][(q,j)] = dN_a/dx_j(g_q) (27 x 375, padded to 32 x 384):
+//   G = X D      rows: (element, component), 5 elements per workgroup -> 15 (+1) rows
+//   T = W (G + G^T) pointwise per quadrature point (in LDS)
+//   Y = T D^T    scattered to the element's vertices (colour launches, no atomics)
+// with v_mfma_{f32,f64}_16x16x4.  Operand layouts (checked on the device): A[i][k] in lane
+// i + 16 k, B[k][j] in lane j + 16 k; the f32 result row of lane l, value v is 4 (l/16) + v,
+// the f64 result row is l/16 + 4 v; the column is l % 16.
+// ---------------------------------------------------------------------------------------
+constexpr int Q2M_E = 5;      // elements per workgroup
+constexpr int Q2M_K = 28;     // 27 vertices padded to a multiple of 4
+constexpr int Q2M_C = 384;    // 125 points x 3 derivatives padded to 24 tiles of 16
+constexpr int Q2M_CS = 388;   // LDS row stride (spreads the 16 rows of a column over the banks)
+
+template <typename T> struct Q2Mma;
+template <> struct Q2Mma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+template <> struct Q2Mma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+
+// REG: the wave's slice of D is loaded into registers once and the block loops over element
+// batches (large grids, FP32); otherwise every MFMA reads its D operand through the cache
+// (FP64, whose register budget the slice would exhaust, and small grids)
+template <typename T, bool REG>
+__global__ __launch_bounds__(256) void k_q2_apply_mfma(Q2G G, int cx, int cy, int cz, int ncx, int ncy, long nce,
+                                                       const T* __restrict__ W, const T* __restrict__ Dm,
+                                                       const T* __restrict__ x, T* __restrict__ y,
+                                                       const uint8_t* __restrict__ bc) {
+  using M = Q2Mma<T>;
+  using v4 = typename M::v4;
+  __shared__ T Xs[16][Q2M_K];
+  __shared__ T Gs[16][Q2M_CS];
+  T(*Red)[16][32] = reinterpret_cast<T(*)[16][32]>(&Gs[0][0]);  // reuses Gs once pass 4 has read it
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  auto elem_of = [&](long ce) -> long {
+    const int i = cx + 2 * (int)(ce % ncx), j = cy + 2 * (int)((ce / ncx) % ncy);
+    const int k = cz + 2 * (int)(ce / ((long)ncx * ncy));
+    return i + (long)G.rx * (j + (long)G.ry * k);
+  };
+  // this wave's B operands of both passes, held in registers for every batch the block does
+  auto b1 = [&](int c, int ks) -> T { return Dm[(4 * ks + (l >> 4)) * Q2M_C + (6 * wv + c) * 16 + (l & 15)]; };
+  auto b2 = [&](int ks, int n) -> T { return Dm[(16 * n + (l & 15)) * Q2M_C + 4 * (24 * wv + ks) + (l >> 4)]; };
+  T B1[REG ? 6 : 1][REG ? Q2M_K / 4 : 1], B2[REG ? 24 : 1][2];
+  if constexpr (REG) {
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+#pragma unroll
+      for (int ks = 0; ks < Q2M_K / 4; ks++) B1[c][ks] = b1(c, ks);
+#pragma unroll
+    for (int ks = 0; ks < 24; ks++) {
+      B2[ks][0] = b2(ks, 0);
+      B2[ks][1] = b2(ks, 1);
+    }
+  }
+  const long nbatch = (nce + Q2M_E - 1) / Q2M_E;
+  for (long bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const long ce0 = bt * Q2M_E;
+    // 1. element dofs, rows r = 3 le + i (Dirichlet dofs read as zero; padding zero)
+    for (int idx = t; idx < 16 * Q2M_K; idx += 256) {
+      const int r = idx / Q2M_K, a = idx % Q2M_K, le = r / 3, i = r % 3;
+      T v = (T)0;
+      if (r < 3 * Q2M_E && a < 27 && ce0 + le < nce) {
+        const long n = q2_vertex<3>(G, elem_of(ce0 + le), a);
+        v = ((bc[n] >> i) & 1) ? (T)0 : x[n * 3 + i];
+      }
+      Xs[r][a] = v;
+    }
+    __syncthreads();
+    // 2. G = X D: wave wv computes column tiles 6 wv .. 6 wv + 5
+    if constexpr (REG) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < Q2M_K / 4; ks++) acc = M::mma(Xs[l & 15][4 * ks + (l >> 4)], B1[c][ks], acc);
+#pragma unroll
+        for (int v = 0; v < 4; v++) Gs[M::row(l, v)][(6 * wv + c) * 16 + (l & 15)] = acc[v];
+      }
+    } else {
+#pragma unroll 1
+      for (int c = 0; c < 6; c++) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < Q2M_K / 4; ks++) acc = M::mma(Xs[l & 15][4 * ks + (l >> 4)], b1(c, ks), acc);
+#pragma unroll
+        for (int v = 0; v < 4; v++) Gs[M::row(l, v)][(6 * wv + c) * 16 + (l & 15)] = acc[v];
+      }
+    }
+    __syncthreads();
+    // 3. tau_ij = W_q (g_ij + g_ji) in place
+    for (int idx = t; idx < Q2M_E * 125; idx += 256) {
+      const int le = idx / 125, q = idx % 125;
+      if (ce0 + le >= nce) continue;
+      const T wq = W[elem_of(ce0 + le) * 125 + q];
+      T g[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) g[i][j] = Gs[3 * le + i][q * 3 + j];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Gs[3 * le + i][q * 3 + j] = wq * (g[i][j] + g[j][i]);
+    }
+    __syncthreads();
+    // 4. Y = T D^T over the 384 columns, split into four quarters (one per wave)
+    v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    if constexpr (REG) {
+#pragma unroll
+      for (int ks = 0; ks < 24; ks++) {
+        const T av = Gs[l & 15][4 * (24 * wv + ks) + (l >> 4)];
+        acc0 = M::mma(av, B2[ks][0], acc0);
+        acc1 = M::mma(av, B2[ks][1], acc1);
+      }
+    } else {
+#pragma unroll 2
+      for (int ks = 0; ks < 24; ks++) {
+        const T av = Gs[l & 15][4 * (24 * wv + ks) + (l >> 4)];
+        acc0 = M::mma(av, b2(ks, 0), acc0);
+        acc1 = M::mma(av, b2(ks, 1), acc1);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      Red[wv][M::row(l, v)][l & 15] = acc0[v];
+      Red[wv][M::row(l, v)][16 + (l & 15)] = acc1[v];
+    }
+    __syncthreads();
+    // 5. sum the quarters, scatter to the free vertex dofs
+    for (int idx = t; idx < 16 * 32; idx += 256) {
+      const int r = idx / 32, a = idx % 32, le = r / 3, i = r % 3;
+      if (r >= 3 * Q2M_E || a >= 27 || ce0 + le >= nce) continue;
+      const T sacc = Red[0][r][a] + Red[1][r][a] + Red[2][r][a] + Red[3][r][a];
+      const long n = q2_vertex<3>(G, elem_of(ce0 + le), a);
+      if (!((bc[n] >> i) & 1)) y[n * 3 + i] += sacc;
+    }
+    __syncthreads();  // Xs / Gs are refilled by the next batch
+  }
+}
+
+// D[a][(q,j)] for the matrix-core apply, [32][384] (zero padding), in T
+void q2_mfma_matrix(const Geom& g, std::vector<double>& D) {
+  const Q2G G = q2_pack(g);
+  D.assign((size_t)32 * Q2M_C, 0.0);
+  for (int a = 0; a < 27; a++) {
+    const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+    for (int q = 0; q < 125; q++) {
+      const double X = g5(q % 5), Y = g5((q / 5) % 5), Z = g5(q / 25);
+      D[(size_t)a * Q2M_C + q * 3 + 0] = dph(ax, X) * G.s[0] * ph(ay, Y) * ph(az, Z);
+      D[(size_t)a * Q2M_C + q * 3 + 1] = ph(ax, X) * dph(ay, Y) * G.s[1] * ph(az, Z);
+      D[(size_t)a * Q2M_C + q * 3 + 2] = ph(ax, X) * ph(ay, Y) * dph(az, Z) * G.s[2];
+    }
+  }
+}
+
+static int g_q2_mfma = 1;
+void set_q2_mfma(int on) { g_q2_mfma = on ? 1 : 0; }
+
+// ---------------------------------------------------------------------------------------
 // matrix-free K apply
 // ---------------------------------------------------------------------------------------
 // y = (bc ? x : 0): identity rows on Dirichlet dofs, the element passes add the free rows
@@ -377,7 +544,7 @@ __global__ __launch_bounds__(128) void k_q2_apply(Q2G G, int cx, int cy, int cz,
 }
 
 template <typename T>
-void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s) {
+void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s, const T* Dm) {
   const Q2G G = q2_pack(g);
   if (g.dim == 3)
     hipLaunchKernelGGL((k_q2_init<T, 3>), q2_blocks(G.N, 256), 256, 0, s, G.N, x, bc, y);
@@ -390,15 +557,31 @@ void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t*
     const int ncx = (G.rx - cx + 1) / 2, ncy = (G.ry - cy + 1) / 2, ncz = g.dim == 3 ? (G.rz - cz + 1) / 2 : 1;
     const long nce = (long)ncx * ncy * ncz;
     if (nce == 0) continue;
-    if (g.dim == 3)
+    if (g.dim == 3 && Dm && g_q2_mfma) {
+      const long nbatch = (nce + Q2M_E - 1) / Q2M_E;
+      if (sizeof(T) == 4 && nbatch >= 2L * 2048) {
+        // persistent blocks (2+ batches each) keep their slice of D in registers
+        hipLaunchKernelGGL((k_q2_apply_mfma<T, true>), 2048, 256, 0, s, G, cx, cy, cz, ncx, ncy, nce, W, Dm, x, y, bc);
+      } else if (sizeof(T) == 8) {
+        hipLaunchKernelGGL((k_q2_apply_mfma<T, false>), (unsigned)nbatch, 256, 0, s, G, cx, cy, cz, ncx, ncy, nce, W,
+                           Dm, x, y, bc);
+      } else {  // small FP32 grids: the vector kernel is faster
+        hipLaunchKernelGGL((k_q2_apply<T, 3>), q2_blocks(nce, epb), 128, 0, s, G, cx, cy, cz, ncx, ncy, nce, W, x, y,
+                           bc);
+      }
+    }
+    else if (g.dim == 3)
       hipLaunchKernelGGL((k_q2_apply<T, 3>), q2_blocks(nce, epb), 128, 0, s, G, cx, cy, cz, ncx, ncy, nce, W, x, y, bc);
     else
       hipLaunchKernelGGL((k_q2_apply<T, 2>), q2_blocks(nce, epb), 128, 0, s, G, cx, cy, cz, ncx, ncy, nce, W, x, y, bc);
     UWG_HIP(hipGetLastError());
   }
 }
-template void launch_q2_apply<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t);
-template void launch_q2_apply<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t);
+template void launch_q2_apply<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t,
+                                      const double*);
+template void launch_q2_apply<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t,
+                                     const float*);
+
 
 // ---------------------------------------------------------------------------------------
 // Jacobi diagonal: K[ai][ai] = sum_e sum_q W_q (|grad N_a|^2 + (dN_a/dx_i)^2)

@@ -133,6 +133,8 @@ struct Q2Level {
   double* Sinv = nullptr;   // their inverses
   float* Sinv32 = nullptr;
   double* fe = nullptr;     // [E][3^dim][dim]
+  double* D = nullptr;      // 3D: derivative matrix of the matrix-core apply [32][384]
+  float* D32 = nullptr;
   long n() const { return gn.N * gn.dim; }
 };
 
@@ -1161,6 +1163,7 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
 namespace {
 template <typename T> struct Q2V;
 template <> struct Q2V<double> {
+  static const double* D(Q2Level& A) { return A.D; }
   static const double* W(Q2Level& A) { return A.W; }
   static const double* dinv(Q2Level& A) { return A.dinv; }
   static const double* Sinv(Q2Level& A) { return A.Sinv; }
@@ -1170,6 +1173,7 @@ template <> struct Q2V<double> {
   static double* d(Q2Level& A) { return A.d; }
 };
 template <> struct Q2V<float> {
+  static const float* D(Q2Level& A) { return A.D32; }
   static const float* W(Q2Level& A) { return A.W32; }
   static const float* dinv(Q2Level& A) { return A.dinv32; }
   static const float* Sinv(Q2Level& A) { return A.Sinv32; }
@@ -1184,7 +1188,7 @@ template <> struct Q2V<float> {
 template <typename T>
 static void top_K(uwg_stokes* st, const T* x, T* y, const uint8_t* bc) {
   if (st->q2)
-    launch_q2_apply<T>(st->g, Q2V<T>::W(*st->q2), x, y, bc, 0);
+    launch_q2_apply<T>(st->g, Q2V<T>::W(*st->q2), x, y, bc, 0, Q2V<T>::D(*st->q2));
   else
     launch_apply_K<T>(st->g, LV<T>::W(st->L[0]), x, y, bc, 0);
 }
@@ -1472,6 +1476,15 @@ extern "C" int uwg_stokes_create(uwg_field* vel, uwg_field* p, const unsigned ch
     A.S = dalloc<double>(g.E * NP * NP);
     A.Sinv = dalloc<double>(g.E * NP * NP);
     A.fe = dalloc<double>(g.E * NEN * g.dim);
+    if (g.dim == 3) {
+      std::vector<double> D;
+      q2_mfma_matrix(g, D);
+      std::vector<float> D32(D.begin(), D.end());
+      A.D = dalloc<double>((long)D.size());
+      A.D32 = dalloc<float>((long)D.size());
+      UWG_HIP(hipMemcpy(A.D, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
+      UWG_HIP(hipMemcpy(A.D32, D32.data(), sizeof(float) * D32.size(), hipMemcpyHostToDevice));
+    }
     st->nu = n;
     st->np = g.E * NP;
   } else {
@@ -1510,7 +1523,7 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
     Q2Level& A = *st->q2;
     for (void* p : {(void*)A.bc, (void*)A.W, (void*)A.W32, (void*)A.dinv, (void*)A.dinv32, (void*)A.x, (void*)A.b,
                     (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32, (void*)A.Gr,
-                    (void*)A.S, (void*)A.Sinv, (void*)A.Sinv32, (void*)A.fe})
+                    (void*)A.S, (void*)A.Sinv, (void*)A.Sinv32, (void*)A.fe, (void*)A.D, (void*)A.D32})
       dfree(p);
     delete st->q2;
   }
@@ -2285,6 +2298,11 @@ extern "C" int uwg_stokes_get_rhs(uwg_stokes* st, double* f) {
   UWG_HIP(hipMemcpy(f, st->f, sizeof(double) * st->nu, hipMemcpyDeviceToHost));
   return 0;
   API_END
+}
+
+extern "C" int uwg_set_q2_mfma(int on) {
+  set_q2_mfma(on);
+  return 0;
 }
 
 extern "C" int uwg_set_apply_chunk(int kz3, int kz2) {

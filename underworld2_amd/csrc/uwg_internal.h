@@ -145,7 +145,10 @@ void launch_q2_weights(const Geom& g, const int* cell_ptr, const int* cell_pid, 
                        double* W, double* fe, hipStream_t s);
 void launch_q2_gather_f(const Geom& g, const double* fe, const uint8_t* bc, double* f, hipStream_t s);
 template <typename T>
-void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
+void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s,
+                     const T* Dm = nullptr);  // Dm: [32][384] derivative matrix -> matrix-core path (3D)
+void q2_mfma_matrix(const Geom& g, std::vector<double>& D);
+void set_q2_mfma(int on);
 void launch_q2_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
 template <typename T>
 void launch_q2_apply_G(const Geom& g, const double* Gr, const T* p, const uint8_t* bc, T* y, hipStream_t s);
