@@ -13,6 +13,8 @@ FP64 residuals are reported in ``modes``; ``advection`` holds the RK2 step and D
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 128] [--ppc 30]
                        [--mode M] [--modes fp64,mixed,learned[,schur]] [--no-cpu-baseline]
                        [--no-advection] [--apply-only]
+       python bench.py --q2 --res 64 --ppc 20 [--modes fp64,mixed]
+       python bench.py --convection --res 96 --ppc 25 --steps 40 [--modes fp64,mixed,learned]
 """
 import argparse
 import ctypes
@@ -313,6 +315,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-advection", action="store_true")
     ap.add_argument("--apply-only", action="store_true")
+    ap.add_argument("--q2", action="store_true",
+                    help="Q2/dPc1 3D sinker: K apply (matrix cores / vector) and solves at --res, --ppc")
     ap.add_argument("--convection", action="store_true",
                     help="run the 3D thermal-convection time series (--res, --ppc, --steps, --modes) instead")
     args = ap.parse_args()
@@ -324,6 +328,9 @@ def main():
     log(uw.device_info())
     if args.convection:
         convection_main(args)
+        return
+    if args.q2:
+        q2_main(args)
         return
     res = (args.res,) * 3
     P = build_problem(res, args.ppc)
@@ -369,6 +376,48 @@ def main():
         if plain in modes:
             line[f"speedup_vs_{plain}"] = H["solves_per_sec"] / modes[plain]["solves_per_sec"]
     print(json.dumps(line), flush=True)
+
+
+def q2_main(args):
+    """Q2/dPc1 3D sinker (sphere r 0.1, eta 1e3, particle quadrature, free slip): the Q2 K apply on
+    the matrix cores and on the vector kernel (FP64 and FP32, HIP-event timed), and one coupled
+    solve per precision mode."""
+    import underworld2_amd as uw
+    import underworld2_amd._lib as L
+    fn = uw.function
+    res = (args.res,) * 3
+    mesh = uw.mesh.FeMesh_Cartesian("Q2/dPc1", res, (0.0,) * 3, (1.0,) * 3)
+    vel = uw.mesh.MeshVariable(mesh, 3)
+    p = uw.mesh.MeshVariable(mesh.subMesh, 1)
+    swarm = uw.swarm.Swarm(mesh)
+    swarm.populate_using_layout(uw.swarm.layouts.PerCellRandomLayout(swarm, particlesPerCell=args.ppc, seed=0))
+    mat = swarm.add_variable("int", 1)
+    c = fn.input() - (0.5, 0.5, 0.6)
+    mat.data[:] = fn.branching.conditional([(fn.math.dot(c, c) < 0.1 ** 2, 1), (True, 0)]).evaluate(swarm)
+    walls = tuple(mesh.specialSets[f"Min{n}_VertexSet"] + mesh.specialSets[f"Max{n}_VertexSet"] for n in "IJK")
+    stokes = uw.systems.Stokes(vel, p, fn_viscosity=fn.branching.map(fn_key=mat, mapping={0: 1.0, 1: 1.0e3}),
+                               fn_bodyforce=fn.branching.map(fn_key=mat, mapping={0: 0.0, 1: 1.0}) * (0.0, 0.0, -1.0),
+                               voronoi_swarm=swarm, conditions=[uw.conditions.DirichletCondition(vel, walls)])
+    stokes._prepare_values()
+    E = mesh.elementsGlobal
+    out = {"workload": f"3D sinker {args.res}^3 Q2/dPc1, {args.ppc} ppc, particle quadrature", "apply": {}, "solve": {}}
+    for mf in (1, 0):
+        L.lib().uwg_set_q2_mfma(mf)
+        for prec in (0, 1):
+            ms = ctypes.c_double()
+            L.check(L.lib().uwg_stokes_time_apply(stokes._h, prec, 0, 20, ctypes.byref(ms)), "time_apply")
+            out["apply"][f"{'mfma' if mf else 'vector'}_{'fp32' if prec else 'fp64'}"] = {
+                "ms": ms.value, "TFLOP_per_s": E * 121.5e3 / (ms.value * 1e-3) / 1e12}
+    L.lib().uwg_set_q2_mfma(1)
+    for m in [x.strip() for x in args.modes.split(",") if x.strip() in ("fp64", "mixed")]:
+        s = make_solver({"uw": uw, "stokes": stokes}, m, args.rtol)
+        s.solve()
+        t0 = time.perf_counter()
+        s.solve()
+        st = s.get_stats()
+        out["solve"][m] = {"s": time.perf_counter() - t0, "outer_its": int(st.outer_its),
+                           "rel_residual_fp64": float(np.hypot(st.res_momentum, st.res_continuity) / st.norm_f)}
+    print(json.dumps(out), flush=True)
 
 
 def convection_main(args):
