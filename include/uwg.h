@@ -209,15 +209,21 @@ int uwg_nn_loss_grad(uwg_nn *h, int B, const float *x, const float *t, const flo
 int uwg_nn_set_data(uwg_nn *h, long S, const float *x, const float *t, const float *mask);
 int uwg_nn_train(uwg_nn *h, int steps, int B, float lr, float weight_decay, long seed, float *losses);
 int uwg_nn_eval(uwg_nn *h, int B, float *loss);
-/* kernel-apply tuning knob: z-chunk sizes of the 3D / 2D K apply (a negative kz3 fixes the
- * 3D chunk at -kz3 on every level instead of adapting it to the grid) */
-int uwg_set_apply_chunk(int kz3, int kz2);
+/* kernel-apply tuning knob: workgroups per compute unit over which the 3D K apply balances its
+ * work (default 2, the resident count), element rows per block of the 2D K apply (default 32);
+ * values <= 0 leave a setting unchanged */
+int uwg_set_apply_chunk(int wg_per_cu3, int kz2);
 /* 3D Q2 K apply on the matrix cores (1, default) or the vector kernel (0) */
 int uwg_set_q2_mfma(int on);
+/* element weights of the 3D FP32 multigrid cycle (precision 1): bfloat16 (1, default; the
+ * element contractions take bf16 inputs, FP32 arithmetic and accumulation) or FP32 (0).
+ * Process-wide; takes effect at the next solve. */
+int uwg_set_cycle_bf16(int on);
 /* STREAM-style device copy of n doubles (reads 8n bytes, writes 8n bytes); ms per copy.
  * Measures attainable HBM bandwidth and calibrates byte counters. */
 int uwg_stream_copy(long n, int reps, double *ms);
-/* time `reps` applies of K on a multigrid level in the given precision; ms per apply */
+/* time `reps` applies of K on a multigrid level; ms per apply.  precision 0: FP64; 1: the FP32
+ * cycle's apply (bfloat16 weights on 3D levels when the cycle uses them); 2: FP32 weights */
 int uwg_stokes_time_apply(uwg_stokes *st, int precision, int level, int reps, double *ms);
 
 /* ---- population control: PopulationControl(swarm, deleteThreshold, splitThreshold,

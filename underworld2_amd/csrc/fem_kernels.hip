@@ -387,9 +387,13 @@ void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv
 // ---------------------------------------------------------------------------------------
 template <typename T>
 struct KArgs {
-  int rx, ry, rz, KZ;
+  int rx, ry, rz, KZ;  // KZ: 2D only, element rows per block
   T sx, sy, sz;
   long Ws, nx, nxy;  // Ws: stride of the per-point weight arrays
+  // 3D work split: (ntx x nty) tiles of 16 x 16 element columns, units = tiles x (rz + 1)
+  // owned node levels, dealt as nwg contiguous ranges in tile-major order
+  int ntx, nty, nwg;
+  long units;
 };
 
 template <typename T, typename WF>
@@ -489,23 +493,232 @@ __device__ __forceinline__ void elem3(WF wq, const T (&u)[8][3], T (&r)[8][3], T
   }
 }
 
+// Line-factorised form of elem3 (same operator, about half the arithmetic).  Each gradient
+// component is constant along one axis: du/dx along x, du/dy along y, du/dz along z.  Its
+// values at the 27 points are therefore 9 "lines" per component (indexed by the other two
+// abscissae), and the dual contraction collects each stress component along the same lines
+// before folding it into the dual coefficients.  The z-lines are formed once per element
+// and accumulated over the three planes (27 values + 27 accumulators held), the y-lines once
+// per plane, the x-lines once per row.  Abscissae enter only as signs s in {-1, 0, 1}
+// (X = s g, g = sqrt(3/5)): line values and folds are additions of pre-scaled coefficients,
+// the powers of g and the 1/8 and 2/h factors are applied once to the coefficients and once
+// to the dual coefficients, and the normal-stress factor 2 is folded into the line of the
+// component's own direction.
+template <typename T, typename WF>
+__device__ __forceinline__ void elem3_lines(WF wq, const T (&u)[8][3], T (&r)[8][3], T sx, T sy, T sz) {
+  const T g = (T)GA, g2 = (T)GA2;
+  const T hx = (T)0.125 * sx, hy = (T)0.125 * sy, hz = (T)0.125 * sz;
+  // x-line of component i at (Y = g sy, Z = g sz): (ax + sz axz) + sy (bx + sz bxz)
+  T ax[3], axz[3], bx[3], bxz[3];
+  T ay[3], ayz[3], by[3], byz[3];
+  T GZ[3][3][3];  // [i][gy][gx]
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const T a0 = u[0][i], a1 = u[1][i], a2 = u[2][i], a3 = u[3][i], a4 = u[4][i], a5 = u[5][i], a6 = u[6][i],
+            a7 = u[7][i];
+    const T d01 = a1 - a0, d23 = a3 - a2, d45 = a5 - a4, d67 = a7 - a6;
+    const T s01 = a1 + a0, s23 = a3 + a2, s45 = a5 + a4, s67 = a7 + a6;
+    const T e1 = d23 - d01, e2 = d67 - d45, f1 = d01 + d23, f2 = d45 + d67;
+    const T k1 = s23 - s01, k2 = s67 - s45, m1 = s01 + s23, m2 = s45 + s67;
+    // sum_a of (node sign products) x u_a: u = (c + cX X + cY Y + cZ Z + cXY XY + ...) / 8
+    const T cX = f1 + f2, cXY = e1 + e2, cXZ = f2 - f1, cXYZ = e2 - e1;
+    const T cY = k1 + k2, cYZ = k2 - k1, cZ = m2 - m1;
+    const T fx = (i == 0 ? (T)2 : (T)1) * hx, fy = (i == 1 ? (T)2 : (T)1) * hy, fz = (i == 2 ? (T)2 : (T)1) * hz;
+    ax[i] = fx * cX;
+    axz[i] = (fx * g) * cXZ;
+    bx[i] = (fx * g) * cXY;
+    bxz[i] = (fx * g2) * cXYZ;
+    ay[i] = fy * cY;
+    ayz[i] = (fy * g) * cYZ;
+    by[i] = (fy * g) * cXY;
+    byz[i] = (fy * g2) * cXYZ;
+    const T A = fz * cZ, B = (fz * g) * cXZ, C = (fz * g) * cYZ, D = (fz * g2) * cXYZ;
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      const T a = gy == 0 ? A - C : (gy == 1 ? A : A + C);
+      const T b = gy == 0 ? B - D : (gy == 1 ? B : B + D);
+      GZ[i][gy][0] = a - b;
+      GZ[i][gy][1] = a;
+      GZ[i][gy][2] = a + b;
+    }
+  }
+  T PZ[3][3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++)
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) PZ[i][gy][gx] = (T)0;
+  // x-, y-part dual accumulators (before the h and g factors)
+  T eX[3], eXYx[3], eXZx[3], eXYZx[3], eY[3], eXYy[3], eYZy[3], eXYZy[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    eX[i] = eXYx[i] = eXZx[i] = eXYZx[i] = (T)0;
+    eY[i] = eXYy[i] = eYZy[i] = eXYZy[i] = (T)0;
+  }
+#pragma unroll
+  for (int gz = 0; gz < 3; gz++) {
+    T GY[3][3], PY[3][3];  // [i][gx]
+    T axp[3], bxp[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const T aY = gz == 0 ? ay[i] - ayz[i] : (gz == 1 ? ay[i] : ay[i] + ayz[i]);
+      const T bY = gz == 0 ? by[i] - byz[i] : (gz == 1 ? by[i] : by[i] + byz[i]);
+      GY[i][0] = aY - bY;
+      GY[i][1] = aY;
+      GY[i][2] = aY + bY;
+      PY[i][0] = PY[i][1] = PY[i][2] = (T)0;
+      axp[i] = gz == 0 ? ax[i] - axz[i] : (gz == 1 ? ax[i] : ax[i] + axz[i]);
+      bxp[i] = gz == 0 ? bx[i] - bxz[i] : (gz == 1 ? bx[i] : bx[i] + bxz[i]);
+    }
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      T GX[3], PX[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        GX[i] = gy == 0 ? axp[i] - bxp[i] : (gy == 1 ? axp[i] : axp[i] + bxp[i]);
+        PX[i] = (T)0;
+      }
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) {
+        const T w = wq(gx + 3 * gy + 9 * gz);
+        const T t01 = GY[0][gx] + GX[1];
+        const T t02 = GZ[0][gy][gx] + GX[2];
+        const T t12 = GZ[1][gy][gx] + GY[2][gx];
+        PX[0] += w * GX[0];
+        PX[1] += w * t01;
+        PX[2] += w * t02;
+        PY[0][gx] += w * t01;
+        PY[1][gx] += w * GY[1][gx];
+        PY[2][gx] += w * t12;
+        PZ[0][gy][gx] += w * t02;
+        PZ[1][gy][gx] += w * t12;
+        PZ[2][gy][gx] += w * GZ[2][gy][gx];
+      }
+      // fold the x-line (gy, gz): weights 1, Y, Z, YZ
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        eX[i] += PX[i];
+        if (gy == 0) eXYx[i] -= PX[i];
+        if (gy == 2) eXYx[i] += PX[i];
+        if (gz == 0) eXZx[i] -= PX[i];
+        if (gz == 2) eXZx[i] += PX[i];
+        if (gy != 1 && gz != 1) {
+          if (gy == gz)
+            eXYZx[i] += PX[i];
+          else
+            eXYZx[i] -= PX[i];
+        }
+      }
+    }
+    // fold the y-lines of plane gz: weights 1, X, Z, XZ
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const T s = PY[i][0] + PY[i][1] + PY[i][2], dd = PY[i][2] - PY[i][0];
+      eY[i] += s;
+      eXYy[i] += dd;
+      if (gz == 0) {
+        eYZy[i] -= s;
+        eXYZy[i] -= dd;
+      }
+      if (gz == 2) {
+        eYZy[i] += s;
+        eXYZy[i] += dd;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    // fold the z-lines: weights 1, X, Y, XY
+    T q0[3], q1[3];
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      q0[gy] = PZ[i][gy][0] + PZ[i][gy][1] + PZ[i][gy][2];
+      q1[gy] = PZ[i][gy][2] - PZ[i][gy][0];
+    }
+    const T eZ = q0[0] + q0[1] + q0[2], eYZz = q0[2] - q0[0];
+    const T eXZz = q1[0] + q1[1] + q1[2], eXYZz = q1[2] - q1[0];
+    const T dX = hx * eX[i], dY = hy * eY[i], dZ = hz * eZ;
+    const T dXY = g * (hx * eXYx[i] + hy * eXYy[i]);
+    const T dXZ = g * (hx * eXZx[i] + hz * eXZz);
+    const T dYZ = g * (hy * eYZy[i] + hz * eYZz);
+    const T dXYZ = g2 * (hx * eXYZx[i] + hy * eXYZy[i] + hz * eXYZz);
+    // r_a = s dX + t dY + r dZ + st dXY + sr dXZ + tr dYZ + str dXYZ, a = (s, t, r) bits
+    const T pp = dX + dXY, pm = dX - dXY;  // s-coefficient of F for t = +1 / -1
+    const T F0 = -pm - dY, F1 = pm - dY, F2 = dY - pp, F3 = pp + dY;  // F(s,t), a = s + 2t bits
+    const T hp = dZ + dYZ, hm = dZ - dYZ, kp = dXZ + dXYZ, km = dXZ - dXYZ;
+    const T H0 = hm - km, H1 = hm + km, H2 = hp - kp, H3 = hp + kp;  // H(s,t)
+    r[0][i] = F0 - H0;
+    r[1][i] = F1 - H1;
+    r[2][i] = F2 - H2;
+    r[3][i] = F3 - H3;
+    r[4][i] = F0 + H0;
+    r[5][i] = F1 + H1;
+    r[6][i] = F2 + H2;
+    r[7][i] = F3 + H3;
+  }
+}
+
+// Loads and stores addressed as a uniform base plus a 32-bit byte offset: the address is then
+// formed by the memory instruction (scalar base + vector offset) instead of by 64-bit vector
+// arithmetic per access.  The launcher checks that the arrays of the 3D apply (the weights of
+// all 27 planes included) stay below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* __restrict__ base, uint32_t boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T* __restrict__ base, uint32_t boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// The element weights of the FP32 apply are read through a buffer resource: the plane offset
+// q * Ws * 4 is a scalar operand of the load and the element offset a 32-bit vector operand,
+// so the weight loads of a level need no vector address arithmetic.  Descriptor word 3 = 0x00020000
+// (32-bit data format, the raw-buffer setting of the gfx9 family); num_records bounds the
+// buffer in bytes (loads past it return 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t weight_rsrc(const float* W, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(W), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t ld_buf_u32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+
+// Node values of one level of an element column tile (corners c = 0..3 of element (ei, ej)).
+// Every load is issued unconditionally (out-of-range corners read node 0 and are zeroed when the
+// level is finished), so no load waits on another: the mask byte does not gate the value loads.
+template <typename T>
+struct LevelLoad {
+  T v[4][3];
+  uint8_t m[4];
+  bool ok[4];
+  __device__ __forceinline__ void issue(const KArgs<T>& A, const T* __restrict__ x, const uint8_t* __restrict__ bc,
+                                        int ei, int ej, int lev) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int ni = ei + (c & 1), nj = ej + (c >> 1);
+      ok[c] = ni >= 0 && ni <= A.rx && nj >= 0 && nj <= A.ry && lev >= 0 && lev <= A.rz;
+      const uint32_t n = ok[c] ? (uint32_t)(ni + A.nx * nj + A.nxy * lev) : 0u;
+      m[c] = ld_off(bc, n);
+#pragma unroll
+      for (int i = 0; i < 3; i++) v[c][i] = ld_off(x, (n * 3u + i) * (uint32_t)sizeof(T));
+    }
+  }
+  // u[off + c] = node values with Dirichlet components and absent nodes zeroed
+  __device__ __forceinline__ void finish(T (&u)[8][3], int off) const {
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) u[off + c][i] = (ok[c] && !((m[c] >> i) & 1)) ? v[c][i] : (T)0;
+  }
+};
+
 template <typename T>
 __device__ __forceinline__ void load_level3(const KArgs<T>& A, const T* __restrict__ x, const uint8_t* __restrict__ bc,
                                             int ei, int ej, int lev, T (&u)[8][3], int off) {
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    int ni = ei + (c & 1), nj = ej + (c >> 1);
-    bool ok = ni >= 0 && ni <= A.rx && nj >= 0 && nj <= A.ry && lev >= 0 && lev <= A.rz;
-    if (ok) {
-      long n = ni + A.nx * nj + A.nxy * lev;
-      uint8_t m = bc[n];
-#pragma unroll
-      for (int i = 0; i < 3; i++) u[off + c][i] = ((m >> i) & 1) ? (T)0 : x[n * 3 + i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < 3; i++) u[off + c][i] = (T)0;
-    }
-  }
+  LevelLoad<T> L;
+  L.issue(A, x, bc, ei, ej, lev);
+  L.finish(u, off);
 }
 
 // What the apply kernel writes for an owned node (the operator value is kx = (K x)_n, or x_n on
@@ -541,73 +754,254 @@ __device__ __forceinline__ void apply_epilogue(const SmoothArgs<T>& S, const T* 
   }
 }
 
-// Occupancy: the FP64 instance is held to 256 VGPRs (two waves per SIMD; the cap costs a few
-// bytes of spill), the FP32 instance needs no cap (153 VGPRs, three waves per SIMD).
+// Operands of the owned node's update, loaded at the top of a level so that their latency
+// overlaps the element work.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void k_apply3(KArgs<T> A, const T* __restrict__ W,
-                                                                     const T* __restrict__ x, T* __restrict__ y,
-                                                                     const uint8_t* __restrict__ bc, SmoothArgs<T> S) {
+struct NodeOperands {
+  uint8_t m;
+  T x[3], b[3], dinv[3], d[3];
+  __device__ __forceinline__ void load(const SmoothArgs<T>& S, const T* __restrict__ xs, const uint8_t* __restrict__ bc,
+                                       uint32_t n) {
+    m = ld_off(bc, n);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const uint32_t o = (n * 3u + i) * (uint32_t)sizeof(T);
+      x[i] = ld_off(xs, o);
+      if (MODE != APPLY_Y) b[i] = ld_off(S.b, o);
+      if (MODE == APPLY_CHEB || MODE == APPLY_CHEB0) dinv[i] = ld_off(S.dinv, o);
+      if (MODE == APPLY_CHEB) d[i] = ld_off((const T*)S.d, o);
+    }
+  }
+  // kx: the operator row sum of component i (replaced by x on a Dirichlet row)
+  __device__ __forceinline__ void store(const SmoothArgs<T>& S, T* __restrict__ y, uint32_t n, int i, T kx) const {
+    const uint32_t o = (n * 3u + i) * (uint32_t)sizeof(T);
+    if ((m >> i) & 1) kx = x[i];
+    if (MODE == APPLY_Y) {
+      st_off(y, o, kx);
+    } else {
+      const T r = b[i] - kx;
+      if (MODE == APPLY_RESIDUAL) {
+        st_off(y, o, r);
+      } else {
+        const T v = MODE == APPLY_CHEB ? S.a * d[i] + S.c * (dinv[i] * r) : S.c * (dinv[i] * r);
+        st_off(S.d, o, v);
+        st_off(y, o, x[i] + v);
+      }
+    }
+  }
+};
+
+// Workgroups are dealt to the 8 XCDs round-robin by linear id; renumber them so that each XCD
+// receives a contiguous run of logical ids.  Consecutive ranges of work (the z-segments of one
+// tile, then the neighbouring tiles) then share the XCD's L2, where the cache lines straddling
+// neighbouring tiles' common element column/row are reused.
+__device__ __forceinline__ int xcd_logical_id() {
+  const int total = gridDim.x;
+  const int id = blockIdx.x;
+  const int xcd = id & 7, loc = id >> 3, per = total >> 3, rem = total & 7;
+  return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+// Runs body(bx, by, k0, k1) over the workgroup's range of work units: units are owned node
+// levels k of tile (bx, by), tile-major; a range that crosses a tile boundary is streamed as two
+// segments.  Every workgroup receives units / nwg (+1) levels, so a launch of nwg = the number
+// of resident workgroups finishes in one balanced round.
+template <typename T, typename Body>
+__device__ __forceinline__ void for_each_segment(const KArgs<T>& A, Body body) {
+  const long wid = xcd_logical_id();
+  long w0 = wid * A.units / A.nwg;
+  const long w1 = (wid + 1) * A.units / A.nwg;
+  const long L = A.rz + 1;
+  while (w0 < w1) {
+    const long tile = w0 / L;
+    const int k0 = (int)(w0 - tile * L);
+    const int k1 = (int)min(L, k0 + (w1 - w0));
+    body((int)(tile % A.ntx), (int)(tile / A.ntx), k0, k1);
+    w0 += k1 - k0;
+  }
+}
+
+// Sum of the four element-column contributions (this level's below / above halves) at the
+// owned node of thread tid, component i.
+template <typename T>
+__device__ __forceinline__ T node_sum(const T (*sb)[256], const T (*st)[256], int tid, int i) {
+  return st[9 + i][tid - 17] + st[6 + i][tid - 16] + st[3 + i][tid - 1] + st[i][tid] + sb[9 + i][tid - 17] +
+         sb[6 + i][tid - 16] + sb[3 + i][tid - 1] + sb[i][tid];
+}
+
+// FP64 instance: held to 256 VGPRs (two waves per SIMD).  The element weights are read as the
+// element kernel consumes them; the node levels and the owned node's operands are issued at the
+// top of each level.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256, 2) void k_apply3(KArgs<T> A, const T* __restrict__ W, const T* __restrict__ x,
+                                                   T* __restrict__ y, const uint8_t* __restrict__ bc,
+                                                   SmoothArgs<T> S) {
   __shared__ T sb[12][256];
   __shared__ T st[12][256];
   const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
-  // Workgroups are dealt to the 8 XCDs round-robin by linear id; renumber them so that each
-  // XCD receives a contiguous run of tiles (x fastest).  Neighbouring tiles then share the
-  // XCD's L2, where the cache lines straddling their common element column/row are reused.
-  int bx, by, bz;
-  {
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int total = gx * gy * gridDim.z;
-    const int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const int xcd = id & 7, loc = id >> 3, per = total >> 3, rem = total & 7;
-    const int t = xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
-    bx = t % gx;
-    by = (t / gx) % gy;
-    bz = t / (gx * gy);
-  }
-  const int ei = bx * 15 - 1 + lx, ej = by * 15 - 1 + ly;
-  const int k0 = bz * A.KZ;
-  const int k1 = min(k0 + A.KZ, A.rz + 1);
-  const bool colE = ei >= 0 && ei < A.rx && ej >= 0 && ej < A.ry;
-  const bool ownN = lx >= 1 && ly >= 1 && ei <= A.rx && ej <= A.ry;
-  T u[8][3];
-  load_level3(A, x, bc, ei, ej, k0 - 1, u, 0);
-  for (int ek = k0 - 1; ek < k1; ++ek) {
-    load_level3(A, x, bc, ei, ej, ek + 1, u, 4);
-    T r[8][3];
-    if (colE && ek >= 0 && ek < A.rz) {
-      const long e = ei + (long)A.rx * (ej + (long)A.ry * ek);
-      elem3<T>([&](int q) { return W[(long)q * A.Ws + e]; }, u, r, A.sx, A.sy, A.sz);
-    } else {
+  for_each_segment(A, [&](int bx, int by, int k0, int k1) {
+    const int ei = bx * 15 - 1 + lx, ej = by * 15 - 1 + ly;
+    const bool colE = ei >= 0 && ei < A.rx && ej >= 0 && ej < A.ry;
+    const bool ownN = lx >= 1 && ly >= 1 && ei <= A.rx && ej <= A.ry;
+    T u[8][3];
+    load_level3(A, x, bc, ei, ej, k0 - 1, u, 0);
+    for (int ek = k0 - 1; ek < k1; ++ek) {
+      load_level3(A, x, bc, ei, ej, ek + 1, u, 4);
+      const bool doN = ownN && ek >= k0;
+      const uint32_t nN = doN ? (uint32_t)(ei + A.nx * ej + A.nxy * ek) : 0u;
+      NodeOperands<T, MODE> nd;
+      nd.load(S, x, bc, nN);
+      T r[8][3];
+      if (colE && ek >= 0 && ek < A.rz) {
+        const uint32_t eo = (uint32_t)(ei + A.rx * (ej + A.ry * ek)) * (uint32_t)sizeof(T);
+        elem3<T>([&](int q) { return ld_off(W + (long)q * A.Ws, eo); }, u, r, A.sx, A.sy, A.sz);
+      } else {
 #pragma unroll
-      for (int a = 0; a < 8; a++)
+        for (int a = 0; a < 8; a++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) r[a][i] = (T)0;
-    }
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) sb[c * 3 + i][tid] = r[c][i];
-    __syncthreads();
-    if (ownN && ek >= k0) {
-      long n = ei + A.nx * ej + A.nxy * ek;
-      uint8_t m = bc[n];
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        T s = st[9 + i][tid - 17] + st[6 + i][tid - 16] + st[3 + i][tid - 1] + st[i][tid] + sb[9 + i][tid - 17] +
-              sb[6 + i][tid - 16] + sb[3 + i][tid - 1] + sb[i][tid];
-        apply_epilogue<T, MODE>(S, x, y, n * 3 + i, ((m >> i) & 1) ? x[n * 3 + i] : s);
+          for (int i = 0; i < 3; i++) r[a][i] = (T)0;
       }
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) sb[c * 3 + i][tid] = r[c][i];
+      __syncthreads();
+      if (doN) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) nd.store(S, y, nN, i, node_sum<T>(sb, st, tid, i));
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) st[c * 3 + i][tid] = r[4 + c][i];
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) u[c][i] = u[4 + c][i];
     }
-    __syncthreads();
+  });
+}
+
+// FP32 instance, software-pipelined: while level ek is computed, the loads of the next level
+// are in flight -- node level ek + 2 and the weights of element level ek + 1, held in
+// registers until the top of the next level, where the weights move to this lane's own LDS
+// column (read back as the line-factorised element kernel consumes them; no barrier is needed,
+// each lane reads only what it wrote).  Memory and arithmetic of a workgroup then overlap
+// instead of alternating.  Two waves per SIMD (up to 256 VGPRs).
+// Weight formats: BF = false reads FP32 weights [27][Ws]; BF = true reads bfloat16 weights
+// packed in pairs, word p of element e = bf16(W_2p) | bf16(W_2p+1) << 16 in [14][Ws] (the high
+// half of word 13 is zero): 14 loads and half the bytes per element, the arithmetic stays FP32
+// (a bf16 value widens exactly to FP32 by a 16-bit shift).
+template <int MODE, bool BF>
+__global__ __launch_bounds__(256, 2) void k_apply3f(KArgs<float> A, const void* __restrict__ W,
+                                                    const float* __restrict__ x, float* __restrict__ y,
+                                                    const uint8_t* __restrict__ bc, SmoothArgs<float> S) {
+  constexpr int NW = BF ? 14 : 27;  // 32-bit weight words per element
+  __shared__ float sb[12][256];
+  __shared__ float st[12][256];
+  __shared__ uint32_t Wl[NW][256];
+  const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
+  const __amdgpu_buffer_rsrc_t wr = weight_rsrc((const float*)W, (uint32_t)(NW * A.Ws * 4));
+  const uint32_t plane = (uint32_t)(A.Ws * 4);
+  for_each_segment(A, [&](int bx, int by, int k0, int k1) {
+    const int ei = bx * 15 - 1 + lx, ej = by * 15 - 1 + ly;
+    const bool colE = ei >= 0 && ei < A.rx && ej >= 0 && ej < A.ry;
+    const bool ownN = lx >= 1 && ly >= 1 && ei <= A.rx && ej <= A.ry;
+    // byte offset of element level k in a weight plane, or 0 (a valid address) outside the
+    // column's elements
+    const auto elem_off = [&](int k) -> uint32_t {
+      return (colE && k >= 0 && k < A.rz) ? (uint32_t)(ei + A.rx * (ej + A.ry * k)) * 4u : 0u;
+    };
+    float u[8][3];
+    load_level3(A, x, bc, ei, ej, k0 - 1, u, 0);
+    LevelLoad<float> xn;
+    xn.issue(A, x, bc, ei, ej, k0);
+    uint32_t wn[NW];
+    {
+      const uint32_t eo = elem_off(k0 - 1);
 #pragma unroll
-    for (int c = 0; c < 4; c++)
+      for (int q = 0; q < NW; q++) wn[q] = ld_buf_u32(wr, eo, q * plane);
+    }
+    for (int ek = k0 - 1; ek < k1; ++ek) {
+      xn.finish(u, 4);
 #pragma unroll
-      for (int i = 0; i < 3; i++) st[c * 3 + i][tid] = r[4 + c][i];
+      for (int q = 0; q < NW; q++) Wl[q][tid] = wn[q];
+      // the owned node's operands first: the memory counter is waited on in issue order, so the
+      // epilogue's wait for them does not also wait for the prefetch issued after them
+      const bool doN = ownN && ek >= k0;
+      const uint32_t nN = doN ? (uint32_t)(ei + A.nx * ej + A.nxy * ek) : 0u;
+      NodeOperands<float, MODE> nd;
+      nd.load(S, x, bc, nN);
+      // prefetch for level ek + 1
+      xn.issue(A, x, bc, ei, ej, ek + 2);
+      {
+        const uint32_t eo = elem_off(ek + 1);
 #pragma unroll
-    for (int c = 0; c < 4; c++)
+        for (int q = 0; q < NW; q++) wn[q] = ld_buf_u32(wr, eo, q * plane);
+      }
+      float r[8][3];
+      if (colE && ek >= 0 && ek < A.rz) {
+        elem3_lines<float>(
+            [&](int q) {
+              if constexpr (BF) {
+                const uint32_t v = Wl[q >> 1][tid];
+                return __uint_as_float((q & 1) ? (v & 0xffff0000u) : (v << 16));
+              } else {
+                return __uint_as_float(Wl[q][tid]);
+              }
+            },
+            u, r, A.sx, A.sy, A.sz);
+      } else {
 #pragma unroll
-      for (int i = 0; i < 3; i++) u[c][i] = u[4 + c][i];
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) r[a][i] = 0.0f;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) sb[c * 3 + i][tid] = r[c][i];
+      __syncthreads();
+      if (doN) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) nd.store(S, y, nN, i, node_sum<float>(sb, st, tid, i));
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) st[c * 3 + i][tid] = r[4 + c][i];
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) u[c][i] = u[4 + c][i];
+    }
+  });
+}
+
+// bfloat16 rounding (round to nearest, ties to even) of a finite float
+__device__ __forceinline__ uint32_t bf16_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+// FP64 weights [27][Es] -> bfloat16 pairs [14][Es] (via FP32: the FP64 -> FP32 -> bf16 double
+// rounding stays within one bf16 half-unit plus 2^-24 relative)
+__global__ void k_pack_w3_bf16(long Es, const double* __restrict__ W, uint32_t* __restrict__ Wb) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= Es) return;
+#pragma unroll
+  for (int p = 0; p < 14; p++) {
+    const uint32_t lo = bf16_rne((float)W[(2 * p) * Es + e]);
+    const uint32_t hi = 2 * p + 1 < 27 ? bf16_rne((float)W[(2 * p + 1) * Es + e]) : 0u;
+    Wb[p * Es + e] = lo | (hi << 16);
   }
+}
+
+void launch_pack_w3_bf16(long Es, const double* W, uint32_t* Wb, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_w3_bf16, nblocks(Es, 256), 256, 0, s, Es, W, Wb);
+  UWG_HIP(hipGetLastError());
 }
 
 // 2D: 64 element columns per block stream along y
@@ -722,23 +1116,30 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
   }
 }
 
-static int g_kz3 = 8, g_kz2 = 32;
-static bool g_kz3_fixed = false;
-void set_apply_chunk(int kz3, int kz2) {
-  if (kz3 > 0) {
-    g_kz3 = kz3;
-    g_kz3_fixed = false;
-  } else if (kz3 < 0) {
-    g_kz3 = -kz3;
-    g_kz3_fixed = true;
-  }
+static int g_wg_per_cu3 = 2, g_kz2 = 32;
+void set_apply_chunk(int wg_per_cu3, int kz2) {
+  if (wg_per_cu3 > 0) g_wg_per_cu3 = wg_per_cu3;
   if (kz2 > 0) g_kz2 = kz2;
+}
+
+// compute units of the current device (cached per device)
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  UWG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    UWG_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    cus[dev] = n > 0 ? n : 256;
+  }
+  return cus[dev];
 }
 
 template <typename T, int MODE>
 static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, const SmoothArgs<T>& S,
-                              hipStream_t s) {
-  KArgs<T> A;
+                              hipStream_t s, const uint32_t* Wbf) {
+  KArgs<T> A{};
   A.rx = g.res[0];
   A.ry = g.res[1];
   A.rz = g.res[2];
@@ -749,19 +1150,22 @@ static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const
   A.nx = g.res[0] + 1;
   A.nxy = A.nx * (g.res[1] + 1);
   if (g.dim == 3) {
-    // z-chunk: one plane per block on the small multigrid levels (latency-bound, more blocks
-    // win), longer chunks where the grid already fills the GPU
-    const long tiles = (long)((g.res[0] + 1 + 14) / 15) * ((g.res[1] + 1 + 14) / 15);
-    // measured at 128^3: 16-plane chunks suit the FP32 instance, 8-plane the FP64 one
-    A.KZ = (g_kz3_fixed || sizeof(T) == 8) ? g_kz3 : 2 * g_kz3;
-    if (!g_kz3_fixed) {
-      if (g.res[2] <= 32)
-        A.KZ = 1;
+    // one balanced round: as many workgroups as are resident at once (two per CU for both
+    // instances), each streaming an equal share of the tiles' node levels
+    A.KZ = 0;
+    if (27.0 * g.Es * sizeof(T) >= 4294967296.0 || 3.0 * g.N * sizeof(T) >= 4294967296.0)
+      throw std::runtime_error("3D K apply: weight planes and node vectors must stay below 4 GiB");
+    A.ntx = (g.res[0] + 1 + 14) / 15;
+    A.nty = (g.res[1] + 1 + 14) / 15;
+    A.units = (long)A.ntx * A.nty * (g.res[2] + 1);
+    A.nwg = (int)std::min<long>(A.units, (long)device_cus() * g_wg_per_cu3);
+    if constexpr (sizeof(T) == 4) {
+      if (Wbf)
+        hipLaunchKernelGGL((k_apply3f<MODE, true>), dim3(A.nwg), 256, 0, s, A, (const void*)Wbf, x, y, bc, S);
       else
-        while (A.KZ > 2 && tiles * ((g.res[2] + A.KZ) / A.KZ) < 400) A.KZ /= 2;
-    }
-    dim3 grid((g.res[0] + 1 + 14) / 15, (g.res[1] + 1 + 14) / 15, (g.res[2] + 1 + A.KZ - 1) / A.KZ);
-    hipLaunchKernelGGL((k_apply3<T, MODE>), grid, 256, 0, s, A, W, x, y, bc, S);
+        hipLaunchKernelGGL((k_apply3f<MODE, false>), dim3(A.nwg), 256, 0, s, A, (const void*)W, x, y, bc, S);
+    } else
+      hipLaunchKernelGGL((k_apply3<T, MODE>), dim3(A.nwg), 256, 0, s, A, W, x, y, bc, S);
   } else {
     A.KZ = g_kz2;
     dim3 grid((g.res[0] + 1 + 62) / 63, (g.res[1] + 1 + A.KZ - 1) / A.KZ);
@@ -771,28 +1175,32 @@ static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const
 }
 
 template <typename T>
-void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s) {
-  launch_apply_mode<T, APPLY_Y>(g, W, x, y, bc, SmoothArgs<T>(), s);
+void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s,
+                    const uint32_t* Wbf) {
+  launch_apply_mode<T, APPLY_Y>(g, W, x, y, bc, SmoothArgs<T>(), s, Wbf);
 }
-template void launch_apply_K<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t);
-template void launch_apply_K<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t);
+template void launch_apply_K<double>(const Geom&, const double*, const double*, double*, const uint8_t*, hipStream_t,
+                                     const uint32_t*);
+template void launch_apply_K<float>(const Geom&, const float*, const float*, float*, const uint8_t*, hipStream_t,
+                                    const uint32_t*);
 
 // r = b - K x
 template <typename T>
-void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s) {
+void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s,
+                       const uint32_t* Wbf) {
   SmoothArgs<T> S;
   S.b = b;
-  launch_apply_mode<T, APPLY_RESIDUAL>(g, W, x, r, bc, S, s);
+  launch_apply_mode<T, APPLY_RESIDUAL>(g, W, x, r, bc, S, s, Wbf);
 }
 template void launch_residual_K<double>(const Geom&, const double*, const double*, const double*, double*,
-                                        const uint8_t*, hipStream_t);
+                                        const uint8_t*, hipStream_t, const uint32_t*);
 template void launch_residual_K<float>(const Geom&, const float*, const float*, const float*, float*, const uint8_t*,
-                                       hipStream_t);
+                                       hipStream_t, const uint32_t*);
 
 // one Chebyshev step: r = b - K x; d = a d + c dinv r (a d dropped when first); xn = x + d
 template <typename T>
 void launch_cheb_step_K(const Geom& g, const T* W, const T* x, T* xn, const T* b, const T* dinv, T* d, T a, T c,
-                        bool first, const uint8_t* bc, hipStream_t s) {
+                        bool first, const uint8_t* bc, hipStream_t s, const uint32_t* Wbf) {
   SmoothArgs<T> S;
   S.b = b;
   S.dinv = dinv;
@@ -800,14 +1208,15 @@ void launch_cheb_step_K(const Geom& g, const T* W, const T* x, T* xn, const T* b
   S.a = a;
   S.c = c;
   if (first)
-    launch_apply_mode<T, APPLY_CHEB0>(g, W, x, xn, bc, S, s);
+    launch_apply_mode<T, APPLY_CHEB0>(g, W, x, xn, bc, S, s, Wbf);
   else
-    launch_apply_mode<T, APPLY_CHEB>(g, W, x, xn, bc, S, s);
+    launch_apply_mode<T, APPLY_CHEB>(g, W, x, xn, bc, S, s, Wbf);
 }
 template void launch_cheb_step_K<double>(const Geom&, const double*, const double*, double*, const double*,
-                                         const double*, double*, double, double, bool, const uint8_t*, hipStream_t);
+                                         const double*, double*, double, double, bool, const uint8_t*, hipStream_t,
+                                         const uint32_t*);
 template void launch_cheb_step_K<float>(const Geom&, const float*, const float*, float*, const float*, const float*,
-                                        float*, float, float, bool, const uint8_t*, hipStream_t);
+                                        float*, float, float, bool, const uint8_t*, hipStream_t, const uint32_t*);
 
 // ---------------------------------------------------------------------------------------
 // G p (node gather) and G^T u (element gather); G_e constant on a regular mesh

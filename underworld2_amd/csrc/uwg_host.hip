@@ -101,6 +101,7 @@ struct Level {
   Geom g;
   double* W = nullptr;
   float* W32 = nullptr;
+  uint32_t* Wbf = nullptr;  // 3D FP32 cycle: the weights as bfloat16 pairs [14][Es] (when enabled)
   uint8_t* bc = nullptr;
   double* dinv = nullptr;
   float* dinv32 = nullptr;
@@ -152,6 +153,7 @@ struct uwg_stokes {
   bool hierarchy_built = false;
   bool assembled = false;
   bool fp32_ready = false;
+  bool fp32_bf16 = false;  // the FP32 data were built with bfloat16 cycle weights
   double Ge[24];
   // assembly products (level 0)
   double* fe = nullptr;       // [E][NN*dim]
@@ -747,8 +749,8 @@ static void build_levels(uwg_stokes* st, int levels_req) {
   double* W0 = st->L.empty() ? nullptr : st->L[0].W;
   for (size_t l = 0; l < st->L.size(); l++) {
     Level& A = st->L[l];
-    for (void* p : {(void*)A.W, (void*)A.W32, (void*)A.bc, (void*)A.dinv, (void*)A.dinv32, (void*)A.x, (void*)A.b,
-                    (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
+    for (void* p : {(void*)A.W, (void*)A.W32, (void*)A.Wbf, (void*)A.bc, (void*)A.dinv, (void*)A.dinv32, (void*)A.x,
+                    (void*)A.b, (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
       if (p != (void*)st->bc0 && p != (void*)W0) dfree(p);
   }
   st->L.clear();
@@ -796,8 +798,12 @@ static void copy_W_to_host(const Level& A, double* out) {
                       hipMemcpyDeviceToHost));
 }
 
+// Whether the 3D FP32 multigrid cycle reads its element weights as bfloat16 (the element
+// contractions then take bf16 inputs with FP32 arithmetic and accumulation).
+static bool g_cycle_bf16 = true;
+
 static void ensure_fp32(uwg_stokes* st) {
-  if (st->fp32_ready) return;
+  if (st->fp32_ready && st->fp32_bf16 == g_cycle_bf16) return;
   for (auto& A : st->L) {
     const long Q = A.g.dim == 3 ? 27 : 9;
     if (!A.W32) {
@@ -810,7 +816,15 @@ static void ensure_fp32(uwg_stokes* st) {
     }
     vk_d2f(Q * A.g.Es, 1.0, A.W, A.W32, 0);
     vk_d2f(A.n(), 1.0, A.dinv, A.dinv32, 0);
+    if (A.g.dim == 3 && g_cycle_bf16) {
+      if (!A.Wbf) A.Wbf = dalloc<uint32_t>(14 * A.g.Es);
+      launch_pack_w3_bf16(A.g.Es, A.W, A.Wbf, 0);
+    } else if (A.Wbf) {
+      dfree(A.Wbf);
+      A.Wbf = nullptr;
+    }
   }
+  st->fp32_bf16 = g_cycle_bf16;
   if (st->q2) {
     Q2Level& A = *st->q2;
     const long nq = st->g.E * (st->g.dim == 3 ? 125 : 25), n = A.n(), ns = st->np * (st->g.dim + 1);
@@ -1023,6 +1037,7 @@ namespace {
 template <typename T> struct LV;
 template <> struct LV<double> {
   static double* W(Level& A) { return A.W; }
+  static const uint32_t* Wbf(Level&) { return nullptr; }
   static double* dinv(Level& A) { return A.dinv; }
   static double* x(Level& A) { return A.x; }
   static double* b(Level& A) { return A.b; }
@@ -1031,6 +1046,7 @@ template <> struct LV<double> {
 };
 template <> struct LV<float> {
   static float* W(Level& A) { return A.W32; }
+  static const uint32_t* Wbf(Level& A) { return A.Wbf; }
   static float* dinv(Level& A) { return A.dinv32; }
   static float* x(Level& A) { return A.x32; }
   static float* b(Level& A) { return A.b32; }
@@ -1066,11 +1082,12 @@ static void chebyshev(Level& A, const T* b, T* x, int degree, bool zero_guess) {
   for (int k = 0; k < steps; k++) {
     T* nxt = (cur == x) ? tmp : x;
     if (!zero_guess && k == 0) {
-      launch_cheb_step_K<T>(A.g, LV<T>::W(A), cur, nxt, b, dinv, d, (T)0, (T)(1.0 / theta), true, A.bc, 0);
+      launch_cheb_step_K<T>(A.g, LV<T>::W(A), cur, nxt, b, dinv, d, (T)0, (T)(1.0 / theta), true, A.bc, 0,
+                            LV<T>::Wbf(A));
     } else {
       const double rho_new = 1.0 / (2.0 * sigma - rho);
       launch_cheb_step_K<T>(A.g, LV<T>::W(A), cur, nxt, b, dinv, d, (T)(rho_new * rho), (T)(2.0 * rho_new / delta),
-                            false, A.bc, 0);
+                            false, A.bc, 0, LV<T>::Wbf(A));
       rho = rho_new;
     }
     cur = nxt;
@@ -1118,7 +1135,7 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   Level& C = st->L[l + 1];
   chebyshev<T>(A, b, x, g_smooth_degree, true);
   T* r = LV<T>::r(A);
-  launch_residual_K<T>(A.g, LV<T>::W(A), x, b, r, A.bc, 0);
+  launch_residual_K<T>(A.g, LV<T>::W(A), x, b, r, A.bc, 0, LV<T>::Wbf(A));
   launch_restrict2<T>(A.g, C.g, r, LV<T>::b(C), C.bc, A.bc, 0);
   const bool capture = st->capture_level == l + 1 && st->captured < st->capture_max &&
                        (st->capture_seen++ % st->capture_every) == 0;
@@ -1131,7 +1148,7 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
     // scaled by the residual-minimising step on this level
     if constexpr (std::is_same<T, float>::value) {
       nn_coarse_correction(st->nn.get(), C.g, C.b32, C.d32, C.bc, 0);
-      launch_apply_K<float>(C.g, C.W32, C.d32, C.r32, C.bc, 0);
+      launch_apply_K<float>(C.g, C.W32, C.d32, C.r32, C.bc, 0, C.Wbf);
       nn_line_search(C.n(), C.b32, C.r32, C.d32, C.x32, false, 0);
     }
   } else {
@@ -1139,16 +1156,16 @@ static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
     if (capture && st->capture_kind == 1) {
       // residual the coarse cycle leaves behind: its error K_C^{-1} r is what an additive
       // correction has to supply
-      launch_residual_K<T>(C.g, LV<T>::W(C), LV<T>::x(C), LV<T>::b(C), LV<T>::r(C), C.bc, 0);
+      launch_residual_K<T>(C.g, LV<T>::W(C), LV<T>::x(C), LV<T>::b(C), LV<T>::r(C), C.bc, 0, LV<T>::Wbf(C));
       capture_vector<T>(st, C.n(), LV<T>::r(C));
     }
     if constexpr (std::is_same<T, float>::value) {
       if (use_nn && st->nn_mode == 1) {
         // learned correction of the coarse cycle's remaining error: x_C += NN(b_C - K_C x_C)
         // (b_C is not needed after the cycle: it receives K_C d for the step length)
-        launch_residual_K<float>(C.g, C.W32, C.x32, C.b32, C.r32, C.bc, 0);
+        launch_residual_K<float>(C.g, C.W32, C.x32, C.b32, C.r32, C.bc, 0, C.Wbf);
         nn_coarse_correction(st->nn.get(), C.g, C.r32, C.d32, C.bc, 0);
-        launch_apply_K<float>(C.g, C.W32, C.d32, C.b32, C.bc, 0);
+        launch_apply_K<float>(C.g, C.W32, C.d32, C.b32, C.bc, 0, C.Wbf);
         nn_line_search(C.n(), C.r32, C.b32, C.d32, C.x32, true, 0);
       }
     }
@@ -1515,8 +1532,8 @@ extern "C" int uwg_stokes_create(uwg_field* vel, uwg_field* p, const unsigned ch
 extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
   if (!st) return 0;
   for (auto& A : st->L)
-    for (void* p : {(void*)A.W, (void*)A.W32, (void*)A.bc, (void*)A.dinv, (void*)A.dinv32, (void*)A.x, (void*)A.b,
-                    (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
+    for (void* p : {(void*)A.W, (void*)A.W32, (void*)A.Wbf, (void*)A.bc, (void*)A.dinv, (void*)A.dinv32, (void*)A.x,
+                    (void*)A.b, (void*)A.r, (void*)A.d, (void*)A.x32, (void*)A.b32, (void*)A.r32, (void*)A.d32})
       if (p != (void*)st->bc0) dfree(p);
   dfree(st->bc0);
   if (st->q2) {
@@ -2305,8 +2322,13 @@ extern "C" int uwg_set_q2_mfma(int on) {
   return 0;
 }
 
-extern "C" int uwg_set_apply_chunk(int kz3, int kz2) {
-  set_apply_chunk(kz3, kz2);
+extern "C" int uwg_set_cycle_bf16(int on) {
+  g_cycle_bf16 = on != 0;
+  return 0;
+}
+
+extern "C" int uwg_set_apply_chunk(int wg_per_cu3, int kz2) {
+  set_apply_chunk(wg_per_cu3, kz2);
   return 0;
 }
 
@@ -2377,11 +2399,13 @@ extern "C" int uwg_stokes_time_apply(uwg_stokes* st, int precision, int level, i
   hipEvent_t e0, e1;
   UWG_HIP(hipEventCreate(&e0));
   UWG_HIP(hipEventCreate(&e1));
-  if (precision == 1) {
+  if (precision == 1 || precision == 2) {
+    // 1: the FP32 cycle's apply (bfloat16 weights when the cycle uses them), 2: FP32 weights
     ensure_fp32(st);
-    launch_apply_K<float>(A.g, A.W32, A.x32, A.r32, A.bc, 0);
+    const uint32_t* wb = precision == 1 ? A.Wbf : nullptr;
+    launch_apply_K<float>(A.g, A.W32, A.x32, A.r32, A.bc, 0, wb);
     UWG_HIP(hipEventRecord(e0, 0));
-    for (int i = 0; i < reps; i++) launch_apply_K<float>(A.g, A.W32, A.x32, A.r32, A.bc, 0);
+    for (int i = 0; i < reps; i++) launch_apply_K<float>(A.g, A.W32, A.x32, A.r32, A.bc, 0, wb);
     UWG_HIP(hipEventRecord(e1, 0));
   } else {
     launch_apply_K<double>(A.g, A.W, A.x, A.r, A.bc, 0);

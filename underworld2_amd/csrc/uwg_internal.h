@@ -90,13 +90,19 @@ void launch_gauss_assemble(const Geom& g, const double* eta_g, const double* for
                            double* approxS, const double* Ge, hipStream_t s);
 void launch_restrict_W(const Geom& fine, const Geom& coarse, const double* Wf, double* Wc, hipStream_t s);
 void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
+// K applies.  Wbf (FP32 instance, 3D only): the weights as bfloat16 pairs [14][Es] (see
+// launch_pack_w3_bf16); when given, the apply reads them instead of W.
 template <typename T>
-void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s);
+void launch_apply_K(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, hipStream_t s,
+                    const uint32_t* Wbf = nullptr);
 template <typename T>
-void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s);
+void launch_residual_K(const Geom& g, const T* W, const T* x, const T* b, T* r, const uint8_t* bc, hipStream_t s,
+                       const uint32_t* Wbf = nullptr);
 template <typename T>
 void launch_cheb_step_K(const Geom& g, const T* W, const T* x, T* xn, const T* b, const T* dinv, T* d, T a, T c,
-                        bool first, const uint8_t* bc, hipStream_t s);
+                        bool first, const uint8_t* bc, hipStream_t s, const uint32_t* Wbf = nullptr);
+// FP64 weights [27][Es] -> bfloat16 pairs [14][Es]: word p = bf16(W_2p) | bf16(W_2p+1) << 16
+void launch_pack_w3_bf16(long Es, const double* W, uint32_t* Wb, hipStream_t s);
 template <typename T>
 void launch_apply_G(const Geom& g, const double* Ge, const T* p, const uint8_t* bc, T* y, hipStream_t s);
 template <typename T>
@@ -107,7 +113,7 @@ void launch_prolong_add(const Geom& fine, const Geom& coarse, const T* xc, T* xf
 template <typename T>
 void launch_restrict2(const Geom& fine, const Geom& coarse, const T* rf, T* rc, const uint8_t* bcc,
                       const uint8_t* bcf, hipStream_t s);
-void set_apply_chunk(int kz3, int kz2);
+void set_apply_chunk(int wg_per_cu3, int kz2);
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap);
 bool dvc_wave_supported(int dim, int num);
 long dvc_wave_bound_shorts(int cap);

@@ -373,10 +373,18 @@ class StokesSolver:
     def set_precision(self, precision):
         """"fp64": every operator, cycle and Krylov vector in FP64.
         "mixed": FP64 Krylov (residual, dot products, orthogonalisation) preconditioned by an
-                 FP32 multigrid cycle (FP32 operator and smoother; FP64 coarsest solve).
+                 FP32 multigrid cycle (FP32 operator and smoother, bf16 element weights on 3D
+                 levels -- see set_cycle_weights; FP64 coarsest solve).
         "mixed-refine": FP64 iterative refinement around an FP32 Krylov solve whose
                  operator, cycle and basis vectors are FP32 (reductions in FP64)."""
         self._precision = {"fp64": 0, "mixed": 1, "mixed-cycle": 1, "mixed-refine": 2}[precision]
+
+    @staticmethod
+    def set_cycle_weights(fmt):
+        """Element weights of the 3D FP32 multigrid cycle: "bf16" (default; bf16 inputs to the
+        element contractions, FP32 arithmetic and accumulation) or "fp32".  Process-wide; takes
+        effect at the next solve."""
+        L.check(L.lib().uwg_set_cycle_bf16({"bf16": 1, "fp32": 0}[fmt]), "set_cycle_bf16")
 
     def set_learned(self, model):
         """Attach a trained learned coarse correction (``underworld2_amd.learned``)."""
