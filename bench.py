@@ -108,28 +108,35 @@ def time_mode(P, mode, steps, warmup, rtol):
 
 def roofline(P, precision, reps=50):
     """Average duration of the K apply (the dominant kernel) measured with HIP events on the
-    stream it runs on; algorithmic bytes = E*27*s (weights) + 2*3*N*s (read x, write y).
-    ``traffic`` (HBM bytes per launch) comes from the FETCH_SIZE / WRITE_SIZE counter runs
-    summarised in profiles/k_apply3_sinker128_traffic.json (128^3 only)."""
+    stream it runs on.  precision 0: FP64 apply (k_apply3<double>); 1: the FP32 multigrid
+    cycle's apply (k_apply3f with bfloat16 weights on 3D levels); 2: FP32 apply with FP32
+    weights.  Algorithmic bytes = E*27*s_w (weights) + 2*3*N*s_v (read x, write y) with
+    s_w = 8 / 2 / 4 and s_v = 8 / 4 / 4.  ``traffic`` (HBM bytes per launch) comes from the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/k_apply3_sinker128_traffic.json
+    (128^3 only)."""
     import underworld2_amd._lib as L
     st = P["stokes"]
     ms = ctypes.c_double()
     L.check(L.lib().uwg_stokes_time_apply(st._h, precision, 0, reps, ctypes.byref(ms)), "time_apply")
     mesh = P["mesh"]
     E, N, d = mesh.elementsGlobal, mesh.nodesGlobal, mesh.dim
-    s = 4 if precision == 1 else 8
+    bf16 = precision == 1 and d == 3
+    s_w = {0: 8, 1: 2 if bf16 else 4, 2: 4}[precision]
+    s_v = 8 if precision == 0 else 4
     Q = 3 ** d
-    bytes_ = E * Q * s + 2 * d * N * s
+    bytes_ = E * Q * s_w + 2 * d * N * s_v
     ach = bytes_ / (ms.value * 1e-3)
+    key = {0: "fp64", 1: "bf16" if bf16 else "fp32", 2: "fp32"}[precision]
+    kernel = {0: "k_apply3<double>", 1: "k_apply3f<bf16 weights>" if bf16 else "k_apply3f<fp32 weights>",
+              2: "k_apply3f<fp32 weights>"}[precision]
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "k_apply3_sinker128_traffic.json")
     if os.path.exists(tfile) and mesh.elementRes == (128, 128, 128):
         with open(tfile) as f:
             t = json.load(f)
-        traffic = t["fp32" if precision == 1 else "fp64"]["traffic_bytes"]
+        traffic = t.get(key, {}).get("traffic_bytes")
     return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
-            "traffic": traffic, "kernel": f"k_apply3<{'float' if precision else 'double'}>",
-            "kernel_ms": ms.value, "algorithmic_bytes": bytes_}
+            "traffic": traffic, "kernel": kernel, "kernel_ms": ms.value, "algorithmic_bytes": bytes_}
 
 
 def cpu_baseline(res_sample=24, ppc=30, rtol=1e-6):
@@ -295,7 +302,8 @@ def stream_copy(n=1 << 27, reps=20):
 def apply_only(P):
     """K applies alone (FP32 and FP64) plus the library's plain copy kernel of known size, for
     hardware counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) that calibrate HBM traffic."""
-    out = {"fp32": roofline(P, 1, reps=20), "fp64": roofline(P, 0, reps=20), "copy": stream_copy(1 << 27, 5)}
+    out = {"bf16": roofline(P, 1, reps=20), "fp32": roofline(P, 2, reps=20), "fp64": roofline(P, 0, reps=20),
+           "copy": stream_copy(1 << 27, 5)}
     print(json.dumps(out), flush=True)
 
 
@@ -354,6 +362,8 @@ def main():
     H = modes[head]
     roof = roofline(P, 0 if head in ("fp64", "schur") else 1)
     roof["stream_copy_GB_per_s"] = stream_copy()["GB_per_s"]
+    # the same apply with FP32 weights and the FP64 apply of the outer Krylov iteration
+    others = {"fp32_weights": roofline(P, 2), "fp64": roofline(P, 0)} if head not in ("fp64", "schur") else {}
     adv = None if args.no_advection else time_advection(P)
     cpu = None
     if not args.no_cpu_baseline:
@@ -369,9 +379,12 @@ def main():
         "data": "synthetic: seeded PerCellRandomLayout sinker (sphere r=0.1 at (0.5,0.5,0.6), eta 1e3, rho 1)",
         "config": {"workload": f"3D sinker {args.res}^3 Q1/dQ0, {args.ppc} ppc, free-slip, outer rtol {args.rtol}, "
                                f"end-to-end solve() on a fixed swarm (particle-quadrature assembly + solve), "
-                               f"mode={head}"},
+                               f"mode={head}" + ("" if head in ("fp64", "schur") else
+                                                 " (FP32 cycle with bf16 element weights, FP64 Krylov)")},
         "roofline": roof, "cpu_baseline": cpu, "modes": modes, "advection": adv,
     }
+    if others:
+        line["roofline_other_applies"] = others
     for plain in ("fp64", "schur"):
         if plain in modes:
             line[f"speedup_vs_{plain}"] = H["solves_per_sec"] / modes[plain]["solves_per_sec"]

@@ -209,10 +209,11 @@ int uwg_nn_loss_grad(uwg_nn *h, int B, const float *x, const float *t, const flo
 int uwg_nn_set_data(uwg_nn *h, long S, const float *x, const float *t, const float *mask);
 int uwg_nn_train(uwg_nn *h, int steps, int B, float lr, float weight_decay, long seed, float *losses);
 int uwg_nn_eval(uwg_nn *h, int B, float *loss);
-/* kernel-apply tuning knob: workgroups per compute unit over which the 3D K apply balances its
- * work (default 2, the resident count), element rows per block of the 2D K apply (default 32);
- * values <= 0 leave a setting unchanged */
-int uwg_set_apply_chunk(int wg_per_cu3, int kz2);
+/* kernel-apply tuning knob: node levels per workgroup of the 3D K apply (kz3 > 0 fixes it on
+ * every grid; kz3 < 0 restores the default, chosen per grid to fill the GPU's resident
+ * workgroup slots evenly), element rows per block of the 2D K apply (kz2 > 0; default 32);
+ * zero leaves a setting unchanged */
+int uwg_set_apply_chunk(int kz3, int kz2);
 /* 3D Q2 K apply on the matrix cores (1, default) or the vector kernel (0) */
 int uwg_set_q2_mfma(int on);
 /* element weights of the 3D FP32 multigrid cycle (precision 1): bfloat16 (1, default; the

@@ -387,13 +387,10 @@ void launch_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv
 // ---------------------------------------------------------------------------------------
 template <typename T>
 struct KArgs {
-  int rx, ry, rz, KZ;  // KZ: 2D only, element rows per block
+  int rx, ry, rz, KZ;  // KZ: node levels per workgroup (3D), element rows per block (2D)
   T sx, sy, sz;
   long Ws, nx, nxy;  // Ws: stride of the per-point weight arrays
-  // 3D work split: (ntx x nty) tiles of 16 x 16 element columns, units = tiles x (rz + 1)
-  // owned node levels, dealt as nwg contiguous ranges in tile-major order
-  int ntx, nty, nwg;
-  long units;
+  int ntx, nty;      // 3D: tiles of 16 x 16 element columns (overlapping by one)
 };
 
 template <typename T, typename WF>
@@ -792,9 +789,7 @@ struct NodeOperands {
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin by linear id; renumber them so that each XCD
-// receives a contiguous run of logical ids.  Consecutive ranges of work (the z-segments of one
-// tile, then the neighbouring tiles) then share the XCD's L2, where the cache lines straddling
-// neighbouring tiles' common element column/row are reused.
+// receives a contiguous run of logical ids.
 __device__ __forceinline__ int xcd_logical_id() {
   const int total = gridDim.x;
   const int id = blockIdx.x;
@@ -802,23 +797,18 @@ __device__ __forceinline__ int xcd_logical_id() {
   return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
 }
 
-// Runs body(bx, by, k0, k1) over the workgroup's range of work units: units are owned node
-// levels k of tile (bx, by), tile-major; a range that crosses a tile boundary is streamed as two
-// segments.  Every workgroup receives units / nwg (+1) levels, so a launch of nwg = the number
-// of resident workgroups finishes in one balanced round.
+// The workgroup's chunk: KZ node levels [k0, k1) of tile (bx, by).  Logical ids run x-fastest
+// through the tiles of one z-chunk, so an XCD's concurrently running workgroups hold
+// x-neighbouring tiles of the same levels, and the cache lines straddling two tiles' columns
+// (each tile row covers 16 weights, half of a 128-byte line) are fetched into that XCD's L2
+// once for both.  Ordering the work tile-major instead (long z-runs per tile) fetched the
+// weights about twice over.
 template <typename T, typename Body>
 __device__ __forceinline__ void for_each_segment(const KArgs<T>& A, Body body) {
-  const long wid = xcd_logical_id();
-  long w0 = wid * A.units / A.nwg;
-  const long w1 = (wid + 1) * A.units / A.nwg;
-  const long L = A.rz + 1;
-  while (w0 < w1) {
-    const long tile = w0 / L;
-    const int k0 = (int)(w0 - tile * L);
-    const int k1 = (int)min(L, k0 + (w1 - w0));
-    body((int)(tile % A.ntx), (int)(tile / A.ntx), k0, k1);
-    w0 += k1 - k0;
-  }
+  const int t = xcd_logical_id();
+  const int bx = t % A.ntx, by = (t / A.ntx) % A.nty, zc = t / (A.ntx * A.nty);
+  const int k0 = zc * A.KZ;
+  body(bx, by, k0, min(k0 + A.KZ, A.rz + 1));
 }
 
 // Sum of the four element-column contributions (this level's below / above halves) at the
@@ -1116,9 +1106,10 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
   }
 }
 
-static int g_wg_per_cu3 = 2, g_kz2 = 32;
-void set_apply_chunk(int wg_per_cu3, int kz2) {
-  if (wg_per_cu3 > 0) g_wg_per_cu3 = wg_per_cu3;
+static int g_kz3 = 0, g_kz2 = 32;  // g_kz3 = 0: automatic 3D chunk
+void set_apply_chunk(int kz3, int kz2) {
+  if (kz3 > 0) g_kz3 = kz3;
+  if (kz3 < 0) g_kz3 = 0;
   if (kz2 > 0) g_kz2 = kz2;
 }
 
@@ -1136,6 +1127,26 @@ static int device_cus() {
   return cus[dev];
 }
 
+// z-chunk of the 3D apply: a workgroup streams KZ + 1 element levels (one of them the halo
+// level below its chunk), and two workgroups are resident per CU, so a launch takes about
+// ceil(workgroups / (2 CUs)) rounds x (KZ + 1) level times.  The chunk minimising that product
+// avoids a nearly empty last round (measured at 128^3: 85 us at 11 or 22 levels, 111 us at 16,
+// whose 729 workgroups need a second round of 217).  Small grids get one level per workgroup.
+static int auto_kz3(long tiles, int rz) {
+  const long slots = 2L * device_cus();
+  int best = 1;
+  long best_cost = -1;
+  for (int kz = 1; kz <= rz + 1; kz++) {
+    const long nwg = tiles * ((rz + kz) / kz);
+    const long cost = ((nwg + slots - 1) / slots) * (kz + 1);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = kz;
+    }
+  }
+  return best;
+}
+
 template <typename T, int MODE>
 static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const uint8_t* bc, const SmoothArgs<T>& S,
                               hipStream_t s, const uint32_t* Wbf) {
@@ -1150,22 +1161,20 @@ static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const
   A.nx = g.res[0] + 1;
   A.nxy = A.nx * (g.res[1] + 1);
   if (g.dim == 3) {
-    // one balanced round: as many workgroups as are resident at once (two per CU for both
-    // instances), each streaming an equal share of the tiles' node levels
-    A.KZ = 0;
     if (27.0 * g.Es * sizeof(T) >= 4294967296.0 || 3.0 * g.N * sizeof(T) >= 4294967296.0)
       throw std::runtime_error("3D K apply: weight planes and node vectors must stay below 4 GiB");
     A.ntx = (g.res[0] + 1 + 14) / 15;
     A.nty = (g.res[1] + 1 + 14) / 15;
-    A.units = (long)A.ntx * A.nty * (g.res[2] + 1);
-    A.nwg = (int)std::min<long>(A.units, (long)device_cus() * g_wg_per_cu3);
+    const long tiles = (long)A.ntx * A.nty;
+    A.KZ = g_kz3 > 0 ? std::min(g_kz3, g.res[2] + 1) : auto_kz3(tiles, g.res[2]);
+    const int nwg = (int)(tiles * ((g.res[2] + A.KZ) / A.KZ));
     if constexpr (sizeof(T) == 4) {
       if (Wbf)
-        hipLaunchKernelGGL((k_apply3f<MODE, true>), dim3(A.nwg), 256, 0, s, A, (const void*)Wbf, x, y, bc, S);
+        hipLaunchKernelGGL((k_apply3f<MODE, true>), dim3(nwg), 256, 0, s, A, (const void*)Wbf, x, y, bc, S);
       else
-        hipLaunchKernelGGL((k_apply3f<MODE, false>), dim3(A.nwg), 256, 0, s, A, (const void*)W, x, y, bc, S);
+        hipLaunchKernelGGL((k_apply3f<MODE, false>), dim3(nwg), 256, 0, s, A, (const void*)W, x, y, bc, S);
     } else
-      hipLaunchKernelGGL((k_apply3<T, MODE>), dim3(A.nwg), 256, 0, s, A, W, x, y, bc, S);
+      hipLaunchKernelGGL((k_apply3<T, MODE>), dim3(nwg), 256, 0, s, A, W, x, y, bc, S);
   } else {
     A.KZ = g_kz2;
     dim3 grid((g.res[0] + 1 + 62) / 63, (g.res[1] + 1 + A.KZ - 1) / A.KZ);

@@ -2327,8 +2327,8 @@ extern "C" int uwg_set_cycle_bf16(int on) {
   return 0;
 }
 
-extern "C" int uwg_set_apply_chunk(int wg_per_cu3, int kz2) {
-  set_apply_chunk(wg_per_cu3, kz2);
+extern "C" int uwg_set_apply_chunk(int kz3, int kz2) {
+  set_apply_chunk(kz3, kz2);
   return 0;
 }
 

@@ -113,7 +113,7 @@ void launch_prolong_add(const Geom& fine, const Geom& coarse, const T* xc, T* xf
 template <typename T>
 void launch_restrict2(const Geom& fine, const Geom& coarse, const T* rf, T* rc, const uint8_t* bcc,
                       const uint8_t* bcf, hipStream_t s);
-void set_apply_chunk(int wg_per_cu3, int kz2);
+void set_apply_chunk(int kz3, int kz2);
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap);
 bool dvc_wave_supported(int dim, int num);
 long dvc_wave_bound_shorts(int cap);
