@@ -15,6 +15,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 128] [--ppc 30
                        [--no-advection] [--apply-only]
        python bench.py --q2 --res 64 --ppc 20 [--modes fp64,mixed]
        python bench.py --convection --res 96 --ppc 25 --steps 40 [--modes fp64,mixed,learned]
+           [--series-out F.json] [--compare-series F.json]
 """
 import argparse
 import ctypes
@@ -327,6 +328,11 @@ def main():
                     help="Q2/dPc1 3D sinker: K apply (matrix cores / vector) and solves at --res, --ppc")
     ap.add_argument("--convection", action="store_true",
                     help="run the 3D thermal-convection time series (--res, --ppc, --steps, --modes) instead")
+    ap.add_argument("--series-out", default=None,
+                    help="convection: write each mode's per-step series (t, dt, Nu, v_rms, its) to this JSON file")
+    ap.add_argument("--compare-series", default=None,
+                    help="convection: FP64 series from an earlier --series-out run to compare against (runs "
+                         "too long for one process can be split by mode)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -445,17 +451,33 @@ def convection_main(args):
         r = run_convection(C, args.steps, m, rtol=args.rtol, log_every=max(1, args.steps // 10))
         if r is not None:
             runs[m] = r
+            if args.series_out:
+                with open(args.series_out, "w") as f:
+                    json.dump({k: {q: v[q] for q in ("t", "dt", "nu", "vrms", "its", "s_stokes", "s_supg", "s_adv")}
+                               for k, v in runs.items()}, f)
     line = {"workload": f"3D thermal convection {args.res}^3 Q1/dQ0, Ra 1e6, eta = exp(-ln(1e4) T), {args.ppc} ppc, "
                         f"{args.steps} steps of SUPG + Stokes (rtol {args.rtol}) + RK2", "modes": {}}
     ref = runs.get("fp64")
+    if ref is None and args.compare_series:
+        with open(args.compare_series) as f:
+            ref = json.load(f).get("fp64")
+        if ref is not None:
+            n = min(len(ref["nu"]), args.steps)
+            ref = {"nu": ref["nu"][:n], "vrms": ref["vrms"][:n]}
+            line["compared_with"] = f"FP64 series from {os.path.basename(args.compare_series)}"
     for m, r in runs.items():
         e = {"final_nu": r["nu"][-1], "final_vrms": r["vrms"][-1], "final_time": r["t"][-1],
              "mean_stokes_its": float(np.mean(r["its"])), "s_per_step": (r["s_supg"] + r["s_stokes"] + r["s_adv"]) / len(r["t"]),
              "s_stokes_per_step": r["s_stokes"] / len(r["t"]), "s_supg_per_step": r["s_supg"] / len(r["t"]),
              "s_rk2_per_step": r["s_adv"] / len(r["t"])}
         if ref is not None and m != "fp64":
-            e["max_rel_diff_nu_vs_fp64"] = float(np.max(np.abs(np.array(r["nu"]) - ref["nu"]) / np.abs(ref["nu"])))
-            e["max_rel_diff_vrms_vs_fp64"] = float(np.max(np.abs(np.array(r["vrms"]) - ref["vrms"]) / np.abs(ref["vrms"])))
+            n = min(len(ref["nu"]), len(r["nu"]))
+            a_nu, r_nu = np.array(r["nu"][:n]), np.array(ref["nu"][:n])
+            a_v, r_v = np.array(r["vrms"][:n]), np.array(ref["vrms"][:n])
+            e["max_rel_diff_nu_vs_fp64"] = float(np.max(np.abs(a_nu - r_nu) / np.abs(r_nu)))
+            e["max_rel_diff_vrms_vs_fp64"] = float(np.max(np.abs(a_v - r_v) / np.abs(r_v)))
+            e["final_rel_diff_nu_vs_fp64"] = float(abs(a_nu[-1] - r_nu[-1]) / abs(r_nu[-1]))
+            e["final_rel_diff_vrms_vs_fp64"] = float(abs(a_v[-1] - r_v[-1]) / abs(r_v[-1]))
         line["modes"][m] = e
     print(json.dumps(line), flush=True)
 

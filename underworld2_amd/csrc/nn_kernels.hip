@@ -420,13 +420,15 @@ __global__ void k_adam(long n, float* __restrict__ p, const float* __restrict__ 
 
 // inference helpers --------------------------------------------------------------------
 // rms of the masked residual (single block, deterministic) -> scale[0] = rms, scale[1] = 1/rms
-__global__ __launch_bounds__(256) void k_rms(long n, const float* __restrict__ r, float* __restrict__ scale) {
-  __shared__ float sm[256];
+// one 1024-thread block: the coarse-level vectors are ~1e4-1e5 values, a single block with a
+// fixed reduction tree keeps the result deterministic
+__global__ __launch_bounds__(1024) void k_rms(long n, const float* __restrict__ r, float* __restrict__ scale) {
+  __shared__ float sm[1024];
   float a = 0.0f;
-  for (long t = threadIdx.x; t < n; t += 256) a += r[t] * r[t];
+  for (long t = threadIdx.x; t < n; t += 1024) a += r[t] * r[t];
   sm[threadIdx.x] = a;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = 512; s > 0; s >>= 1) {
     if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
     __syncthreads();
   }
@@ -701,19 +703,19 @@ void nn_set_feature(NNModel* m, const float* feat_host, double eta_ref) {
 // Residual-minimising step length for a correction d of K x = b with residual r:
 // omega = (r . Kd) / (Kd . Kd), clamped to [0, 2] (a prediction pointing the wrong way, or a
 // non-finite one, is dropped); then x = (accumulate ? x : 0) + omega d.  Single block, FP64 sums, deterministic.
-__global__ __launch_bounds__(256) void k_line_search(long n, const float* __restrict__ r, const float* __restrict__ kd,
-                                                     const float* __restrict__ d, float* __restrict__ x,
-                                                     int accumulate) {
-  __shared__ double s1[256], s2[256];
+__global__ __launch_bounds__(1024) void k_line_search(long n, const float* __restrict__ r, const float* __restrict__ kd,
+                                                      const float* __restrict__ d, float* __restrict__ x,
+                                                      int accumulate) {
+  __shared__ double s1[1024], s2[1024];
   double a1 = 0.0, a2 = 0.0;
-  for (long t = threadIdx.x; t < n; t += 256) {
+  for (long t = threadIdx.x; t < n; t += 1024) {
     a1 += (double)r[t] * kd[t];
     a2 += (double)kd[t] * kd[t];
   }
   s1[threadIdx.x] = a1;
   s2[threadIdx.x] = a2;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
+  for (int st = 512; st > 0; st >>= 1) {
     if (threadIdx.x < st) {
       s1[threadIdx.x] += s1[threadIdx.x + st];
       s2[threadIdx.x] += s2[threadIdx.x + st];
@@ -724,18 +726,18 @@ __global__ __launch_bounds__(256) void k_line_search(long n, const float* __rest
   if (!(om > 0.0)) om = 0.0;  // also catches a non-finite prediction
   if (om > 2.0) om = 2.0;
   const float w = (float)om;
-  for (long t = threadIdx.x; t < n; t += 256) x[t] = (accumulate ? x[t] : 0.0f) + w * d[t];
+  for (long t = threadIdx.x; t < n; t += 1024) x[t] = (accumulate ? x[t] : 0.0f) + w * d[t];
 }
 
 void nn_line_search(long n, const float* r, const float* kd, const float* d, float* x, bool accumulate,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_line_search, 1, 256, 0, s, n, r, kd, d, x, accumulate ? 1 : 0);
+  hipLaunchKernelGGL(k_line_search, 1, 1024, 0, s, n, r, kd, d, x, accumulate ? 1 : 0);
   UWG_HIP(hipGetLastError());
 }
 
 void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, const uint8_t* bc, hipStream_t s) {
   const long N = m->grid[0].N();
-  hipLaunchKernelGGL(k_rms, 1, 256, 0, s, N * 3, r, m->scale);
+  hipLaunchKernelGGL(k_rms, 1, 1024, 0, s, N * 3, r, m->scale);
   hipLaunchKernelGGL(k_pack_input, nbk(N, 256), 256, 0, s, N, 3, r, m->feat, m->scale, m->xin);
   nn_forward(m, 1, s);
   hipLaunchKernelGGL(k_unpack_output, nbk(N, 256), 256, 0, s, N, 3, m->y, m->scale, m->out_scale, bc, x);
