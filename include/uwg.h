@@ -85,7 +85,8 @@ int uwg_swarm_get_integration(uwg_swarm *s, double *xi_slot, double *w_slot);
 int uwg_swarm_set_integration(uwg_swarm *s, const double *xi_slot, const double *w_slot);
 
 /* ---- advection: SwarmAdvector(vel, swarm, order=2).integrate(dt, update_owners)
- * PY/systems/_timeintegration.py:115-157, TimeIntegrand.c:249-319, SwarmAdvector.c:238-312 */
+ * PY/systems/_timeintegration.py:115-157, TimeIntegrand.c:249-319, SwarmAdvector.c:238-312.
+ * The velocity field may live on a Q1 or a Q2 mesh (Biquadratic / Triquadratic lookup). */
 int uwg_advect_rk2(uwg_swarm *s, uwg_field *vel, double dt, int update_owners, long *outside);
 /* SwarmAdvector.get_max_dt (SwarmAdvector.c:370-385) */
 int uwg_advect_max_dt(uwg_field *vel, double *dt);

@@ -48,6 +48,9 @@ def lib():
         L.orc_dvc_all.restype = ctypes.c_int
         L.orc_element_approxS.restype = ctypes.c_double
         L.orc_advect_rk2.restype = ctypes.c_long
+        L.orc_advect_rk2_q2.restype = ctypes.c_long
+        L.orc_q2_newton_local.restype = ctypes.c_int
+        L.orc_q2_min_sep.restype = ctypes.c_double
         L.orc_max_dt.restype = ctypes.c_double
         L.orc_update_owners.restype = ctypes.c_long
         L.orc_bsscr_solve.restype = ctypes.c_int
@@ -252,6 +255,55 @@ def interpolate(mesh, vel, x):
 
 def max_dt(mesh, vel):
     return lib().orc_max_dt(*mesh.args(), _d(np.ascontiguousarray(vel, dtype=np.float64)))
+
+
+# --- lookups in a Q2 velocity field (vertex grid of 2 res + 1 per direction on the same element
+# grid); see the Q2 section of uwg_oracle.c for the reference citations and pins
+def q2_vertex_coords(mesh):
+    """[(2r+1)^d][d] vertex coordinates of the Q2 mesh on ``mesh``'s element grid."""
+    fine = Mesh(tuple(2 * int(r) for r in mesh.res), mesh.lo, mesh.hi)
+    return fine.coords()
+
+
+def q2_basis(dim, xi):
+    """(N[3^d], G[d][3^d]) of the Biquadratic / Triquadratic element types at xi."""
+    nn = 3 ** dim
+    N = np.zeros(27)
+    G = np.zeros((3, 27))
+    lib().orc_q2_basis(ctypes.c_int(dim), _d(np.ascontiguousarray(xi, dtype=np.float64)), _d(N), _d(G))
+    return N[:nn], G[:dim, :nn]
+
+
+def q2_newton_local(dim, verts, x):
+    xi = np.zeros(3)
+    it = lib().orc_q2_newton_local(ctypes.c_int(dim), _d(np.ascontiguousarray(verts, dtype=np.float64)),
+                                   _d(np.ascontiguousarray(x, dtype=np.float64)), _d(xi))
+    return xi[:dim], int(it)
+
+
+def advect_rk2_q2(mesh, vel, coords, dt):
+    c = np.ascontiguousarray(coords, dtype=np.float64).copy()
+    fb = ctypes.c_long(0)
+    bad = lib().orc_advect_rk2_q2(*mesh.args(), _d(np.ascontiguousarray(vel, dtype=np.float64)),
+                                  ctypes.c_long(c.shape[0]), _d(c), ctypes.c_double(dt), ctypes.byref(fb))
+    return c, int(bad), int(fb.value)
+
+
+def interpolate_q2(mesh, vel, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    P = x.shape[0]
+    v = np.zeros_like(x)
+    ok = np.zeros(P, dtype=np.int32)
+    lib().orc_interpolate_q2(*mesh.args(), _d(np.ascontiguousarray(vel, dtype=np.float64)), ctypes.c_long(P),
+                             _d(x), _d(v), _i(ok))
+    return v, ok.astype(bool)
+
+
+def max_dt_q2(mesh, vel):
+    """0.5 * minSep(Q2 mesh) / max_vertices |v| (SwarmAdvector_MaxDt on a Q2 velocity field)."""
+    ms = lib().orc_q2_min_sep(*mesh.args())
+    vmax = float(np.max(np.sqrt(np.sum(np.asarray(vel, dtype=np.float64) ** 2, axis=1))))
+    return 0.5 * ms / vmax
 
 
 class CellTables:

@@ -22,6 +22,9 @@
  *   orc_advect_rk2        doctest [0.27856742,-0.12143258] (_timeintegration.py:131-147)
  *                         and exactness for affine velocity fields
  *   orc_update_owners     hand-worked swap-with-last/append sequences
+ *   orc_q2_* / *_q2       Lagrange property, partition of unity, derivatives against
+ *                         central differences, Newton closed form on affine cells, exact
+ *                         interpolation and RK2 closed form for quadratic velocity fields
  */
 #include <math.h>
 #include <stdlib.h>
@@ -819,6 +822,230 @@ void orc_interpolate(int dim, const int *res, const double *lo, const double *hi
                      const double *vel, long P, const double *x, double *v, int *ok)
 {
     for (long p = 0; p < P; p++) ok[p] = velocity_at(dim, res, lo, hi, vel, x + p * dim, v + p * dim);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* A15/A16 with a Q2 velocity field: the lookup through the Biquadratic / Triquadratic    */
+/* element types (LU/StgFEM/Discretisation/src/Biquadratic.c:138-192, Triquadratic.c:    */
+/* 193-374), the generic Newton ElementType.c:177-301 over the element's 3^d vertices     */
+/* (C2Generator incidence order a = ax + 3 ay + 9 az on the vertex grid of 2 res + 1).    */
+/* Pins (tests/test_q2_advect_oracle.py): Lagrange property and partition of unity of     */
+/* the basis, derivatives against central differences, Newton = closed form on affine     */
+/* cells, interpolation exact for quadratic fields, RK2 = closed form for quadratic        */
+/* velocity fields.                                                                         */
+/* ------------------------------------------------------------------------------------ */
+/* 1D factors f(t) = (t-1, 1-t^2, t+1); prefactor names of Triquadratic_EvalBasis          */
+static void q2_basis3(const double *xi, double *N)
+{
+    double x = xi[0], y = xi[1], z = xi[2];
+    double f[3][3] = {{x - 1.0, 1.0 - x * x, x + 1.0}, {y - 1.0, 1.0 - y * y, y + 1.0},
+                      {z - 1.0, 1.0 - z * z, z + 1.0}};
+    double m0 = 0.5 * x, m1 = 0.5 * y, m2 = 0.5 * z;
+    double m3 = 0.25 * x * y, m4 = 0.25 * x * z, m5 = 0.25 * y * z;
+    double m6 = 0.125 * x * y * z;
+    for (int az = 0; az < 3; az++)
+        for (int ay = 0; ay < 3; ay++)
+            for (int ax = 0; ax < 3; ax++) {
+                int a = ax + 3 * ay + 9 * az;
+                int mx = ax == 1, my = ay == 1, mz = az == 1;
+                double pre;
+                if (mx && my && mz) { N[a] = f[0][1] * f[1][1] * f[2][1]; continue; }
+                if (mx && my) pre = m2;
+                else if (mx && mz) pre = m1;
+                else if (my && mz) pre = m0;
+                else if (mx) pre = m5;
+                else if (my) pre = m4;
+                else if (mz) pre = m3;
+                else pre = m6;
+                N[a] = pre * f[0][ax] * f[1][ay] * f[2][az];
+            }
+}
+
+static void q2_derivs3(const double *xi, double G[3][27])
+{
+    double x = xi[0], y = xi[1], z = xi[2];
+    double f[3][3] = {{x - 1.0, 1.0 - x * x, x + 1.0}, {y - 1.0, 1.0 - y * y, y + 1.0},
+                      {z - 1.0, 1.0 - z * z, z + 1.0}};
+    double g[3][3] = {{x - 0.5, -2.0 * x, x + 0.5}, {y - 0.5, -2.0 * y, y + 0.5},
+                      {z - 0.5, -2.0 * z, z + 0.5}};
+    double m0 = 0.5 * x, m1 = 0.5 * y, m2 = 0.5 * z;
+    double m3 = 0.25 * x * y, m4 = 0.25 * x * z, m5 = 0.25 * y * z;
+    for (int az = 0; az < 3; az++)
+        for (int ay = 0; ay < 3; ay++)
+            for (int ax = 0; ax < 3; ax++) {
+                int a = ax + 3 * ay + 9 * az;
+                int mx = ax == 1, my = ay == 1, mz = az == 1;
+                /* d/dxi: g_x(ax) * pre(y,z) * f_y * f_z */
+                if (my && mz) G[0][a] = g[0][ax] * f[1][ay] * f[2][az];
+                else G[0][a] = g[0][ax] * (my ? m2 : (mz ? m1 : m5)) * f[1][ay] * f[2][az];
+                /* d/deta: g_y(ay) * pre(x,z) * f_x * f_z */
+                if (mx && mz) G[1][a] = g[1][ay] * f[0][ax] * f[2][az];
+                else G[1][a] = g[1][ay] * (mx ? m2 : (mz ? m0 : m4)) * f[0][ax] * f[2][az];
+                /* d/dzeta: g_z(az) * pre(x,y) * f_x * f_y */
+                if (mx && my) G[2][a] = g[2][az] * f[0][ax] * f[1][ay];
+                else G[2][a] = g[2][az] * (mx ? m1 : (my ? m0 : m3)) * f[0][ax] * f[1][ay];
+            }
+}
+
+static void q2_basis2(const double *xi, double *N)
+{
+    double x = xi[0], y = xi[1];
+    double a0 = x - 1.0, b0 = y - 1.0, a1 = 1.0 - x * x, b1 = 1.0 - y * y, a2 = x + 1.0, b2 = y + 1.0;
+    double m0 = 0.5 * x, m1 = 0.5 * y, m2 = 0.25 * x * y;
+    N[0] = m2 * a0 * b0; N[1] = m1 * a1 * b0; N[2] = m2 * a2 * b0;
+    N[3] = m0 * a0 * b1; N[4] = a1 * b1;      N[5] = m0 * a2 * b1;
+    N[6] = m2 * a0 * b2; N[7] = m1 * a1 * b2; N[8] = m2 * a2 * b2;
+}
+
+static void q2_derivs2(const double *xi, double G[3][27])
+{
+    double x = xi[0], y = xi[1];
+    double a0 = x - 1.0, b0 = y - 1.0, a1 = x + 1.0, b1 = y + 1.0;
+    double a2 = 2.0 * x - 1.0, b2 = 2.0 * y - 1.0, a3 = 2.0 * x + 1.0, b3 = 2.0 * y + 1.0;
+    double a4 = 1.0 - x * x, b4 = 1.0 - y * y;
+    double m0 = 0.25 * x, m1 = 0.25 * y, m2 = -x * y;
+    G[0][0] = m1 * a2 * b0; G[0][2] = m1 * a3 * b0; G[0][6] = m1 * a2 * b1; G[0][8] = m1 * a3 * b1;
+    G[1][0] = m0 * a0 * b2; G[1][2] = m0 * a1 * b2; G[1][6] = m0 * a0 * b3; G[1][8] = m0 * a1 * b3;
+    G[0][1] = m2 * b0; G[0][7] = m2 * b1; G[0][3] = 0.5 * a2 * b4; G[0][5] = 0.5 * a3 * b4;
+    G[1][1] = 0.5 * a4 * b2; G[1][7] = 0.5 * a4 * b3; G[1][3] = m2 * a0; G[1][5] = m2 * a1;
+    G[0][4] = -2.0 * x * b4; G[1][4] = -2.0 * y * a4;
+}
+
+/* basis and local derivatives for the tests */
+void orc_q2_basis(int dim, const double *xi, double *N, double *G /*[3][27]*/)
+{
+    double GG[3][27];
+    if (dim == 3) { q2_basis3(xi, N); q2_derivs3(xi, GG); }
+    else { q2_basis2(xi, N); q2_derivs2(xi, GG); }
+    for (int d = 0; d < dim; d++)
+        for (int a = 0; a < (dim == 3 ? 27 : 9); a++) G[d * 27 + a] = GG[d][a];
+}
+
+static void q2_element_verts(int dim, const int *res, const double *lo, const double *hi, long e, double *v)
+{
+    long i = e % res[0], j = (e / res[0]) % res[1], k = dim == 3 ? e / ((long)res[0] * res[1]) : 0;
+    int nn = dim == 3 ? 27 : 9;
+    for (int a = 0; a < nn; a++) {
+        v[a * dim + 0] = node_coord_1d((int)(2 * i + a % 3), 2 * res[0] + 1, lo[0], hi[0]);
+        v[a * dim + 1] = node_coord_1d((int)(2 * j + (a / 3) % 3), 2 * res[1] + 1, lo[1], hi[1]);
+        if (dim == 3) v[a * dim + 2] = node_coord_1d((int)(2 * k + a / 9), 2 * res[2] + 1, lo[2], hi[2]);
+    }
+}
+
+/* ElementType.c:177-301 with the quadratic element's 9 / 27 nodes; returns iterations */
+int orc_q2_newton_local(int dim, const double *verts, const double *x, double *xi)
+{
+    int nn = dim == 3 ? 27 : 9;
+    for (int d = 0; d < dim; d++) xi[d] = 0.0;
+    for (int it = 0; it < 100; it++) {
+        double J[9] = {0}, rhs[3] = {0, 0, 0}, inc[3], N[27], G[3][27];
+        if (dim == 3) { q2_basis3(xi, N); q2_derivs3(xi, G); }
+        else { q2_basis2(xi, N); q2_derivs2(xi, G); }
+        for (int n = 0; n < nn; n++) {
+            const double *c = verts + n * dim;
+            double sf = N[n];
+            J[0 * dim + 0] += G[0][n] * c[0];
+            J[0 * dim + 1] += G[1][n] * c[0];
+            J[1 * dim + 0] += G[0][n] * c[1];
+            J[1 * dim + 1] += G[1][n] * c[1];
+            rhs[0] -= sf * c[0];
+            rhs[1] -= sf * c[1];
+            if (dim == 3) {
+                J[2] += G[2][n] * c[0];
+                J[5] += G[2][n] * c[1];
+                J[6] += G[0][n] * c[2];
+                J[7] += G[1][n] * c[2];
+                J[8] += G[2][n] * c[2];
+                rhs[2] -= sf * c[2];
+            }
+        }
+        rhs[0] += x[0];
+        rhs[1] += x[1];
+        if (dim == 3) rhs[2] += x[2];
+        cramer(dim, J, inc, rhs);
+        xi[0] += inc[0];
+        xi[1] += inc[1];
+        if (dim == 3) xi[2] += inc[2];
+        {
+            double m = fabs(inc[0]);
+            if (m < fabs(inc[1])) m = fabs(inc[1]);
+            if (dim == 3 && m < fabs(inc[2])) m = fabs(inc[2]);
+            if (m < 0.0001) return it + 1;
+        }
+    }
+    xi[0] = 1.1;
+    return 100;
+}
+
+static int velocity_at_q2(int dim, const int *res, const double *lo, const double *hi,
+                          const double *vel, const double *x, double *v)
+{
+    long el;
+    double verts[81], xi[3], N[27];
+    int nn = dim == 3 ? 27 : 9;
+    if (!orc_search_element(dim, res, lo, hi, x, &el)) return 0;
+    q2_element_verts(dim, res, lo, hi, el, verts);
+    orc_q2_newton_local(dim, verts, x, xi);
+    if (dim == 3) q2_basis3(xi, N); else q2_basis2(xi, N);
+    {
+        long i = el % res[0], j = (el / res[0]) % res[1], k = dim == 3 ? el / ((long)res[0] * res[1]) : 0;
+        long nx = 2L * res[0] + 1, ny = 2L * res[1] + 1;
+        for (int d = 0; d < dim; d++) v[d] = 0.0;
+        for (int a = 0; a < nn; a++) {
+            long node = (2 * i + a % 3) + nx * ((2 * j + (a / 3) % 3) + ny * (dim == 3 ? 2 * k + a / 9 : 0));
+            for (int d = 0; d < dim; d++) v[d] += vel[node * dim + d] * N[a];
+        }
+    }
+    for (int d = 0; d < dim; d++) if (isinf(v[d])) return 0;
+    return 1;
+}
+
+long orc_advect_rk2_q2(int dim, const int *res, const double *lo, const double *hi,
+                       const double *vel, long P, double *coords, double dt, long *nfallback)
+{
+    long bad = 0, fb = 0;
+    for (long p = 0; p < P; p++) {
+        double *x = coords + p * dim, start[3], v0[3], v1[3];
+        for (int d = 0; d < dim; d++) start[d] = x[d];
+        if (!velocity_at_q2(dim, res, lo, hi, vel, x, v0)) { bad++; continue; }
+        for (int d = 0; d < dim; d++) x[d] = start[d] + 0.5 * dt * v0[d];
+        if (velocity_at_q2(dim, res, lo, hi, vel, x, v1)) {
+            for (int d = 0; d < dim; d++) x[d] = start[d] + dt * v1[d];
+        } else {
+            for (int d = 0; d < dim; d++) x[d] = start[d];
+            velocity_at_q2(dim, res, lo, hi, vel, x, v0);
+            for (int d = 0; d < dim; d++) x[d] = start[d] + dt * v0[d];
+            fb++;
+        }
+    }
+    if (nfallback) *nfallback = fb;
+    return bad;
+}
+
+void orc_interpolate_q2(int dim, const int *res, const double *lo, const double *hi,
+                        const double *vel, long P, const double *x, double *v, int *ok)
+{
+    for (long p = 0; p < P; p++) ok[p] = velocity_at_q2(dim, res, lo, hi, vel, x + p * dim, v + p * dim);
+}
+
+/* Mesh_HexType_GetMinimumSeparationGeneral on the Q2 mesh: pairwise distances of every
+   element's 3^d vertices */
+double orc_q2_min_sep(int dim, const int *res, const double *lo, const double *hi)
+{
+    long E = (long)res[0] * res[1] * (dim == 3 ? res[2] : 1);
+    int nn = dim == 3 ? 27 : 9;
+    double ms = HUGE_VAL, v[81];
+    for (long e = 0; e < E; e++) {
+        q2_element_verts(dim, res, lo, hi, e, v);
+        for (int a = 0; a < nn - 1; a++)
+            for (int b = a + 1; b < nn; b++) {
+                double m = 0.0, t;
+                for (int d = 0; d < dim; d++) { t = v[a * dim + d] - v[b * dim + d]; m += t * t; }
+                m = sqrt(m);
+                if (m < ms) ms = m;
+            }
+    }
+    return ms;
 }
 
 /* A18  SwarmAdvector_MaxDt (SwarmAdvector.c:370-385): 0.5*minSep/max_node|v|.

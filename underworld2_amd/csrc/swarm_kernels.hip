@@ -585,35 +585,76 @@ __device__ bool velocity_at(const GeomArgs& g, const double* vel, const double* 
   return true;
 }
 
+// The same lookup in a Q2 velocity field (vertex grid of 2 res + 1 per direction): element search
+// on the element grid, Newton on the element's 9 / 27 vertices with the quadratic shape
+// functions (Biquadratic / Triquadratic element types), value summed over the vertices in
+// incidence order.
 template <int DIM>
+__device__ bool velocity_at_q2(const GeomArgs& g, const double* vel, const double* x, double* v) {
+  long el;
+  if (!search_element<DIM>(g.res, g.lo, g.hi, x, &el)) return false;
+  constexpr int NN = DIM == 3 ? 27 : 9;
+  double verts[NN * DIM], xi[3], N[NN];
+  q2_element_verts<DIM>(g.res, g.lo, g.hi, el, verts);
+  q2_newton_local<DIM>(verts, x, xi);
+  q2_basis<DIM>(xi, N);
+  const long i = el % g.res[0], j = (el / g.res[0]) % g.res[1], k = DIM == 3 ? el / ((long)g.res[0] * g.res[1]) : 0;
+  const long nx = 2L * g.res[0] + 1, ny = 2L * g.res[1] + 1;
+  for (int d = 0; d < DIM; d++) v[d] = 0.0;
+  for (int a = 0; a < NN; a++) {
+    const long n = (2 * i + a % 3) + nx * ((2 * j + (a / 3) % 3) + ny * (DIM == 3 ? 2 * k + a / 9 : 0));
+    for (int d = 0; d < DIM; d++) v[d] += vel[n * DIM + d] * N[a];
+  }
+  for (int d = 0; d < DIM; d++)
+    if (isinf(v[d])) return false;
+  return true;
+}
+
+template <int DIM, int ORDER>
+__device__ __forceinline__ bool velocity_lookup(const GeomArgs& g, const double* vel, const double* x, double* v) {
+  if constexpr (ORDER == 2)
+    return velocity_at_q2<DIM>(g, vel, x, v);
+  else
+    return velocity_at<DIM>(g, vel, x, v);
+}
+
+template <int DIM, int ORDER>
 __global__ void k_advect_rk2(GeomArgs g, long P, const double* vel, double* coords, double dt, int* counts) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   double x[3], start[3], v0[3], v1[3];
   for (int d = 0; d < DIM; d++) start[d] = x[d] = coords[p * DIM + d];
-  if (!velocity_at<DIM>(g, vel, x, v0)) {
+  if (!velocity_lookup<DIM, ORDER>(g, vel, x, v0)) {
     atomicAdd(&counts[0], 1);
     return;
   }
   for (int d = 0; d < DIM; d++) x[d] = start[d] + 0.5 * dt * v0[d];
-  if (velocity_at<DIM>(g, vel, x, v1)) {
+  if (velocity_lookup<DIM, ORDER>(g, vel, x, v1)) {
     for (int d = 0; d < DIM; d++) x[d] = start[d] + dt * v1[d];
   } else {
     for (int d = 0; d < DIM; d++) x[d] = start[d];
-    velocity_at<DIM>(g, vel, x, v0);
+    velocity_lookup<DIM, ORDER>(g, vel, x, v0);
     for (int d = 0; d < DIM; d++) x[d] = start[d] + dt * v0[d];
     atomicAdd(&counts[1], 1);
   }
   for (int d = 0; d < DIM; d++) coords[p * DIM + d] = x[d];
 }
 
+// g: the element grid; order 2: vel lives on the Q2 vertex grid
 void launch_advect_rk2(const Geom& g, long P, const double* vel, double* coords, double dt, int* d_counts,
-                       hipStream_t s) {
+                       hipStream_t s, int order) {
   if (P == 0) return;
-  if (g.dim == 3)
-    hipLaunchKernelGGL(k_advect_rk2<3>, nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
-  else
-    hipLaunchKernelGGL(k_advect_rk2<2>, nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+  if (order == 2) {
+    if (g.dim == 3)
+      hipLaunchKernelGGL((k_advect_rk2<3, 2>), nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+    else
+      hipLaunchKernelGGL((k_advect_rk2<2, 2>), nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+  } else {
+    if (g.dim == 3)
+      hipLaunchKernelGGL((k_advect_rk2<3, 1>), nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+    else
+      hipLaunchKernelGGL((k_advect_rk2<2, 1>), nblocks(P, 256), 256, 0, s, pack(g), P, vel, coords, dt, d_counts);
+  }
   UWG_HIP(hipGetLastError());
 }
 

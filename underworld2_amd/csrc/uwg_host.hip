@@ -295,8 +295,9 @@ extern "C" int uwg_mesh_create(int dim, const int* res, const double* minc, cons
   API_END
 }
 
-// Q2 mesh (C2Generator): (2 res + 1)^dim vertices on the same element grid; the minimum
-// vertex separation of a quadratic element is half the element spacing
+// Q2 mesh (C2Generator): (2 res + 1)^dim vertices on the same element grid.  The minimum
+// separation is again the minimum over elements of the pairwise distances of the element's
+// vertices, now its 3^dim vertices (Mesh_HexType_GetMinimumSeparationGeneral).
 extern "C" int uwg_mesh_create_q2(int dim, const int* res, const double* minc, const double* maxc, uwg_mesh** out) {
   int rc = uwg_mesh_create(dim, res, minc, maxc, out);
   if (rc) return rc;
@@ -306,7 +307,27 @@ extern "C" int uwg_mesh_create_q2(int dim, const int* res, const double* minc, c
   for (int d = 0; d < 3; d++) r2[d] = d < dim ? 2 * res[d] : 1;
   m->gn = make_geom(dim, r2, minc, maxc);
   m->order = 2;
-  m->min_sep *= 0.5;
+  double ms = HUGE_VAL;
+  const int nn = dim == 3 ? 27 : 9;
+  for (long e = 0; e < m->g.E; e++) {
+    double v[81];
+    if (dim == 3)
+      q2_element_verts<3>(m->g.res, m->g.lo, m->g.hi, e, v);
+    else
+      q2_element_verts<2>(m->g.res, m->g.lo, m->g.hi, e, v);
+    for (int a = 0; a < nn - 1; a++)
+      for (int b = a + 1; b < nn; b++) {
+        double sq = 0.0;
+        for (int d = 0; d < dim; d++) {
+          double t = v[a * dim + d] - v[b * dim + d];
+          sq += t * t;
+        }
+        sq = sqrt(sq);
+        if (sq < ms) ms = sq;
+      }
+    if (e > 4096 && m->g.E > 100000) break;  // regular spacing: remaining elements repeat
+  }
+  m->min_sep = ms;
   return 0;
   API_END
 }
@@ -657,11 +678,10 @@ extern "C" int uwg_swarm_set_integration(uwg_swarm* s, const double* xi_slot, co
 extern "C" int uwg_advect_rk2(uwg_swarm* s, uwg_field* vel, double dt, int update_owners, long* outside) {
   if (vel->mesh != s->mesh || vel->on_elements || vel->ncomp != s->mesh->g.dim)
     return arg_error("velocity field must be a nodal vector field on the swarm's mesh");
-  if (vel->mesh->order != 1) return arg_error("swarm advection through a Q2 velocity field is not provided");
   API_BEGIN
   int* cnt = dalloc<int>(2);
   UWG_HIP(hipMemset(cnt, 0, sizeof(int) * 2));
-  launch_advect_rk2(s->mesh->g, s->P, vel->d, s->coords, dt, cnt, 0);
+  launch_advect_rk2(s->mesh->g, s->P, vel->d, s->coords, dt, cnt, 0, vel->mesh->order);
   int h[2];
   UWG_HIP(hipMemcpy(h, cnt, sizeof(int) * 2, hipMemcpyDeviceToHost));
   dfree(cnt);

@@ -64,8 +64,9 @@ void launch_cell_of(const Geom& g, long P, const double* coords, int* owner, hip
 void launch_coincident_map(const Geom& g, long P, const int* slot_cell, const int* cell_pid,
                            const double* coords, double* xi_slot, hipStream_t s);
 void launch_slot_cell(long E, const int* cell_ptr, int* slot_cell, hipStream_t s);
+// g: the element grid; order 1: vel on the element grid's nodes, order 2: on the Q2 vertex grid
 void launch_advect_rk2(const Geom& g, long P, const double* vel, double* coords, double dt,
-                       int* d_counts /*[2]: bad, fallback*/, hipStream_t s);
+                       int* d_counts /*[2]: bad, fallback*/, hipStream_t s, int order = 1);
 void launch_interpolate(const Geom& g, long P, const double* vel, const double* x, double* v,
                         int* ok, hipStream_t s);
 void launch_node_speed_max(const Geom& g, const double* vel, double* partial, int nblk, hipStream_t s);
