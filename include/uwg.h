@@ -165,7 +165,9 @@ int uwg_stokes_solve(uwg_stokes *st, const uwg_solve_opts *o, uwg_solve_stats *s
 int uwg_stokes_residuals(uwg_stokes *st, double *out3);
 
 /* ---- operator hooks (tests and the learned-part trainer) */
-/* which: 0 = K (level), 1 = G, 2 = G^T, 3 = one multigrid V-cycle on K (level 0).
+/* which: 0 = K (level), 1 = G, 2 = G^T, 3 = one multigrid V-cycle on K (level 0), 4 = K as the
+ * FP32 cycle applies it (FP32 vectors; bfloat16 weights on 3D levels when the cycle uses them),
+ * 5 = K in FP32 with FP32 weights (4 and 5: Q1 levels; x rounded to FP32, y widened).
  * Q2/dPc1 systems: level 0 is the Q2 level (pressure vectors [E][dim+1]), levels 1.. are the
  * Q1 levels of the multigrid hierarchy below it (K only) */
 int uwg_stokes_apply(uwg_stokes *st, int which, int level, const double *x, double *y);

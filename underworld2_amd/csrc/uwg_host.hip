@@ -2260,13 +2260,29 @@ extern "C" int uwg_stokes_apply(uwg_stokes* st, int which, int level, const doub
   API_BEGIN
   Level& A = st->L[level];
   long nin, nout;
-  if (which == 0 || which == 3) nin = nout = A.n();
+  if (which == 0 || which == 3 || which == 4 || which == 5) nin = nout = A.n();
   else if (which == 1) { nin = st->g.E; nout = st->L[0].n(); }
   else if (which == 2) { nin = st->L[0].n(); nout = st->g.E; }
-  else return arg_error("which must be 0..3");
+  else return arg_error("which must be 0..5");
   double* dx = dalloc<double>(nin);
   double* dy = dalloc<double>(nout);
   UWG_HIP(hipMemcpy(dx, x, sizeof(double) * nin, hipMemcpyHostToDevice));
+  if (which == 4 || which == 5) {
+    // the FP32 cycle's K (4: with the cycle's weights, bfloat16 on 3D levels when enabled;
+    // 5: FP32 weights) on x rounded to FP32, result widened to FP64
+    ensure_fp32(st);
+    float* fx = dalloc<float>(nin);
+    float* fy = dalloc<float>(nout);
+    vk_d2f(nin, 1.0, dx, fx, 0);
+    launch_apply_K<float>(A.g, A.W32, fx, fy, A.bc, 0, which == 4 ? A.Wbf : nullptr);
+    vk_f2d(nout, 1.0, fy, dy, 0);
+    UWG_HIP(hipMemcpy(y, dy, sizeof(double) * nout, hipMemcpyDeviceToHost));
+    dfree(fx);
+    dfree(fy);
+    dfree(dx);
+    dfree(dy);
+    return 0;
+  }
   if (which == 0)
     launch_apply_K<double>(A.g, A.W, dx, dy, A.bc, 0);
   else if (which == 1)
