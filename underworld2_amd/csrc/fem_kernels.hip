@@ -1677,11 +1677,59 @@ void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial,
     hipLaunchKernelGGL((k_mdot<1, T>), RB, 256, 0, s, n, k, x, Y, partial);
   else if (k <= 4)
     hipLaunchKernelGGL((k_mdot<4, T>), RB, 256, 0, s, n, k, x, Y, partial);
-  else
+  else if (k <= 8)
     hipLaunchKernelGGL((k_mdot<8, T>), RB, 256, 0, s, n, k, x, Y, partial);
+  else
+    hipLaunchKernelGGL((k_mdot<16, T>), RB, 256, 0, s, n, k, x, Y, partial);
   hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, k, partial, out);
   UWG_HIP(hipGetLastError());
 }
+
+// Classical Gram-Schmidt update on the device, with the coefficients left in device memory:
+// w -= sum_j h[j] V[j] (accumulated in double, as k_multi_axpy with c = -h) fused with the
+// partial sums of |w|^2 over the same grid and in the same order as the one-vector k_mdot, so
+// that out = |w|^2 equals a separate dot product bit for bit.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gs_update(long n, int k, const T* const* V, const double* __restrict__ h,
+                                                   T* __restrict__ w, double* partial) {
+  __shared__ double sm[256];
+  double sq = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double acc = (double)w[i];
+    for (int j = 0; j < k; j++) acc += (-h[j]) * (double)V[j][i];
+    const T wn = (T)acc;
+    w[i] = wn;
+    sq += (double)wn * (double)wn;
+  }
+  sm[threadIdx.x] = sq;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+// v = w / sqrt(max(0, *dot)) (the scale rounded to T as vk_scale takes it); nothing is written
+// when the norm is zero
+template <typename T>
+__global__ void k_scale_by_norm(long n, const double* __restrict__ dot, const T* __restrict__ w, T* __restrict__ v) {
+  const double hn = sqrt(fmax(0.0, *dot));
+  if (!(hn > 0.0)) return;
+  const T a = (T)(1.0 / hn);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = a * w[i];
+}
+
+template <typename T>
+void vk_gs_step(long n, int k, const T* const* V, T* w, T* v_next, double* h, double* partial, hipStream_t s) {
+  for (int j0 = 0; j0 < k; j0 += 16) vk_multi_dot<T>(n, std::min(16, k - j0), w, V + j0, partial, h + j0, s);
+  hipLaunchKernelGGL(k_gs_update<T>, RB, 256, 0, s, n, k, V, h, w, partial);
+  hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, 1, partial, h + k);
+  hipLaunchKernelGGL(k_scale_by_norm<T>, vgrid(n), 256, 0, s, n, h + k, w, v_next);
+  UWG_HIP(hipGetLastError());
+}
+template void vk_gs_step<double>(long, int, const double* const*, double*, double*, double*, double*, hipStream_t);
+template void vk_gs_step<float>(long, int, const float* const*, float*, float*, double*, double*, hipStream_t);
 template void vk_multi_dot<double>(long, int, const double*, const double* const*, double*, double*, hipStream_t);
 template void vk_multi_dot<float>(long, int, const float*, const float* const*, double*, double*, hipStream_t);
 

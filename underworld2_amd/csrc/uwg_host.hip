@@ -175,6 +175,7 @@ struct uwg_stokes {
   double* red_partial = nullptr;
   double* red_out = nullptr;
   double* dcoef = nullptr;
+  double* dh = nullptr;  // Gram-Schmidt coefficients of the current Arnoldi step (device)
   const double** dptrs = nullptr;
   int levels_req = 0;
   // learned correction
@@ -1370,16 +1371,6 @@ static double dev_dot(uwg_stokes* st, long n, const T* a, const T* b) {
   return r;
 }
 
-// multi-dot against K.V[0..k-1] in groups of 8
-template <typename T>
-static void dev_mdot(uwg_stokes* st, KSpaceT<T>& K, int k, const T* x, double* out) {
-  for (int j0 = 0; j0 < k; j0 += 8) {
-    int kk = std::min(8, k - j0);
-    vk_multi_dot<T>(K.n, kk, x, K.dV + j0, st->red_partial, st->red_out, 0);
-    UWG_HIP(hipMemcpy(out + j0, st->red_out, sizeof(double) * kk, hipMemcpyDeviceToHost));
-  }
-}
-
 // Right-preconditioned FGMRES with classical Gram-Schmidt; vectors of type T, every
 // reduction and the Hessenberg least-squares problem in double.
 template <typename T>
@@ -1415,14 +1406,11 @@ static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const 
     for (; j < m && its < maxit; j++) {
       M(K.V[j], K.Z[j]);
       A(K.Z[j], K.w);
-      dev_mdot<T>(st, K, j + 1, K.w, hcol.data());
-      for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
-      UWG_HIP(hipMemcpy(st->dcoef, hcol.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice));
-      vk_multi_axpy<T>(n, j + 1, K.dV, st->dcoef, K.w, 0);
-      for (int i = 0; i <= j; i++) hcol[i] = -hcol[i];
-      double hn = sqrt(std::max(0.0, dev_dot<T>(st, n, K.w, K.w)));
+      // Gram-Schmidt on the device; one copy of the Hessenberg column per iteration
+      vk_gs_step<T>(n, j + 1, K.dV, K.w, K.V[j + 1], st->dh, st->red_partial, 0);
+      UWG_HIP(hipMemcpy(hcol.data(), st->dh, sizeof(double) * (j + 2), hipMemcpyDeviceToHost));
+      double hn = sqrt(std::max(0.0, hcol[j + 1]));
       hcol[j + 1] = hn;
-      if (hn > 0) vk_scale<T>(n, (T)(1.0 / hn), K.w, K.V[j + 1], 0);
       // Givens
       for (int i = 0; i < j; i++) {
         double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
@@ -1540,9 +1528,10 @@ extern "C" int uwg_stokes_create(uwg_field* vel, uwg_field* p, const unsigned ch
   st->ubc = dalloc<double>(st->nu);
   st->approxS = dalloc<double>(g.E);
   st->h = dalloc<double>(st->np);
-  st->red_partial = dalloc<double>(8 * vk_reduce_blocks());
+  st->red_partial = dalloc<double>(16 * vk_reduce_blocks());
   st->red_out = dalloc<double>(8);
   st->dcoef = dalloc<double>(256);
+  st->dh = dalloc<double>(256);
   UWG_HIP(hipMalloc(&st->dptrs, sizeof(double*) * 256));
   *out = st;
   return 0;
@@ -1572,7 +1561,7 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
   for (int k = 0; k < WS_COUNT; k++) dfree(st->ws_ptr[k]);
   for (void* p : {(void*)st->fe, (void*)st->f, (void*)st->ubc, (void*)st->approxS, (void*)st->h, (void*)st->eta_tab,
                   (void*)st->force_tab, (void*)st->eta_p, (void*)st->force_p, (void*)st->red_partial,
-                  (void*)st->red_out, (void*)st->dcoef, (void*)st->dptrs, (void*)st->coarse.Ainv})
+                  (void*)st->red_out, (void*)st->dcoef, (void*)st->dh, (void*)st->dptrs, (void*)st->coarse.Ainv})
     dfree(p);
   delete st;
   return 0;

@@ -198,6 +198,11 @@ void vk_multi_axpy(long n, int k, const T* const* X, const double* c, T* y, hipS
 // partial sums for dot products: out[j] = <x, Y[j]> for j < k ; deterministic
 template <typename T>
 void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s);
+// one classical Gram-Schmidt step with device-resident coefficients: h[0..k) = V^T w,
+// w -= sum h_j V_j, h[k] = |w|^2 (bit-identical to a separate dot), v_next = w / |w| (skipped
+// when |w| = 0); partial holds 16 x vk_reduce_blocks() doubles
+template <typename T>
+void vk_gs_step(long n, int k, const T* const* V, T* w, T* v_next, double* h, double* partial, hipStream_t s);
 int vk_reduce_blocks();
 // chebyshev helper: d = a*d + b*(dinv .* r) ; x += d  (fused)
 template <typename T>
