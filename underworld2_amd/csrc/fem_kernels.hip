@@ -1266,13 +1266,25 @@ __global__ void k_apply_GT(int rx, int ry, int rz, long E, GeConst Ge, const T* 
   long nx = rx + 1, ny = ry + 1;
   int i = (int)(e % rx), j = (int)((e / rx) % ry), k = DIM == 3 ? (int)(e / ((long)rx * ry)) : 0;
   long base = i + nx * (j + ny * k);
-  T acc = 0;
-  for (int a = 0; a < (DIM == 3 ? 8 : 4); a++) {
-    long n = base + (a & 1) + ((a >> 1) & 1) * nx + (DIM == 3 ? ((a >> 2) & 1) * nx * ny : 0);
-    uint8_t m = bc[n];
-    for (int d = 0; d < DIM; d++)
-      if (!((m >> d) & 1)) acc += (T)Ge.v[a * DIM + d] * u[n * DIM + d];
+  constexpr int NN = DIM == 3 ? 8 : 4;
+  // every load first (the Dirichlet mask selects afterwards, so no load waits on another)
+  uint8_t m[NN];
+  T uv[NN][DIM];
+#pragma unroll
+  for (int a = 0; a < NN; a++) {
+    const long n = base + (a & 1) + ((a >> 1) & 1) * nx + (DIM == 3 ? ((a >> 2) & 1) * nx * ny : 0);
+    m[a] = bc[n];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) uv[a][d] = u[n * DIM + d];
   }
+  T acc = 0;
+#pragma unroll
+  for (int a = 0; a < NN; a++)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const T t = (T)Ge.v[a * DIM + d] * uv[a][d];
+      acc = ((m[a] >> d) & 1) ? acc : acc + t;
+    }
   q[e] = acc;
 }
 
@@ -1656,23 +1668,41 @@ __global__ __launch_bounds__(256) void k_mdot(long n, int k, const T* __restrict
   }
   if (threadIdx.x < KM && threadIdx.x < k) partial[threadIdx.x * RB + blockIdx.x] = sm[threadIdx.x][0];
 }
-__global__ void k_mdot_final(int k, const double* partial, double* out) {
+// x.x with the one-vector k_mdot's grid, order and partial layout (bit-identical to it)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sqnorm(long n, const T* __restrict__ x, double* partial) {
   __shared__ double sm[256];
-  for (int j = 0; j < k; j++) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < RB; b += 256) a += partial[j * RB + b];
-    sm[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[j] = sm[0];
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double xv = (double)x[i];
+    acc += xv * (double)x[i];
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+// one block per dot product (block j sums the RB partials of dot j in a fixed order)
+__global__ void k_mdot_final(const double* partial, double* out) {
+  __shared__ double sm[256];
+  const int j = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < RB; b += 256) a += partial[j * RB + b];
+  sm[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sm[0];
 }
 template <typename T>
 void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s) {
+  if (k < 1) return;
   if (k <= 1)
     hipLaunchKernelGGL((k_mdot<1, T>), RB, 256, 0, s, n, k, x, Y, partial);
   else if (k <= 4)
@@ -1681,33 +1711,19 @@ void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial,
     hipLaunchKernelGGL((k_mdot<8, T>), RB, 256, 0, s, n, k, x, Y, partial);
   else
     hipLaunchKernelGGL((k_mdot<16, T>), RB, 256, 0, s, n, k, x, Y, partial);
-  hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, k, partial, out);
+  hipLaunchKernelGGL(k_mdot_final, k, 256, 0, s, partial, out);
   UWG_HIP(hipGetLastError());
 }
 
-// Classical Gram-Schmidt update on the device, with the coefficients left in device memory:
-// w -= sum_j h[j] V[j] (accumulated in double, as k_multi_axpy with c = -h) fused with the
-// partial sums of |w|^2 over the same grid and in the same order as the one-vector k_mdot, so
-// that out = |w|^2 equals a separate dot product bit for bit.
+// Classical Gram-Schmidt update on the device with the coefficients left in device memory:
+// w -= sum_j h[j] V[j], accumulated in double (k_multi_axpy with c = -h).
 template <typename T>
-__global__ __launch_bounds__(256) void k_gs_update(long n, int k, const T* const* V, const double* __restrict__ h,
-                                                   T* __restrict__ w, double* partial) {
-  __shared__ double sm[256];
-  double sq = 0.0;
+__global__ void k_gs_update(long n, int k, const T* const* V, const double* __restrict__ h, T* __restrict__ w) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double acc = (double)w[i];
     for (int j = 0; j < k; j++) acc += (-h[j]) * (double)V[j][i];
-    const T wn = (T)acc;
-    w[i] = wn;
-    sq += (double)wn * (double)wn;
+    w[i] = (T)acc;
   }
-  sm[threadIdx.x] = sq;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
 }
 
 // v = w / sqrt(max(0, *dot)) (the scale rounded to T as vk_scale takes it); nothing is written
@@ -1720,11 +1736,14 @@ __global__ void k_scale_by_norm(long n, const double* __restrict__ dot, const T*
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = a * w[i];
 }
 
+// The update runs on the streaming grid (a reduction-sized grid holds too few loads in flight
+// for the k-vector gather); |w|^2 is then the one-vector dot of vk_multi_dot.
 template <typename T>
 void vk_gs_step(long n, int k, const T* const* V, T* w, T* v_next, double* h, double* partial, hipStream_t s) {
   for (int j0 = 0; j0 < k; j0 += 16) vk_multi_dot<T>(n, std::min(16, k - j0), w, V + j0, partial, h + j0, s);
-  hipLaunchKernelGGL(k_gs_update<T>, RB, 256, 0, s, n, k, V, h, w, partial);
-  hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, 1, partial, h + k);
+  hipLaunchKernelGGL(k_gs_update<T>, vgrid(n), 256, 0, s, n, k, V, h, w);
+  hipLaunchKernelGGL(k_sqnorm<T>, RB, 256, 0, s, n, w, partial);
+  hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, partial, h + k);
   hipLaunchKernelGGL(k_scale_by_norm<T>, vgrid(n), 256, 0, s, n, h + k, w, v_next);
   UWG_HIP(hipGetLastError());
 }
