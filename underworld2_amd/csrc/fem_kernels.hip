@@ -1338,9 +1338,12 @@ __global__ void k_prolong_add(int fnx, int fny, int fnz, int cnx, int cny, long 
         long cn = ci + (long)cnx * (cj + (long)cny * ck);
         for (int d = 0; d < DIM; d++) acc[d] += xc[cn * DIM + d];
       }
-  uint8_t m = bcf[n];
-  for (int d = 0; d < DIM; d++)
-    if (!((m >> d) & 1)) xf[n * DIM + d] += wgt * acc[d];
+  // the mask and the old values load independently of each other; a Dirichlet dof is
+  // rewritten with its own value
+  const uint8_t m = bcf[n];
+  T old[3];
+  for (int d = 0; d < DIM; d++) old[d] = xf[n * DIM + d];
+  for (int d = 0; d < DIM; d++) xf[n * DIM + d] = ((m >> d) & 1) ? old[d] : old[d] + wgt * acc[d];
 }
 
 template <typename T, int DIM>
@@ -1350,23 +1353,27 @@ __global__ void k_restrict(int fnx, int fny, int fnz, int cnx, int cny, int cnz,
   if (n >= Nc) return;
   int i = (int)(n % cnx), j = (int)((n / cnx) % cny), k = (int)(n / ((long)cnx * cny));
   T acc[3] = {0, 0, 0};
+  // every fine value and mask is loaded unconditionally (out-of-grid neighbours read node 0
+  // and are skipped by the select), so no load waits on a mask byte
+#pragma unroll
   for (int oz = (DIM == 3 ? -1 : 0); oz <= (DIM == 3 ? 1 : 0); oz++) {
-    int K = 2 * k + oz;
-    if (K < 0 || K >= fnz) continue;
+    const int K = 2 * k + oz;
+#pragma unroll
     for (int oy = -1; oy <= 1; oy++) {
-      int J = 2 * j + oy;
-      if (J < 0 || J >= fny) continue;
+      const int J = 2 * j + oy;
+#pragma unroll
       for (int ox = -1; ox <= 1; ox++) {
-        int I = 2 * i + ox;
-        if (I < 0 || I >= fnx) continue;
+        const int I = 2 * i + ox;
+        const bool in = K >= 0 && K < fnz && J >= 0 && J < fny && I >= 0 && I < fnx;
         T w = (T)1;
         if (ox) w *= (T)0.5;
         if (oy) w *= (T)0.5;
         if (oz) w *= (T)0.5;
-        long fn = I + (long)fnx * (J + (long)fny * K);
-        uint8_t mf = bcf[fn];
-        for (int d = 0; d < DIM; d++)
-          if (!((mf >> d) & 1)) acc[d] += w * rf[fn * DIM + d];
+        const long fn = in ? I + (long)fnx * (J + (long)fny * K) : 0;
+        const uint8_t mf = bcf[fn];
+        T v[3];
+        for (int d = 0; d < DIM; d++) v[d] = rf[fn * DIM + d];
+        for (int d = 0; d < DIM; d++) acc[d] = (in && !((mf >> d) & 1)) ? acc[d] + w * v[d] : acc[d];
       }
     }
   }
