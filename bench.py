@@ -335,8 +335,8 @@ def main():
                          "too long for one process can be split by mode)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "not part of it)"}), flush=True)
+        print(json.dumps({"error": "the engine runs on one GPU; --gpus must be 1 (multi-GPU sharding of the "
+                                   "element grid is not implemented)"}), flush=True)
         sys.exit(2)
     import underworld2_amd as uw
     log(uw.device_info())
