@@ -177,8 +177,8 @@ int uwg_stokes_levels(uwg_stokes *st, int *nlev, int *res_out /*[nlev][3]*/);
  * [E][dim+1][dim+1] (UzawaPreconditionerTerm.c:124-209) */
 int uwg_stokes_get_assembly(uwg_stokes *st, int level, double *W, double *fe, double *approxS);
 int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
-/* ---- learned coarse correction (no counterpart in the reference: the preconditioner
- * extension named by the build's method).  The network acts on the nodes of one multigrid
+/* ---- learned coarse correction (no counterpart in the reference: a convolutional network
+ * standing in for, or correcting, the multigrid coarse correction).  The network acts on the nodes of one multigrid
  * level; mode 0 replaces the coarse cycle below that level, mode 1 adds a learned
  * correction of the coarse cycle's remaining error.  FP32 cycles only (precision 1). */
 int uwg_stokes_load_nn(uwg_stokes *st, int level, int nparams, const float *params, int channels, int depth);
