@@ -734,35 +734,18 @@ struct SmoothArgs {
   T* d = nullptr;
   T a = 0, c = 0;
 };
-template <typename T, int MODE>
-__device__ __forceinline__ void apply_epilogue(const SmoothArgs<T>& S, const T* __restrict__ x, T* __restrict__ y,
-                                               long k, T kx) {
-  if (MODE == APPLY_Y) {
-    y[k] = kx;
-  } else {
-    const T r = S.b[k] - kx;
-    if (MODE == APPLY_RESIDUAL) {
-      y[k] = r;
-    } else {
-      const T v = MODE == APPLY_CHEB ? S.a * S.d[k] + S.c * (S.dinv[k] * r) : S.c * (S.dinv[k] * r);
-      S.d[k] = v;
-      y[k] = x[k] + v;
-    }
-  }
-}
-
 // Operands of the owned node's update, loaded at the top of a level so that their latency
 // overlaps the element work.
-template <typename T, int MODE>
+template <typename T, int MODE, int NC = 3>
 struct NodeOperands {
   uint8_t m;
-  T x[3], b[3], dinv[3], d[3];
+  T x[NC], b[NC], dinv[NC], d[NC];
   __device__ __forceinline__ void load(const SmoothArgs<T>& S, const T* __restrict__ xs, const uint8_t* __restrict__ bc,
                                        uint32_t n) {
     m = ld_off(bc, n);
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const uint32_t o = (n * 3u + i) * (uint32_t)sizeof(T);
+    for (int i = 0; i < NC; i++) {
+      const uint32_t o = (n * (uint32_t)NC + i) * (uint32_t)sizeof(T);
       x[i] = ld_off(xs, o);
       if (MODE != APPLY_Y) b[i] = ld_off(S.b, o);
       if (MODE == APPLY_CHEB || MODE == APPLY_CHEB0) dinv[i] = ld_off(S.dinv, o);
@@ -771,7 +754,7 @@ struct NodeOperands {
   }
   // kx: the operator row sum of component i (replaced by x on a Dirichlet row)
   __device__ __forceinline__ void store(const SmoothArgs<T>& S, T* __restrict__ y, uint32_t n, int i, T kx) const {
-    const uint32_t o = (n * 3u + i) * (uint32_t)sizeof(T);
+    const uint32_t o = (n * (uint32_t)NC + i) * (uint32_t)sizeof(T);
     if ((m >> i) & 1) kx = x[i];
     if (MODE == APPLY_Y) {
       st_off(y, o, kx);
@@ -1041,19 +1024,19 @@ __device__ __forceinline__ void elem2(const T* __restrict__ W, long E, long e, c
 template <typename T>
 __device__ __forceinline__ void load_level2(const KArgs<T>& A, const T* __restrict__ x, const uint8_t* __restrict__ bc,
                                             int ei, int lev, T (&u)[4][2], int off) {
+  // unconditional loads (absent nodes read node 0), selected afterwards: no load waits on
+  // the mask byte
 #pragma unroll
   for (int c = 0; c < 2; c++) {
-    int ni = ei + c;
-    bool ok = ni >= 0 && ni <= A.rx && lev >= 0 && lev <= A.ry;
-    if (ok) {
-      long n = ni + A.nx * lev;
-      uint8_t m = bc[n];
+    const int ni = ei + c;
+    const bool ok = ni >= 0 && ni <= A.rx && lev >= 0 && lev <= A.ry;
+    const long n = ok ? ni + A.nx * lev : 0;
+    const uint8_t m = bc[n];
+    T v[2];
 #pragma unroll
-      for (int i = 0; i < 2; i++) u[off + c][i] = ((m >> i) & 1) ? (T)0 : x[n * 2 + i];
-    } else {
+    for (int i = 0; i < 2; i++) v[i] = x[n * 2 + i];
 #pragma unroll
-      for (int i = 0; i < 2; i++) u[off + c][i] = (T)0;
-    }
+    for (int i = 0; i < 2; i++) u[off + c][i] = (ok && !((m >> i) & 1)) ? v[i] : (T)0;
   }
 }
 
@@ -1071,6 +1054,11 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
   T u[4][2];
   load_level2(A, x, bc, ei, k0 - 1, u, 0);
   for (int ek = k0 - 1; ek < k1; ++ek) {
+    // the owned node's operands and the next node level are loaded before the element work
+    const bool doN = ownN && ek >= k0;
+    const uint32_t nN = doN ? (uint32_t)(ei + A.nx * ek) : 0u;
+    NodeOperands<T, MODE, 2> nd;
+    nd.load(S, x, bc, nN);
     load_level2(A, x, bc, ei, ek + 1, u, 2);
     T r[4][2];
     if (colE && ek >= 0 && ek < A.ry) {
@@ -1085,14 +1073,10 @@ __global__ __launch_bounds__(64) void k_apply2(KArgs<T> A, const T* __restrict__
 #pragma unroll
       for (int i = 0; i < 2; i++) sb[c * 2 + i][tid] = r[c][i];
     __syncthreads();
-    if (ownN && ek >= k0) {
-      long n = ei + A.nx * ek;
-      uint8_t m = bc[n];
+    if (doN) {
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
-        T s = st[2 + i][tid - 1] + st[i][tid] + sb[2 + i][tid - 1] + sb[i][tid];
-        apply_epilogue<T, MODE>(S, x, y, n * 2 + i, ((m >> i) & 1) ? x[n * 2 + i] : s);
-      }
+      for (int i = 0; i < 2; i++)
+        nd.store(S, y, nN, i, st[2 + i][tid - 1] + st[i][tid] + sb[2 + i][tid - 1] + sb[i][tid]);
     }
     __syncthreads();
 #pragma unroll
@@ -1176,6 +1160,8 @@ static void launch_apply_mode(const Geom& g, const T* W, const T* x, T* y, const
     } else
       hipLaunchKernelGGL((k_apply3<T, MODE>), dim3(nwg), 256, 0, s, A, W, x, y, bc, S);
   } else {
+    if (2.0 * g.N * sizeof(T) >= 4294967296.0)
+      throw std::runtime_error("2D K apply: node vectors must stay below 4 GiB");
     A.KZ = g_kz2;
     dim3 grid((g.res[0] + 1 + 62) / 63, (g.res[1] + 1 + A.KZ - 1) / A.KZ);
     hipLaunchKernelGGL((k_apply2<T, MODE>), grid, 64, 0, s, A, W, x, y, bc, S);
