@@ -313,7 +313,9 @@ __global__ __launch_bounds__(256) void k_q2_apply_mfma(Q2G G, int cx, int cy, in
       T v = (T)0;
       if (r < 3 * Q2M_E && a < 27 && ce0 + le < nce) {
         const long n = q2_vertex<3>(G, elem_of(ce0 + le), a);
-        v = ((bc[n] >> i) & 1) ? (T)0 : x[n * 3 + i];
+        const uint8_t m = bc[n];
+        const T xv = x[n * 3 + i];  // loaded unconditionally: the mask does not gate the load
+        v = ((m >> i) & 1) ? (T)0 : xv;
       }
       Xs[r][a] = v;
     }
@@ -444,7 +446,9 @@ __global__ __launch_bounds__(128) void k_q2_apply(Q2G G, int cx, int cy, int cz,
     if (ce < nce) {
       const long n = q2_vertex<DIM>(G, elem_of(ce), r / DIM);
       const int i = r % DIM;
-      v = ((bc[n] >> i) & 1) ? (T)0 : x[n * DIM + i];
+      const uint8_t m = bc[n];
+      const T xv = x[n * DIM + i];  // loaded unconditionally: the mask does not gate the load
+      v = ((m >> i) & 1) ? (T)0 : xv;
     }
     xs[le][r] = v;
   }
