@@ -13,11 +13,15 @@ Sources (relative to the reference tree, LU/ = src/underworld/libUnderworld/):
                               PY/systems/_timeintegration.py:131-147 (RK2) documented outputs
   solcx.npz                   Velic solCx closed form (LU/Underworld/SysTest/AnalyticPlugins/Velic_solCx),
                               evaluated with oracle/_ref/libsolcx.so built from those sources
+  tempfield_inp_64_Ra1e4.h5   docs/examples/input/03_BlankenbachBenchmark/tempfield_inp_64_Ra1e4.h5, copied
+                              as is: the steady-state temperature field (64^2 Q1, float32) that
+                              docs/examples/03_BlankenbachBenchmark.ipynb loads with MeshVariable.load
 """
 import ctypes
 import json
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
@@ -131,6 +135,8 @@ def main():
         np.savez(os.path.join(HERE, "solcx.npz"), **s)
     else:
         print("oracle/_ref/libsolcx.so not built; solcx.npz not written", file=sys.stderr)
+    shutil.copyfile(os.path.join(REF, "docs/examples/input/03_BlankenbachBenchmark/tempfield_inp_64_Ra1e4.h5"),
+                    os.path.join(HERE, "tempfield_inp_64_Ra1e4.h5"))
     print("fixtures written")
 
 

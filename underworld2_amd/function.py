@@ -60,9 +60,11 @@ class Function:
         raise NotImplementedError
 
     def evaluate(self, inp):
-        """Evaluate at a swarm (its particles) or at an array of coordinates."""
+        """Evaluate at a swarm (its particles), a mesh (its nodes) or an array of coordinates."""
         if hasattr(inp, "particleCoordinates"):
             ctx = _Ctx(swarm=inp)
+        elif hasattr(inp, "elementRes") and hasattr(inp, "data"):
+            ctx = _Ctx(coords=np.array(inp.data, dtype=np.float64))
         else:
             ctx = _Ctx(coords=np.atleast_2d(np.asarray(inp, dtype=np.float64)))
         return np.asarray(self._fn_eval(ctx))

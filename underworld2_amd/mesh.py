@@ -14,11 +14,24 @@ element-major, at the element-local points of Inner2DGenerator.c:286-353.
 view is refreshed on access and pushed back before the next device operation.
 """
 import ctypes
+import os as _os
 
 import numpy as np
 
+from . import _h5
 from . import _lib as L
 from . import function as _fn
+
+
+def _check_no_units(attrs, filename):
+    """Files whose 'units' attribute names a unit hold dimensional data (PY/scaling); this engine
+    works in model units only, so those are refused rather than misread."""
+    u = attrs.get("units")
+    if u is not None:
+        u = u.decode() if isinstance(u, bytes) else str(u)
+        if u not in ("None", ""):
+            raise NotImplementedError(f"'{filename}' holds data in units '{u}'; dimensional (pint) data are "
+                                      "not provided by this engine")
 
 
 class IndexSet:
@@ -148,6 +161,62 @@ class FeMesh_Cartesian:
             self._special = sets
         return self._special
 
+    def _element_nodes(self):
+        """[elements][(order+1)^dim] global node ids; each element's nodes x-fastest over its node
+        grid (the Q1 order of CartesianGenerator.c:743-800, the Q2 order of C2Generator.c)."""
+        d, o = self.dim, self._order
+        nn = np.array(self._nn, dtype=np.int64)
+        stride = np.concatenate([[1], np.cumprod(nn[:-1])])
+        eidx = np.indices(self.elementRes[::-1]).reshape(d, -1)[::-1].astype(np.int64)  # x fastest
+        lidx = np.indices((o + 1,) * d).reshape(d, -1)[::-1].astype(np.int64)
+        ids = sum((o * eidx[k][:, None] + lidx[k][None, :]) * stride[k] for k in range(d))
+        return ids.astype(np.uint32)
+
+    def save(self, filename, units=None, **kwargs):
+        """Mesh.save (PY/mesh/_mesh.py:560-600): root attributes 'dimensions', 'mesh resolution',
+        'max', 'min', 'regular', 'elementType' and each keyword as a string attribute; datasets
+        'vertices' [nodes][dim] (float64) and 'en_map' [elements][nodes per element] (uint32)."""
+        if not isinstance(filename, str):
+            raise TypeError("'filename', must be of type 'str'")
+        if units:
+            raise NotImplementedError("dimensional (pint) output is not provided by this engine")
+        with _h5.File(filename, "w") as f:
+            f.attrs["dimensions"] = self.dim
+            f.attrs["mesh resolution"] = np.array(self.elementRes, dtype=np.int64)
+            f.attrs["max"] = np.array(self.maxCoord)
+            f.attrs["min"] = np.array(self.minCoord)
+            f.attrs["regular"] = np.bool_(True)
+            f.attrs["elementType"] = self.elementType
+            for k, v in kwargs.items():
+                f.attrs[str(k)] = str(v)
+            f.write_dataset("vertices", self.data)
+            f.write_dataset("en_map", self._element_nodes())
+        return _h5.SavedFileData(self, filename)
+
+    def load(self, filename):
+        """Mesh.load (PY/mesh/_mesh.py:602-680): the file's resolution must equal this mesh's and
+        its vertex array must have this mesh's shape.  The engine's meshes are regular boxes, so
+        vertices that differ from this mesh's own (a deformed mesh) are refused."""
+        if not isinstance(filename, str):
+            raise TypeError("Expected filename to be provided as a string")
+        with _h5.File(filename, "r") as f:
+            res = f.attrs.get("mesh resolution")
+            if res is None:
+                raise RuntimeError("Can't read the 'mesh resolution' for the field hdf5 file, was it created correctly?")
+            if not np.array_equal(np.asarray(res).ravel(), np.array(self.elementRes)):
+                raise RuntimeError("Provided file mesh resolution does not appear to correspond to\n"
+                                   "resolution of mesh object.")
+            _check_no_units(f.attrs, filename)
+            shape = f.dataset_shape("vertices")
+            if len(shape) != 2 or shape[1] != self.dim:
+                raise RuntimeError("Can't load hdf5 '{0}', incompatible data shape".format(filename))
+            if shape[0] != self.nodesGlobal:
+                raise RuntimeError("Provided data file appears to be for a different resolution mesh.")
+            verts = f.read_dataset("vertices", np.float64)
+        span = np.array(self.maxCoord) - np.array(self.minCoord)
+        if np.abs(verts - self.data).max(initial=0.0) > 1e-12 * span.max():
+            raise NotImplementedError("the file holds a deformed mesh; this engine's meshes are regular boxes")
+
     def _min_separation(self):
         v = ctypes.c_double()
         L.check(L.lib().uwg_mesh_min_separation(self._h, ctypes.byref(v)))
@@ -197,6 +266,79 @@ class MeshVariable(_fn.Function):
     @data.setter
     def data(self, value):
         self.data[...] = value
+
+    def save(self, filename, meshHandle=None, units=None, **kwargs):
+        """MeshVariable.save (PY/mesh/_meshvariable.py:330-470): dataset 'data' [nodes][dof]
+        (float64, global node order), attributes 'units' (the string 'None'), 'elementType'
+        (fixed-length ASCII) and each keyword as a string; with ``meshHandle`` (what
+        ``mesh.save`` returned, or a file name) an external link 'mesh' to that file's root."""
+        if not isinstance(filename, str):
+            raise TypeError("Expected 'filename' to be provided as a string")
+        if units:
+            raise NotImplementedError("dimensional (pint) output is not provided by this engine")
+        mesh_file = None
+        if meshHandle:
+            if not isinstance(meshHandle, (str, _h5.SavedFileData)):
+                raise TypeError("Expected 'meshHandle' to be of type 'uw.utils.SavedFileData'")
+            mesh_file = meshHandle if isinstance(meshHandle, str) else meshHandle.filename
+            if not _os.path.exists(mesh_file):
+                raise ValueError("You are trying to link against the mesh file '{}'\n"
+                                 "that does not appear to exist. If you need to link \n"
+                                 "against a mesh file, please make sure it is created first.".format(mesh_file))
+        data = np.array(self.data, dtype=np.float64)
+        with _h5.File(filename, "w") as f:
+            f.write_dataset("data", data)
+            for k, v in kwargs.items():
+                f.attrs[str(k)] = str(v)
+            f.attrs["units"] = str(None)
+            f.attrs["elementType"] = np.bytes_(self.mesh.elementType.encode())
+            if mesh_file is not None:
+                f.external_link("mesh", mesh_file, ".")
+        return _h5.SavedFileData(self, filename)
+
+    def load(self, filename, interpolate=False):
+        """MeshVariable.load (PY/mesh/_meshvariable.py:472-600).  A file with this variable's node
+        count is copied (converted to float64); another resolution needs ``interpolate=True`` and
+        the file's 'mesh' link, from whose attributes the file's regular mesh is rebuilt and the
+        file field interpolated at this variable's nodes."""
+        if not isinstance(filename, str):
+            raise TypeError("Expected filename to be provided as a string")
+        with _h5.File(filename, "r") as f:
+            shape = f.dataset_shape("data")
+            if len(shape) != 2 or shape[1] != self.nodeDofCount:
+                raise RuntimeError("Can't load hdf5 '{0}', incompatible data shape".format(filename))
+            _check_no_units(f.attrs, filename)
+            if shape[0] == self.mesh.nodesGlobal:
+                self.data[:] = f.read_dataset("data", np.float64)
+                return
+            if not interpolate:
+                raise RuntimeError("Provided data file appears to be for a different resolution MeshVariable.\n"
+                                   "If you would like to interpolate the data to the current variable, please set\n"
+                                   "the 'interpolate' parameter. Check docstring for important caveats of "
+                                   "interpolation method.")
+            m = f.open_linked("mesh")
+            if m is None:
+                raise RuntimeError("The hdf5 field to be loaded with interpolation must have an associated "
+                                   "'mesh' hdf5 file. Resave the field with its associated mesh."
+                                   "i.e. myField.save(\"filename.h5\", meshFilename)")
+            res, hi, lo = (m.attrs.get(k) for k in ("mesh resolution", "max", "min"))
+            for val, k in ((res, "mesh resolution"), (hi, "max"), (lo, "min")):
+                if val is None:
+                    raise RuntimeError(f"Can't read the '{k}' for the field hdf5 file, was it created correctly?")
+            regular = m.attrs.get("regular")
+            if regular is not None and not bool(np.asarray(regular).ravel()[0]):
+                raise RuntimeError("Saved mesh file appears to correspond to a irregular mesh.\n"
+                                   "Interpolating from irregular mesh not currently supported.")
+            et = m.attrs.get("elementType")
+            et = "Q1" if et is None else (et.decode() if isinstance(et, bytes) else str(et))
+            src_mesh = FeMesh_Cartesian(et, tuple(int(r) for r in np.ravel(res)), tuple(np.ravel(lo)),
+                                        tuple(np.ravel(hi)))
+            if shape[0] != src_mesh.nodesGlobal:
+                raise RuntimeError("The saved mesh file can't be read onto the interpolation grid.\n"
+                                   "Note: only nodal (vertex) variables can be interpolated by this engine")
+            src = MeshVariable(src_mesh, self.nodeDofCount)
+            src.data[:] = f.read_dataset("data", np.float64)
+        self.data[:] = src.evaluate(self.mesh.data)
 
     # device synchronisation helpers used by the systems
     def _push(self):

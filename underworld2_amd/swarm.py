@@ -5,11 +5,41 @@ Particle coordinates and per-cell particle tables live on the GPU.  ``particleCo
 ``update_particle_owners()`` pushes the new coordinates back.
 """
 import ctypes
+import os as _os
 
 import numpy as np
 
+from . import _h5
 from . import _lib as L
 from . import function as _fn
+from .mesh import _check_no_units
+
+
+def _save_particle_data(filename, data, swarmHandle=None, units=None, **kwargs):
+    """SwarmVariable.save's layout (PY/swarm/_swarmvariable.py:419-500): dataset 'data'
+    [particles][components] in the variable's dtype, attributes 'proc_offset' (the one process's
+    particle count, int64[1]), 'units' (the string 'None') and each keyword as a string, and with
+    ``swarmHandle`` an external link 'swarm' to that swarm file."""
+    if not isinstance(filename, str):
+        raise TypeError("'filename' parameter must be of type 'str'")
+    if units:
+        raise NotImplementedError("dimensional (pint) output is not provided by this engine")
+    swarm_file = None
+    if swarmHandle is not None:
+        if not isinstance(swarmHandle, (str, _h5.SavedFileData)):
+            raise TypeError("Expected 'swarmHandle' to be of type 'uw.utils.SavedFileData'")
+        swarm_file = swarmHandle if isinstance(swarmHandle, str) else swarmHandle.filename
+        if not _os.path.exists(swarm_file):
+            raise ValueError("You are trying to link against the swarm file '{}'\n"
+                             "that does not appear to exist.".format(swarm_file))
+    with _h5.File(filename, "w") as f:
+        f.write_dataset("data", data)
+        if swarm_file is not None:
+            f.external_link("swarm", swarm_file, "./")
+        f.attrs["proc_offset"] = np.array([data.shape[0]], dtype=np.int64)
+        f.attrs["units"] = str(None)
+        for k, v in kwargs.items():
+            f.attrs[str(k)] = str(v)
 
 
 class _CoordVariable:
@@ -19,6 +49,10 @@ class _CoordVariable:
     @property
     def data(self):
         return self._swarm._coords_host()
+
+    def save(self, filename, collective=False, swarmHandle=None, units=None, **kwargs):
+        _save_particle_data(filename, np.array(self.data, dtype=np.float64), swarmHandle, units, **kwargs)
+        return _h5.SavedFileData(self, filename)
 
 
 class SwarmVariable(_fn.Function):
@@ -48,6 +82,38 @@ class SwarmVariable(_fn.Function):
             self._data = new
         self.version += 1
         return self._data
+
+    def save(self, filename, collective=False, swarmHandle=None, units=None, **kwargs):
+        """SwarmVariable.save (PY/swarm/_swarmvariable.py:364-500); ``collective`` has no effect
+        in one process."""
+        _save_particle_data(filename, np.array(self.data), swarmHandle, units, **kwargs)
+        return _h5.SavedFileData(self, filename)
+
+    def load(self, filename, collective=False):
+        """SwarmVariable.load (PY/swarm/_swarmvariable.py:244-362): after ``swarm.load`` of the
+        matching swarm file, row ``gid`` of the file's 'data' goes to the particle that swarm.load
+        created from row ``gid`` of the swarm file."""
+        if not isinstance(filename, str):
+            raise TypeError("'filename' parameter must be of type 'str'")
+        sw = self.swarm
+        if getattr(sw, "_checkpointMapsToState", None) != sw.stateId:
+            raise RuntimeError("'Swarm' associate with this 'SwarmVariable' does not appear to be in the correct "
+                               "state.\nPlease ensure that you have loaded the swarm prior to loading any swarm "
+                               "variables.")
+        gids = sw._local2globalMap
+        with _h5.File(filename, "r") as f:
+            shape = f.dataset_shape("data")
+            if len(shape) != 2 or shape[1] != self.count:
+                raise RuntimeError("Cannot load file data on current swarm. Data in file '{0}', has {1} components "
+                                   "-the particlesCoords has {2} components".format(filename, shape[-1], self.count))
+            if shape[0] != sw.particleGlobalCount:
+                raise RuntimeError("It appears that the swarm has {} particles, but provided h5 file has {} data "
+                                   "points. Please check that both the Swarm and the SwarmVariable were saved at "
+                                   "the same time, and that you have reloaded using the correct "
+                                   "files.".format(sw.particleGlobalCount, shape[0]))
+            vals = f.read_dataset("data", self._data.dtype)
+            _check_no_units(f.attrs, filename)
+        self.data[:len(gids)] = vals[gids]
 
     def _fn_inputs(self):
         return {("swarmvar", id(self))}
@@ -139,6 +205,37 @@ class Swarm:
         if rc < 0:
             raise RuntimeError(L.last_error())
         return np.arange(P0, P0 + c.shape[0], dtype=np.int32)
+
+    def save(self, filename, collective=False, units=None, **kwargs):
+        """Swarm.save (PY/swarm/_swarm.py:221-268): the particle coordinates in the layout of
+        ``SwarmVariable.save``."""
+        if not isinstance(filename, str):
+            raise TypeError("Expected filename to be provided as a string")
+        self.particleCoordinates.save(filename, collective, units=units, **kwargs)
+        return _h5.SavedFileData(self, filename)
+
+    def load(self, filename, collective=False, try_optimise=True):
+        """Swarm.load (PY/swarm/_swarm.py:270-420): *adds* the file's particles to the swarm.  Rows
+        whose coordinates lie outside the mesh are skipped, as the reference skips particles no
+        process owns; the file rows that were added are kept, in order, for ``SwarmVariable.load``."""
+        if not isinstance(filename, str):
+            raise TypeError("Expected 'filename' to be provided as a string")
+        dim = self.mesh.dim
+        with _h5.File(filename, "r") as f:
+            _check_no_units(f.attrs, filename)
+            shape = f.dataset_shape("data")
+            if len(shape) != 2 or shape[1] != dim:
+                raise RuntimeError("Cannot load file data on current swarm. Data in file '{0}', has {1} components "
+                                   "-the particlesCoords has {2} components".format(filename, shape[-1], dim))
+            xyz = f.read_dataset("data", np.float64)
+        lo = np.array(self.mesh.minCoord)
+        hi = np.array(self.mesh.maxCoord)
+        inside = np.all((xyz >= lo) & (xyz <= hi), axis=1)
+        valid = np.flatnonzero(inside).astype(np.int64)
+        if valid.size:
+            self.add_particles_with_coordinates(xyz[valid])
+        self._local2globalMap = valid
+        self._checkpointMapsToState = self.stateId
 
     def update_particle_owners(self):
         """Swarm.update_particle_owners (PY/swarm/_swarm.py:613-654)."""
