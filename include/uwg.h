@@ -223,6 +223,11 @@ int uwg_set_q2_mfma(int on);
  * element contractions take bf16 inputs, FP32 arithmetic and accumulation) or FP32 (0).
  * Process-wide; takes effect at the next solve. */
 int uwg_set_cycle_bf16(int on);
+/* Arnoldi basis of the coupled FP64 FGMRES at precision 1: FP32 storage (1, the default) or
+ * FP64 (0).  Dot products, the Hessenberg problem, the preconditioned vectors, the iterate and
+ * the stopping test stay FP64.  No counterpart in the reference (an engine extension).
+ * Process-wide; takes effect at the next solve. */
+int uwg_set_krylov_basis_fp32(int on);
 /* STREAM-style device copy of n doubles (reads 8n bytes, writes 8n bytes); ms per copy.
  * Measures attainable HBM bandwidth and calibrates byte counters. */
 int uwg_stream_copy(long n, int reps, double *ms);

@@ -118,6 +118,7 @@ SIGNATURES = {
     "uwg_nn_eval": [_vp, ctypes.c_int, _fp],
     "uwg_set_apply_chunk": [ctypes.c_int, ctypes.c_int],
     "uwg_set_cycle_bf16": [ctypes.c_int],
+    "uwg_set_krylov_basis_fp32": [ctypes.c_int],
     "uwg_stokes_time_apply": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
     "uwg_stream_copy": [ctypes.c_long, ctypes.c_int, _dp],
 }

@@ -1632,51 +1632,58 @@ void vk_multi_axpy(long n, int k, const T* const* X, const double* c, T* y, hipS
 template void vk_multi_axpy<double>(long, int, const double* const*, const double*, double*, hipStream_t);
 template void vk_multi_axpy<float>(long, int, const float* const*, const double*, float*, hipStream_t);
 
-// deterministic multi-dot: fixed grid of RB blocks, fixed-order tree in each block, then
-// one block sums the RB partials in order.  Products and sums in double for either input
-// precision (FP32 Krylov bases keep FP64 reductions).
+// deterministic multi-dot: fixed grid of RB blocks; in each block every wave reduces its
+// lanes by a fixed butterfly, the four wave sums are added in wave order, and one block then
+// sums the RB partials in order.  Products and sums in double for any input precision (an
+// FP32 Krylov basis keeps FP64 reductions).  Up to KM = 32 vectors share one pass over x.
 #define RB 1024
+#define KMAX 32
 int vk_reduce_blocks() { return RB; }
-template <int KM, typename T>
-__global__ __launch_bounds__(256) void k_mdot(long n, int k, const T* __restrict__ x, const T* const* Y,
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// the block's KM sums, in sm[KM][4] (per wave) then partial[j * RB + block] (thread j)
+template <int KM>
+__device__ inline void block_sums(double (&acc)[KM], int k, double (*sm)[4], double* partial) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < KM; j++) {
+    const double t = wave_sum(acc[j]);
+    if (lane == 0) sm[j][wv] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < KM && threadIdx.x < k) {
+    const int j = threadIdx.x;
+    partial[j * RB + blockIdx.x] = ((sm[j][0] + sm[j][1]) + sm[j][2]) + sm[j][3];
+  }
+}
+template <int KM, typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_mdot(long n, int k, const TX* __restrict__ x, const TY* const* Y,
                                               double* partial) {
-  __shared__ double sm[KM][256];
+  __shared__ double sm[KM][4];
   double acc[KM];
 #pragma unroll
   for (int j = 0; j < KM; j++) acc[j] = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double xv = (double)x[i];
+    const double xv = (double)x[i];
 #pragma unroll
     for (int j = 0; j < KM; j++)
       if (j < k) acc[j] += xv * (double)Y[j][i];
   }
-#pragma unroll
-  for (int j = 0; j < KM; j++) sm[j][threadIdx.x] = acc[j];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-#pragma unroll
-      for (int j = 0; j < KM; j++) sm[j][threadIdx.x] += sm[j][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x < KM && threadIdx.x < k) partial[threadIdx.x * RB + blockIdx.x] = sm[threadIdx.x][0];
+  block_sums<KM>(acc, k, sm, partial);
 }
 // x.x with the one-vector k_mdot's grid, order and partial layout (bit-identical to it)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sqnorm(long n, const T* __restrict__ x, double* partial) {
-  __shared__ double sm[256];
-  double acc = 0.0;
+  __shared__ double sm[1][4];
+  double acc[1] = {0.0};
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double xv = (double)x[i];
-    acc += xv * (double)x[i];
+    acc[0] += xv * (double)x[i];
   }
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+  block_sums<1>(acc, 1, sm, partial);
 }
 
 // one block per dot product (block j sums the RB partials of dot j in a fixed order)
@@ -1693,55 +1700,73 @@ __global__ void k_mdot_final(const double* partial, double* out) {
   }
   if (threadIdx.x == 0) out[j] = sm[0];
 }
+template <typename TX, typename TY>
+static void mdot_pass(long n, int k, const TX* x, const TY* const* Y, double* partial, hipStream_t s) {
+  if (k <= 1)
+    hipLaunchKernelGGL((k_mdot<1, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+  else if (k <= 4)
+    hipLaunchKernelGGL((k_mdot<4, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+  else if (k <= 8)
+    hipLaunchKernelGGL((k_mdot<8, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+  else if (k <= 16)
+    hipLaunchKernelGGL((k_mdot<16, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+  else
+    hipLaunchKernelGGL((k_mdot<32, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+}
 template <typename T>
 void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s) {
   if (k < 1) return;
-  if (k <= 1)
-    hipLaunchKernelGGL((k_mdot<1, T>), RB, 256, 0, s, n, k, x, Y, partial);
-  else if (k <= 4)
-    hipLaunchKernelGGL((k_mdot<4, T>), RB, 256, 0, s, n, k, x, Y, partial);
-  else if (k <= 8)
-    hipLaunchKernelGGL((k_mdot<8, T>), RB, 256, 0, s, n, k, x, Y, partial);
-  else
-    hipLaunchKernelGGL((k_mdot<16, T>), RB, 256, 0, s, n, k, x, Y, partial);
-  hipLaunchKernelGGL(k_mdot_final, k, 256, 0, s, partial, out);
+  for (int j0 = 0; j0 < k; j0 += KMAX) {
+    const int kk = std::min(KMAX, k - j0);
+    mdot_pass<T, T>(n, kk, x, Y + j0, partial, s);
+    hipLaunchKernelGGL(k_mdot_final, kk, 256, 0, s, partial, out + j0);
+  }
   UWG_HIP(hipGetLastError());
 }
 
 // Classical Gram-Schmidt update on the device with the coefficients left in device memory:
 // w -= sum_j h[j] V[j], accumulated in double (k_multi_axpy with c = -h).
-template <typename T>
-__global__ void k_gs_update(long n, int k, const T* const* V, const double* __restrict__ h, T* __restrict__ w) {
+template <typename TW, typename TV>
+__global__ void k_gs_update(long n, int k, const TV* const* V, const double* __restrict__ h, TW* __restrict__ w) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     double acc = (double)w[i];
     for (int j = 0; j < k; j++) acc += (-h[j]) * (double)V[j][i];
-    w[i] = (T)acc;
+    w[i] = (TW)acc;
   }
 }
 
-// v = w / sqrt(max(0, *dot)) (the scale rounded to T as vk_scale takes it); nothing is written
-// when the norm is zero
-template <typename T>
-__global__ void k_scale_by_norm(long n, const double* __restrict__ dot, const T* __restrict__ w, T* __restrict__ v) {
+// v = w / sqrt(max(0, *dot)) (the scale rounded to TW as vk_scale takes it, the product rounded
+// to the basis type TV); nothing is written when the norm is zero
+template <typename TW, typename TV>
+__global__ void k_scale_by_norm(long n, const double* __restrict__ dot, const TW* __restrict__ w,
+                                TV* __restrict__ v) {
   const double hn = sqrt(fmax(0.0, *dot));
   if (!(hn > 0.0)) return;
-  const T a = (T)(1.0 / hn);
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = a * w[i];
+  const TW a = (TW)(1.0 / hn);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    v[i] = (TV)(a * w[i]);
 }
 
 // The update runs on the streaming grid (a reduction-sized grid holds too few loads in flight
 // for the k-vector gather); |w|^2 is then the one-vector dot of vk_multi_dot.
-template <typename T>
-void vk_gs_step(long n, int k, const T* const* V, T* w, T* v_next, double* h, double* partial, hipStream_t s) {
-  for (int j0 = 0; j0 < k; j0 += 16) vk_multi_dot<T>(n, std::min(16, k - j0), w, V + j0, partial, h + j0, s);
-  hipLaunchKernelGGL(k_gs_update<T>, vgrid(n), 256, 0, s, n, k, V, h, w);
-  hipLaunchKernelGGL(k_sqnorm<T>, RB, 256, 0, s, n, w, partial);
+template <typename TW, typename TV>
+void vk_gs_step(long n, int k, const TV* const* V, TW* w, TV* v_next, double* h, double* partial, hipStream_t s) {
+  for (int j0 = 0; j0 < k; j0 += KMAX) {
+    const int kk = std::min(KMAX, k - j0);
+    mdot_pass<TW, TV>(n, kk, w, V + j0, partial, s);
+    hipLaunchKernelGGL(k_mdot_final, kk, 256, 0, s, partial, h + j0);
+  }
+  hipLaunchKernelGGL((k_gs_update<TW, TV>), vgrid(n), 256, 0, s, n, k, V, h, w);
+  hipLaunchKernelGGL(k_sqnorm<TW>, RB, 256, 0, s, n, w, partial);
   hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, partial, h + k);
-  hipLaunchKernelGGL(k_scale_by_norm<T>, vgrid(n), 256, 0, s, n, h + k, w, v_next);
+  hipLaunchKernelGGL((k_scale_by_norm<TW, TV>), vgrid(n), 256, 0, s, n, h + k, w, v_next);
   UWG_HIP(hipGetLastError());
 }
-template void vk_gs_step<double>(long, int, const double* const*, double*, double*, double*, double*, hipStream_t);
-template void vk_gs_step<float>(long, int, const float* const*, float*, float*, double*, double*, hipStream_t);
+template void vk_gs_step<double, double>(long, int, const double* const*, double*, double*, double*, double*,
+                                         hipStream_t);
+template void vk_gs_step<float, float>(long, int, const float* const*, float*, float*, double*, double*, hipStream_t);
+template void vk_gs_step<double, float>(long, int, const float* const*, double*, float*, double*, double*,
+                                        hipStream_t);
 template void vk_multi_dot<double>(long, int, const double*, const double* const*, double*, double*, hipStream_t);
 template void vk_multi_dot<float>(long, int, const float*, const float* const*, double*, double*, hipStream_t);
 

@@ -37,7 +37,7 @@ static void dfree(void* p) {
 enum WsSlot {
   WS_FU, WS_HP, WS_U, WS_P, WS_UBC_FLAG, WS_LIFT_T, WS_LIFT_Q, WS_FREEMASK,
   WS_RES_FU, WS_RES_HP, WS_RES_UF, WS_RES_T, WS_RES_GP, WS_RES_Q,
-  WS_BLK_B, WS_BLK_X, WS_BLK_DS, WS_BLK_Q, WS_BLK_GP,
+  WS_BLK_B, WS_BLK_X, WS_BLK_DS, WS_BLK_Q, WS_BLK_GP, WS_BLK_RP,
   WS_SCH_USTAR, WS_SCH_HHAT, WS_SCH_T, WS_SCH_Z, WS_SCH_DS,
   WS_MIX_B, WS_MIX_X, WS_MIX_R, WS_MIX_GP, WS_MIX_R32, WS_MIX_D32, WS_MIX_DS32, WS_MIX_Q32, WS_MIX_GP32,
   WS_Q2_ETAG, WS_Q2_FORCEG, WS_Q2_BAD,
@@ -95,7 +95,7 @@ struct uwg_swarm {
 namespace uwg {
 
 struct KrylovSpace;
-template <typename T> struct KSpaceT;
+template <typename T, typename TV = T> struct KSpaceT;
 
 struct Level {
   Geom g;
@@ -188,6 +188,7 @@ struct uwg_stokes {
   KrylovSpace* kout = nullptr;
   KrylovSpace* kb = nullptr;
   KSpaceT<float>* kf = nullptr;  // FP32 inner Krylov space of the mixed-precision solver
+  KSpaceT<double, float>* kb32 = nullptr;  // FP64 Krylov space with an FP32 Arnoldi basis
   // learned-correction support
   bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
   int nn_window = 10;            // fallback monitor: over nn_window iterations the residual must
@@ -822,6 +823,7 @@ static void copy_W_to_host(const Level& A, double* out) {
 // Whether the 3D FP32 multigrid cycle reads its element weights as bfloat16 (the element
 // contractions then take bf16 inputs with FP32 arithmetic and accumulation).
 static bool g_cycle_bf16 = true;
+static bool g_basis_fp32 = true;  // FP32 Arnoldi basis in the coupled FP64 iteration at precision 1
 
 static void ensure_fp32(uwg_stokes* st) {
   if (st->fp32_ready && st->fp32_bf16 == g_cycle_bf16) return;
@@ -1324,13 +1326,16 @@ static T* top_buf_x(uwg_stokes* st) {
 // Converged when |r| <= max(rtol |r0|, atol) with r0 the initial residual
 // (KSPConvergedDefault with zero initial guess).
 // ---------------------------------------------------------------------------------------
-template <typename T>
+// The Arnoldi basis V may be stored in a narrower type TV than the preconditioned vectors Z,
+// the work vector w and the iterate (T).
+template <typename T, typename TV>
 struct uwg::KSpaceT {
   long n = 0;
   int m = 0;
-  std::vector<T*> V, Z;
+  std::vector<TV*> V;
+  std::vector<T*> Z;
   T* w = nullptr;
-  const T** dV = nullptr;
+  const TV** dV = nullptr;
   void init(long n_, int m_) {
     if (n_ == n && m_ == m) return;
     free_all();
@@ -1338,11 +1343,11 @@ struct uwg::KSpaceT {
     m = m_;
     V.resize(m + 1);
     Z.resize(m);
-    for (auto& v : V) v = dalloc<T>(n);
+    for (auto& v : V) v = dalloc<TV>(n);
     for (auto& z : Z) z = dalloc<T>(n);
     w = dalloc<T>(n);
-    UWG_HIP(hipMalloc(&dV, sizeof(T*) * (m + 1)));
-    UWG_HIP(hipMemcpy(dV, V.data(), sizeof(T*) * (m + 1), hipMemcpyHostToDevice));
+    UWG_HIP(hipMalloc(&dV, sizeof(TV*) * (m + 1)));
+    UWG_HIP(hipMemcpy(dV, V.data(), sizeof(TV*) * (m + 1), hipMemcpyHostToDevice));
   }
   void free_all() {
     for (auto& v : V) dfree(v);
@@ -1353,6 +1358,11 @@ struct uwg::KSpaceT {
     Z.clear();
     w = nullptr;
     dV = nullptr;
+  }
+  void release() {
+    free_all();
+    n = 0;
+    m = 0;
   }
   ~KSpaceT() { free_all(); }
 };
@@ -1371,11 +1381,12 @@ static double dev_dot(uwg_stokes* st, long n, const T* a, const T* b) {
   return r;
 }
 
-// Right-preconditioned FGMRES with classical Gram-Schmidt; vectors of type T, every
-// reduction and the Hessenberg least-squares problem in double.
-template <typename T>
-static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const T*, T*)>& A,
-                  const std::function<void(const T*, T*)>& M, const T* b, T* x, double rtol, double atol, int maxit,
+// Right-preconditioned FGMRES with classical Gram-Schmidt; vectors of type T, the Arnoldi
+// basis of type TV (T, or float under an FP64 iteration), every reduction and the Hessenberg
+// least-squares problem in double.  The preconditioner reads a basis vector.
+template <typename T, typename TV = T>
+static int fgmres(uwg_stokes* st, KSpaceT<T, TV>& K, const std::function<void(const T*, T*)>& A,
+                  const std::function<void(const TV*, T*)>& M, const T* b, T* x, double rtol, double atol, int maxit,
                   int* its_out, double* rnorm_out, bool x0_zero = true,
                   const std::function<void(int, double)>* monitor = nullptr) {
   const long n = K.n;
@@ -1385,21 +1396,29 @@ static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const 
   double r0 = -1.0, rn = 0.0;
   bool conv = false;
   while (true) {
-    // r = b - A x
+    // r = b - A x (formed in the basis' first vector, or in w when the basis is narrower)
+    T* r = nullptr;
+    if constexpr (std::is_same<T, TV>::value)
+      r = K.V[0];
+    else
+      r = K.w;
     if (x0_zero && its == 0) {
-      vk_copy<T>(n, b, K.V[0], 0);
+      vk_copy<T>(n, b, r, 0);
     } else {
-      A(x, K.V[0]);
-      vk_axpby<T>(n, (T)1, b, (T)-1, K.V[0], 0);
+      A(x, r);
+      vk_axpby<T>(n, (T)1, b, (T)-1, r, 0);
     }
-    double beta = sqrt(std::max(0.0, dev_dot<T>(st, n, K.V[0], K.V[0])));
+    double beta = sqrt(std::max(0.0, dev_dot<T>(st, n, r, r)));
     if (r0 < 0) r0 = beta;
     rn = beta;
     if (beta <= std::max(rtol * r0, atol) || its >= maxit) {
       conv = beta <= std::max(rtol * r0, atol);
       break;
     }
-    vk_scale<T>(n, (T)(1.0 / beta), K.V[0], K.V[0], 0);
+    if constexpr (std::is_same<T, TV>::value)
+      vk_scale<T>(n, (T)(1.0 / beta), K.V[0], K.V[0], 0);
+    else
+      vk_d2f(n, 1.0 / beta, r, K.V[0], 0);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
     int j = 0;
@@ -1407,7 +1426,7 @@ static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const 
       M(K.V[j], K.Z[j]);
       A(K.Z[j], K.w);
       // Gram-Schmidt on the device; one copy of the Hessenberg column per iteration
-      vk_gs_step<T>(n, j + 1, K.dV, K.w, K.V[j + 1], st->dh, st->red_partial, 0);
+      vk_gs_step<T, TV>(n, j + 1, K.dV, K.w, K.V[j + 1], st->dh, st->red_partial, 0);
       UWG_HIP(hipMemcpy(hcol.data(), st->dh, sizeof(double) * (j + 2), hipMemcpyDeviceToHost));
       double hn = sqrt(std::max(0.0, hcol[j + 1]));
       hcol[j + 1] = hn;
@@ -1447,11 +1466,13 @@ static int fgmres(uwg_stokes* st, KSpaceT<T>& K, const std::function<void(const 
     UWG_HIP(hipMemcpy(st->dptrs, zp.data(), sizeof(T*) * k, hipMemcpyHostToDevice));
     vk_multi_axpy<T>(n, k, (const T* const*)st->dptrs, st->dcoef, x, 0);
     x0_zero = false;
-    if (rn <= std::max(rtol * r0, atol)) {
+    // with a narrower basis the estimate can run ahead of the true residual by the basis'
+    // rounding: the loop head then decides on the true residual (one more apply of A)
+    if (rn <= std::max(rtol * r0, atol) && std::is_same<T, TV>::value) {
       conv = true;
       break;
     }
-    if (its >= maxit) break;
+    if (its >= maxit && std::is_same<T, TV>::value) break;
   }
   if (its_out) *its_out = its;
   if (rnorm_out) *rnorm_out = rn;
@@ -1528,7 +1549,7 @@ extern "C" int uwg_stokes_create(uwg_field* vel, uwg_field* p, const unsigned ch
   st->ubc = dalloc<double>(st->nu);
   st->approxS = dalloc<double>(g.E);
   st->h = dalloc<double>(st->np);
-  st->red_partial = dalloc<double>(16 * vk_reduce_blocks());
+  st->red_partial = dalloc<double>(32 * vk_reduce_blocks());
   st->red_out = dalloc<double>(8);
   st->dcoef = dalloc<double>(256);
   st->dh = dalloc<double>(256);
@@ -1557,6 +1578,7 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
   delete st->kout;
   delete st->kb;
   delete st->kf;
+  delete st->kb32;
   dfree(st->capture_buf);
   for (int k = 0; k < WS_COUNT; k++) dfree(st->ws_ptr[k]);
   for (void* p : {(void*)st->fe, (void*)st->f, (void*)st->ubc, (void*)st->approxS, (void*)st->h, (void*)st->eta_tab,
@@ -1988,7 +2010,6 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   const uint8_t* bc = top_bc(st);
   if (!st->kb) st->kb = new KrylovSpace();
   KrylovSpace& Kb = *st->kb;
-  Kb.init(nt, o.restart);
   double* b = wsbuf<double>(st, WS_BLK_B, nt);
   double* x = wsbuf<double>(st, WS_BLK_X, nt);
   double* diagS = wsbuf<double>(st, WS_BLK_DS, E);
@@ -2037,7 +2058,33 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
     }
   };
   st->nn_disabled = !o.learned;
-  int rc = fgmres<double>(st, Kb, Aop, Mop, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true, &mon);
+  int rc;
+  if (o.precision == 1 && g_basis_fp32) {
+    // FP32 Arnoldi basis: the FP32 cycle reads the basis vector's velocity part as it is (no
+    // conversion pass); Z, w, the iterate and every reduction stay FP64, and the restarts and
+    // the stopping test use the FP64 true residual
+    if (!st->kb32) st->kb32 = new KSpaceT<double, float>();
+    KSpaceT<double, float>& Kb32 = *st->kb32;
+    Kb32.init(nt, o.restart);
+    Kb.release();  // the FP64 basis of the other precisions is not kept alongside
+    double* rp = wsbuf<double>(st, WS_BLK_RP, E);
+    std::function<void(const float*, double*)> Mop32 = [&](const float* rr, double* zz) {
+      st->vcycles++;
+      float* x32 = top_buf_x<float>(st);
+      top_vcycle<float>(st, rr, x32);
+      vk_f2d(n, 1.0, x32, zz, 0);
+      top_GT<double>(st, zz, q, bc);
+      vk_f2d(E, 1.0, rr + n, rp, 0);
+      vk_axpby<double>(E, 1.0, rp, -1.0, q, 0);  // q = r_p - G^T z_u
+      top_schur<double>(st, diagS, q, zz + n);
+    };
+    rc = fgmres<double, float>(st, Kb32, Aop, Mop32, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true,
+                               &mon);
+  } else {
+    if (st->kb32) st->kb32->release();
+    Kb.init(nt, o.restart);
+    rc = fgmres<double>(st, Kb, Aop, Mop, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true, &mon);
+  }
   S->outer_its = its;
   vk_copy<double>(n, x, u, 0);
   vk_copy<double>(E, x + n, p, 0);
@@ -2349,6 +2396,11 @@ extern "C" int uwg_set_q2_mfma(int on) {
 
 extern "C" int uwg_set_cycle_bf16(int on) {
   g_cycle_bf16 = on != 0;
+  return 0;
+}
+
+extern "C" int uwg_set_krylov_basis_fp32(int on) {
+  g_basis_fp32 = on != 0;
   return 0;
 }
 

@@ -200,9 +200,10 @@ template <typename T>
 void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s);
 // one classical Gram-Schmidt step with device-resident coefficients: h[0..k) = V^T w,
 // w -= sum h_j V_j, h[k] = |w|^2 (bit-identical to a separate dot), v_next = w / |w| (skipped
-// when |w| = 0); partial holds 16 x vk_reduce_blocks() doubles
-template <typename T>
-void vk_gs_step(long n, int k, const T* const* V, T* w, T* v_next, double* h, double* partial, hipStream_t s);
+// when |w| = 0).  The basis V (and v_next) may be stored in a narrower type TV than w; every
+// product and sum is in double.  partial holds 32 x vk_reduce_blocks() doubles
+template <typename TW, typename TV>
+void vk_gs_step(long n, int k, const TV* const* V, TW* w, TV* v_next, double* h, double* partial, hipStream_t s);
 int vk_reduce_blocks();
 // chebyshev helper: d = a*d + b*(dinv .* r) ; x += d  (fused)
 template <typename T>

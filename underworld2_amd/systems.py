@@ -374,7 +374,8 @@ class StokesSolver:
         """"fp64": every operator, cycle and Krylov vector in FP64.
         "mixed": FP64 Krylov (residual, dot products, orthogonalisation) preconditioned by an
                  FP32 multigrid cycle (FP32 operator and smoother, bf16 element weights on 3D
-                 levels -- see set_cycle_weights; FP64 coarsest solve).
+                 levels -- see set_cycle_weights; FP64 coarsest solve); the coupled ("block")
+                 method stores its Arnoldi basis in FP32 (see set_krylov_basis).
         "mixed-refine": FP64 iterative refinement around an FP32 Krylov solve whose
                  operator, cycle and basis vectors are FP32 (reductions in FP64)."""
         self._precision = {"fp64": 0, "mixed": 1, "mixed-cycle": 1, "mixed-refine": 2}[precision]
@@ -385,6 +386,13 @@ class StokesSolver:
         element contractions, FP32 arithmetic and accumulation) or "fp32".  Process-wide; takes
         effect at the next solve."""
         L.check(L.lib().uwg_set_cycle_bf16({"bf16": 1, "fp32": 0}[fmt]), "set_cycle_bf16")
+
+    @staticmethod
+    def set_krylov_basis(fmt):
+        """Storage of the Arnoldi basis of the coupled FP64 FGMRES under "mixed" precision:
+        "fp32" (default; halves the orthogonalisation's memory traffic, every dot product and
+        the stopping test stay FP64) or "fp64".  Process-wide; takes effect at the next solve."""
+        L.check(L.lib().uwg_set_krylov_basis_fp32({"fp32": 1, "fp64": 0}[fmt]), "set_krylov_basis")
 
     def set_learned(self, model):
         """Attach a trained learned coarse correction (``underworld2_amd.learned``)."""
