@@ -1674,6 +1674,41 @@ __global__ __launch_bounds__(256) void k_mdot(long n, int k, const TX* __restric
   }
   block_sums<KM>(acc, k, sm, partial);
 }
+// k_mdot over element pairs (one 2-wide load per vector and thread: half the load
+// instructions); the Gram-Schmidt step's vectors are whole allocations, so the pairs are aligned.
+// An odd last element is added by the grid's first thread.
+template <typename T> struct Pair;
+template <> struct Pair<double> { using V = double2; };
+template <> struct Pair<float> { using V = float2; };
+template <int KM, typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_mdot_pairs(long n, int k, const TX* __restrict__ x, const TY* const* Y,
+                                                    double* partial) {
+  using XV = typename Pair<TX>::V;
+  using YV = typename Pair<TY>::V;
+  __shared__ double sm[KM][4];
+  double acc[KM];
+#pragma unroll
+  for (int j = 0; j < KM; j++) acc[j] = 0.0;
+  const long n2 = n >> 1;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) {
+    const XV xv = reinterpret_cast<const XV*>(x)[i];
+    const double x0 = (double)xv.x, x1 = (double)xv.y;
+#pragma unroll
+    for (int j = 0; j < KM; j++)
+      if (j < k) {
+        const YV yv = reinterpret_cast<const YV*>(Y[j])[i];
+        acc[j] += x0 * (double)yv.x;
+        acc[j] += x1 * (double)yv.y;
+      }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const double xl = (double)x[n - 1];
+#pragma unroll
+    for (int j = 0; j < KM; j++)
+      if (j < k) acc[j] += xl * (double)Y[j][n - 1];
+  }
+  block_sums<KM>(acc, k, sm, partial);
+}
 // x.x with the one-vector k_mdot's grid, order and partial layout (bit-identical to it)
 template <typename T>
 __global__ __launch_bounds__(256) void k_sqnorm(long n, const T* __restrict__ x, double* partial) {
@@ -1713,6 +1748,15 @@ static void mdot_pass(long n, int k, const TX* x, const TY* const* Y, double* pa
   else
     hipLaunchKernelGGL((k_mdot<32, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
 }
+template <typename TX, typename TY>
+static void mdot_pairs_pass(long n, int k, const TX* x, const TY* const* Y, double* partial, hipStream_t s) {
+  if (k <= 8)
+    hipLaunchKernelGGL((k_mdot_pairs<8, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+  else if (k <= 16)
+    hipLaunchKernelGGL((k_mdot_pairs<16, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+  else
+    hipLaunchKernelGGL((k_mdot_pairs<32, TX, TY>), RB, 256, 0, s, n, k, x, Y, partial);
+}
 template <typename T>
 void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial, double* out, hipStream_t s) {
   if (k < 1) return;
@@ -1748,12 +1792,13 @@ __global__ void k_scale_by_norm(long n, const double* __restrict__ dot, const TW
 }
 
 // The update runs on the streaming grid (a reduction-sized grid holds too few loads in flight
-// for the k-vector gather); |w|^2 is then the one-vector dot of vk_multi_dot.
+// for the k-vector gather); |w|^2 is then the one-vector dot of vk_multi_dot.  V[j] and w must
+// be whole allocations (16-byte aligned) for the paired loads of the dot products.
 template <typename TW, typename TV>
 void vk_gs_step(long n, int k, const TV* const* V, TW* w, TV* v_next, double* h, double* partial, hipStream_t s) {
   for (int j0 = 0; j0 < k; j0 += KMAX) {
     const int kk = std::min(KMAX, k - j0);
-    mdot_pass<TW, TV>(n, kk, w, V + j0, partial, s);
+    mdot_pairs_pass<TW, TV>(n, kk, w, V + j0, partial, s);
     hipLaunchKernelGGL(k_mdot_final, kk, 256, 0, s, partial, h + j0);
   }
   hipLaunchKernelGGL((k_gs_update<TW, TV>), vgrid(n), 256, 0, s, n, k, V, h, w);
