@@ -1769,13 +1769,33 @@ void vk_multi_dot(long n, int k, const T* x, const T* const* Y, double* partial,
 }
 
 // Classical Gram-Schmidt update on the device with the coefficients left in device memory:
-// w -= sum_j h[j] V[j], accumulated in double (k_multi_axpy with c = -h).
+// w -= sum_j h[j] V[j], accumulated in double (k_multi_axpy with c = -h), over element pairs
+// (2-wide loads and stores; whole allocations, so aligned); an odd last element is done by the
+// grid's first thread.
 template <typename TW, typename TV>
-__global__ void k_gs_update(long n, int k, const TV* const* V, const double* __restrict__ h, TW* __restrict__ w) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double acc = (double)w[i];
-    for (int j = 0; j < k; j++) acc += (-h[j]) * (double)V[j][i];
-    w[i] = (TW)acc;
+__global__ void k_gs_update_pairs(long n, int k, const TV* const* V, const double* __restrict__ h,
+                                  TW* __restrict__ w) {
+  using WV = typename Pair<TW>::V;
+  using VV = typename Pair<TV>::V;
+  const long n2 = n >> 1;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) {
+    const WV wv = reinterpret_cast<const WV*>(w)[i];
+    double a0 = (double)wv.x, a1 = (double)wv.y;
+    for (int j = 0; j < k; j++) {
+      const VV v = reinterpret_cast<const VV*>(V[j])[i];
+      const double c = -h[j];
+      a0 += c * (double)v.x;
+      a1 += c * (double)v.y;
+    }
+    WV o;
+    o.x = (TW)a0;
+    o.y = (TW)a1;
+    reinterpret_cast<WV*>(w)[i] = o;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double acc = (double)w[n - 1];
+    for (int j = 0; j < k; j++) acc += (-h[j]) * (double)V[j][n - 1];
+    w[n - 1] = (TW)acc;
   }
 }
 
@@ -1801,7 +1821,7 @@ void vk_gs_step(long n, int k, const TV* const* V, TW* w, TV* v_next, double* h,
     mdot_pairs_pass<TW, TV>(n, kk, w, V + j0, partial, s);
     hipLaunchKernelGGL(k_mdot_final, kk, 256, 0, s, partial, h + j0);
   }
-  hipLaunchKernelGGL((k_gs_update<TW, TV>), vgrid(n), 256, 0, s, n, k, V, h, w);
+  hipLaunchKernelGGL((k_gs_update_pairs<TW, TV>), vgrid(std::max(n >> 1, 1L)), 256, 0, s, n, k, V, h, w);
   hipLaunchKernelGGL(k_sqnorm<TW>, RB, 256, 0, s, n, w, partial);
   hipLaunchKernelGGL(k_mdot_final, 1, 256, 0, s, partial, h + k);
   hipLaunchKernelGGL((k_scale_by_norm<TW, TV>), vgrid(n), 256, 0, s, n, h + k, w, v_next);
