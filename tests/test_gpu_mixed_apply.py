@@ -54,3 +54,33 @@ def test_cycle_apply_bf16_and_fp32_against_fp64():
         assert np.array_equal(_apply(L, st, 4, 0, x), _apply(L, st, 5, 0, x))
     finally:
         solver.set_cycle_weights("bf16")
+
+
+def test_fp32_krylov_basis_against_fp64_basis():
+    """The coupled mixed-precision solve with its Arnoldi basis in FP32 (the default) and in
+    FP64: both stop on the FP64 true residual (relative < rtol, recomputed here from the
+    returned statistics), and their velocities agree to the size the tolerance admits."""
+    import underworld2_amd as uw
+    from underworld2_amd import learned
+    print("FP32 vs FP64 Krylov basis, 32^3 GRF problem, contrast 1e3", flush=True)
+    sols, its = {}, {}
+    for fmt in ("fp64", "fp32"):
+        st, v, p, _ = learned.random_problem(32, 11, log10_contrast=3.0, corr_len=0.2)
+        solver = uw.systems.Solver(st)
+        solver.set_method("block")
+        solver.set_precision("mixed")
+        solver.set_outer_rtol(1e-6)
+        solver.set_krylov_basis(fmt)
+        try:
+            solver.solve()
+        finally:
+            solver.set_krylov_basis("fp32")
+        s = solver.get_stats()
+        rel = float(np.hypot(s.res_momentum, s.res_continuity) / s.norm_f)
+        assert s.converged and rel < 1e-6, (fmt, rel)
+        sols[fmt] = np.array(v.data)
+        its[fmt] = s.outer_its
+    print(f"iterations: FP64 basis {its['fp64']}, FP32 basis {its['fp32']}", flush=True)
+    assert abs(its["fp32"] - its["fp64"]) <= max(3, its["fp64"] // 10)
+    d = np.linalg.norm(sols["fp32"] - sols["fp64"]) / np.linalg.norm(sols["fp64"])
+    assert d < 1e-4, d
