@@ -63,6 +63,10 @@ int uwg_swarm_destroy(uwg_swarm *s);
 /* PerCellRandomLayout(swarm, ppc, seed) + populate_using_layout:
  * PY/swarm/layouts.py:186-229, MeshParticleLayout.c:194-229, Random.c:36-53 */
 int uwg_swarm_populate_random(uwg_swarm *s, int ppc, long seed);
+/* PerCellSpaceFillerLayout (PY/swarm/layouts.py:146-181 -> MeshParticleLayout.c:194-229,
+ * filltype 0): per-axis Sobol sequences (SobolGenerator.c) with the Park-Miller skip draw per
+ * particle; the reference's default seed is 13 */
+int uwg_swarm_populate_sobol(uwg_swarm *s, int ppc, long seed);
 /* add_particles_with_coordinates (PY/swarm/_swarm.py): appended in order, owners updated */
 int uwg_swarm_add_particles(uwg_swarm *s, long n, const double *coords);
 int uwg_swarm_count(uwg_swarm *s, long *P);
@@ -167,7 +171,9 @@ int uwg_stokes_residuals(uwg_stokes *st, double *out3);
 /* ---- operator hooks (tests and the learned-part trainer) */
 /* which: 0 = K (level), 1 = G, 2 = G^T, 3 = one multigrid V-cycle on K (level 0), 4 = K as the
  * FP32 cycle applies it (FP32 vectors; bfloat16 weights on 3D levels when the cycle uses them),
- * 5 = K in FP32 with FP32 weights (4 and 5: Q1 levels; x rounded to FP32, y widened).
+ * 5 = K in FP32 with FP32 weights (4 and 5: Q1 levels; x rounded to FP32, y widened),
+ * 6 = the multigrid residual kernel y = b - K x with b = x (FP64), 7 = the same kernel as the
+ * FP32 cycle runs it (cycle weights; x rounded to FP32, y widened).
  * Q2/dPc1 systems: level 0 is the Q2 level (pressure vectors [E][dim+1]), levels 1.. are the
  * Q1 levels of the multigrid hierarchy below it (K only) */
 int uwg_stokes_apply(uwg_stokes *st, int which, int level, const double *x, double *y);

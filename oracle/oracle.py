@@ -127,6 +127,19 @@ def layout_random(mesh, ppc, seed):
     return out
 
 
+def layout_sobol(mesh, ppc, seed=13):
+    P = mesh.E * ppc
+    out = np.zeros((P, mesh.dim))
+    lib().orc_layout_sobol(*mesh.args(), ctypes.c_int(ppc), ctypes.c_long(seed), _d(out))
+    return out
+
+
+def sobol_stream(axis, n):
+    out = np.zeros(n)
+    lib().orc_sobol_stream(ctypes.c_int(axis), ctypes.c_long(n), _d(out))
+    return out
+
+
 def rng_stream(seed, n):
     out = np.zeros(n)
     lib().orc_rng_stream(ctypes.c_long(seed), ctypes.c_long(n), _d(out))
@@ -428,7 +441,7 @@ class System:
         return float(np.linalg.norm(r1)), float(np.linalg.norm(r2))
 
 
-def full_pipeline(dim, res, lo, hi, ppc, seed, material_fn, eta_table, rho_table, gravity, dvc=True):
+def full_pipeline(dim, res, lo, hi, ppc, seed, material_fn, eta_table, rho_table, gravity, dvc=True, solve=True):
     """Oracle run of construction + solve: layout -> material -> coincident map -> DVC ->
     element integration -> assembled system -> exact solve.  material_fn(coords) -> int array.
     Returns a dict with all intermediate arrays (slot order where applicable)."""
@@ -447,6 +460,6 @@ def full_pipeline(dim, res, lo, hi, ppc, seed, material_fn, eta_table, rho_table
     Ke, fe, Ge, S = assemble_elements(mesh, ptr, xi_w, w, eta, force)
     bc = mesh.freeslip_mask()
     sysm = System(mesh, bc, Ke, fe, Ge, S)
-    u, p = sysm.solve()
+    u, p = sysm.solve() if solve else (None, None)
     return dict(mesh=mesh, coords=coords, mat=mat, ptr=ptr, pid=pid, xi_map=xi, xi=xi_w, w=w,
                 eta=eta, force=force, Ke=Ke, fe=fe, Ge=Ge, S=S, system=sysm, u=u, p=p)

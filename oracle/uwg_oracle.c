@@ -11,6 +11,10 @@
  *
  * Pinning (what checks each routine beyond retyping):
  *   orc_layout_random     PerCellRandomLayout docstring values (layouts.py:200-214)
+ *   orc_layout_sobol      PerCellSpaceFillerLayout docstring values (layouts.py:160-172), the
+ *                         Sobol sequence against its Gray-code closed form (XOR of the
+ *                         direction numbers over the set bits of gray(n)) and against the
+ *                         first terms of the standard Sobol sequence for these three axes
  *   orc_newton_local      closed form xi = 2(x-x0)/h - 1 on affine cells, rtol 1e-12
  *   orc_search_element    brute force on the integer rule incl. adversarial faces
  *   orc_dvc_cell          DVCWeightsSuite expected centroids/weights (3D and 2D), grid
@@ -205,6 +209,88 @@ void orc_layout_random(int dim, const int *res, const double *lo, const double *
             local_to_global(dim, verts, xi, coords + (e * ppc + p) * dim);
         }
     }
+}
+
+/* Sobol generator, one per axis (SobolGenerator.c; MeshParticleLayout.c:58-60 builds the
+   table entries 0, 1, 2: degree 1/2/3, polynomial coefficient 0/1/1, initial direction numbers
+   {1} / {1,1} / {1,3,7}).  The running value is an unsigned int (SobolGenerator.h:31; the
+   0.5 assigned at init, SobolGenerator.c _Init, truncates to 0).  Direction number i >= degree:
+   v_i = XOR over c = 0..degree-1 with c == degree-1 or bit (degree-2-c) of the coefficient
+   set of 2^(c+1) v_(i-1-c), then XOR v_(i-degree) (SobolGenerator_GetDirectionalNumber).
+   Draw: c = rightmost zero bit of the draw count, value ^= v_c << (30 - c - 1), returned as
+   value / 2^30 (SobolGenerator_GetNextNumber, WORDLENGTH 30). */
+typedef struct {
+    unsigned degree, coef, count, value;
+    unsigned v[32];
+} sobol_t;
+
+static void sobol_init(sobol_t *g, int axis)
+{
+    static const unsigned deg[3] = {1, 2, 3}, coef[3] = {0, 1, 1};
+    static const unsigned init[3][3] = {{1, 0, 0}, {1, 1, 0}, {1, 3, 7}};
+    g->degree = deg[axis];
+    g->coef = coef[axis];
+    g->count = 0;
+    g->value = 0;
+    for (unsigned i = 0; i < g->degree; i++) g->v[i] = init[axis][i];
+    for (unsigned i = g->degree; i < 32; i++) {
+        unsigned result = 0, dn = 0;
+        for (unsigned c = 0; c < g->degree; c++) {
+            if (c == g->degree - 1 || (g->coef & (1u << (g->degree - 2 - c)))) {
+                dn = g->v[i - 1 - c];
+                result ^= (1u << (c + 1)) * dn;
+            }
+        }
+        result ^= dn;
+        g->v[i] = result;
+    }
+}
+
+static double sobol_next(sobol_t *g)
+{
+    unsigned bit = 0, n = g->count;
+    while (n & 1u) {
+        n >>= 1;
+        bit++;
+    }
+    g->count++;
+    g->value ^= g->v[bit] << (30 - bit - 1);
+    return (double)g->value / (double)(1L << 30);
+}
+
+/* PerCellSpaceFillerLayout (layouts.py:146-181) -> _MeshParticleLayout_InitialiseParticlesOfCell
+   (MeshParticleLayout.c:194-229, filltype 0): per particle one Park-Miller draw
+   Swarm_Random_Random_WithMinMax(0, 1) (seeded at layout init with the default seed 13,
+   MeshParticleLayout.c:153, :74); per axis, when that draw exceeds 0.9 the axis' Sobol
+   generator skips one number; the local coordinate is (1 - -1) * next + -1. */
+void orc_layout_sobol(int dim, const int *res, const double *lo, const double *hi, int ppc, long seed,
+                      double *coords)
+{
+    long s = seed ^ 123459876L;
+    long E = (long)res[0] * res[1] * (dim == 3 ? res[2] : 1);
+    sobol_t gen[3];
+    for (int d = 0; d < 3; d++) sobol_init(&gen[d], d);
+    double verts[24];
+    for (long e = 0; e < E; e++) {
+        element_verts(dim, res, lo, hi, e, verts);
+        for (int p = 0; p < ppc; p++) {
+            double xi[3];
+            double randnum = (1.0 - 0.0) * pm_next(&s) / (double)2147483647L + 0.0;
+            for (int d = 0; d < dim; d++) {
+                if (randnum > 0.9) (void)sobol_next(&gen[d]);
+                xi[d] = (1.0 - -1.0) * sobol_next(&gen[d]) + -1.0;
+            }
+            local_to_global(dim, verts, xi, coords + (e * ppc + p) * dim);
+        }
+    }
+}
+
+/* Raw stream of one axis' Sobol generator (values in [0, 1)). */
+void orc_sobol_stream(int axis, long n, double *out)
+{
+    sobol_t g;
+    sobol_init(&g, axis);
+    for (long i = 0; i < n; i++) out[i] = sobol_next(&g);
 }
 
 /* Raw Park-Miller stream (for jump-ahead tests of the device generator). */

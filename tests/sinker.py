@@ -40,13 +40,14 @@ def build_uw(res, ppc, seed, centre, radius, eta=(1.0, 100.0), rho=(0.0, 1.0), l
     return dict(uw=uw, mesh=mesh, vel=vel, p=p, swarm=swarm, mat=mat, stokes=stokes)
 
 
-def build_oracle(res, ppc, seed, centre, radius, eta=(1.0, 100.0), rho=(0.0, 1.0), lo=None, hi=None):
+def build_oracle(res, ppc, seed, centre, radius, eta=(1.0, 100.0), rho=(0.0, 1.0), lo=None, hi=None, solve=True):
     from oracle import oracle as O
     dim = len(res)
     lo = lo or (0.0,) * dim
     hi = hi or (1.0,) * dim
     g = (0.0, -1.0) if dim == 2 else (0.0, 0.0, -1.0)
-    return O.full_pipeline(dim, res, lo, hi, ppc, seed, lambda x: sphere_material(x, centre, radius), eta, rho, g)
+    return O.full_pipeline(dim, res, lo, hi, ppc, seed, lambda x: sphere_material(x, centre, radius), eta, rho, g,
+                           solve=solve)
 
 
 def K_from_W(W, dim, h):

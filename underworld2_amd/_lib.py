@@ -66,6 +66,7 @@ SIGNATURES = {
     "uwg_swarm_create": [_vp, ctypes.POINTER(_vp)],
     "uwg_swarm_destroy": [_vp],
     "uwg_swarm_populate_random": [_vp, ctypes.c_int, ctypes.c_long],
+    "uwg_swarm_populate_sobol": [_vp, ctypes.c_int, ctypes.c_long],
     "uwg_swarm_add_particles": [_vp, ctypes.c_long, _dp],
     "uwg_swarm_count": [_vp, _lp],
     "uwg_swarm_get_coords": [_vp, _dp],

@@ -206,4 +206,15 @@ long relocate(const Geom& g, long P, const double* coords, int* owner, int* cell
   return outside;
 }
 
+// Exclusive prefix sum of 0/1 flags into 64-bit counts (the particle layout's skip counts).
+void scan_exclusive_flags(const int* flags, long long* out, long n, hipStream_t s) {
+  size_t bytes = 0;
+  UWG_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, flags, out, hipcub::Sum(), 0LL, (int)n, s));
+  void* tmp = nullptr;
+  UWG_HIP(hipMalloc(&tmp, bytes + 16));
+  UWG_HIP(hipcub::DeviceScan::ExclusiveScan(tmp, bytes, flags, out, hipcub::Sum(), 0LL, (int)n, s));
+  UWG_HIP(hipStreamSynchronize(s));
+  UWG_HIP(hipFree(tmp));
+}
+
 }  // namespace uwg

@@ -76,6 +76,94 @@ void launch_layout_random(const Geom& g, int ppc, long seed, double* coords, hip
   UWG_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------------------
+// PerCellSpaceFillerLayout (layouts.py:146-181; MeshParticleLayout.c:194-229, filltype 0).
+// Sequentially the reference draws, per particle, one Park-Miller number r (seeded as above,
+// default seed 13) and then per axis: if r > 0.9 one Sobol number is skipped, and the next is
+// the local coordinate.  Each axis' Sobol generator (SobolGenerator.c, WORDLENGTH 30) after n
+// draws holds the XOR of its direction numbers v_b << (29 - b) over the set bits b of the
+// Gray code n ^ (n >> 1), so particle p's value is a closed form of its draw count
+// n_p = p + 1 + (skips of the particles before it) + (its own skip): one flag kernel, an
+// exclusive scan of the flags, then every particle is independent.
+// ---------------------------------------------------------------------------------------
+__global__ void k_sobol_skip_flags(long P, unsigned long long s0, int* __restrict__ flags) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const unsigned long long st = mulmod(powmod(16807ULL, (unsigned long long)p + 1ULL), s0);
+  const double randnum = (1.0 - 0.0) * (double)(long)st / (double)2147483647L + 0.0;
+  flags[p] = randnum > 0.9 ? 1 : 0;
+}
+
+template <int DIM>
+__global__ void k_layout_sobol(GeomArgs g, int ppc, SobolDirs dirs, const int* __restrict__ flags,
+                               const long long* __restrict__ before, double* coords, long P) {
+  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  long e = p / ppc;
+  const unsigned long long n = (unsigned long long)p + 1ULL + (unsigned long long)before[p] + (unsigned long long)flags[p];
+  const unsigned long long gray = n ^ (n >> 1);
+  double xi[3], v[24], N[8];
+  for (int d = 0; d < DIM; d++) {
+    unsigned int val = 0u;
+    for (int b = 0; b < 30; b++)
+      if ((gray >> b) & 1ULL) val ^= dirs.v[d][b] << (30 - b - 1);
+    const double x = (double)val / (double)(1L << 30);
+    xi[d] = (1.0 - -1.0) * x + -1.0;
+  }
+  element_verts<DIM>(g.res, g.lo, g.hi, e, v);
+  shape<DIM>(xi, N);
+  double x[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < (DIM == 3 ? 8 : 4); a++)
+    for (int d = 0; d < DIM; d++) x[d] += N[a] * v[a * DIM + d];
+  for (int d = 0; d < DIM; d++) coords[p * DIM + d] = x[d];
+}
+
+// Direction numbers of the generators of table entries 0, 1, 2 (polynomial degree 1/2/3,
+// coefficient 0/1/1, initial numbers {1}, {1,1}, {1,3,7}; SobolGenerator.c table, built by
+// MeshParticleLayout.c:58-60), by the generator's recursion (SobolGenerator_GetDirectionalNumber).
+static SobolDirs sobol_dirs() {
+  static const unsigned deg[3] = {1, 2, 3}, coef[3] = {0, 1, 1};
+  static const unsigned init[3][3] = {{1, 0, 0}, {1, 1, 0}, {1, 3, 7}};
+  SobolDirs D;
+  for (int a = 0; a < 3; a++) {
+    for (unsigned i = 0; i < deg[a]; i++) D.v[a][i] = init[a][i];
+    for (unsigned i = deg[a]; i < 30; i++) {
+      unsigned result = 0, dn = 0;
+      for (unsigned c = 0; c < deg[a]; c++) {
+        if (c == deg[a] - 1 || (coef[a] & (1u << (deg[a] - 2 - c)))) {
+          dn = D.v[a][i - 1 - c];
+          result ^= (1u << (c + 1)) * dn;
+        }
+      }
+      D.v[a][i] = result ^ dn;
+    }
+  }
+  return D;
+}
+
+void launch_layout_sobol(const Geom& g, int ppc, long seed, double* coords, hipStream_t s) {
+  const long P = g.E * ppc;
+  // the generator's word has 30 bits: a draw count must stay below 2^30 (at most 2 draws per particle)
+  if (2 * P + 2 >= (1L << 30)) throw std::runtime_error("PerCellSpaceFillerLayout: too many particles for the 30-bit Sobol generator");
+  const unsigned long long s0 = (unsigned long long)(seed ^ 123459876L);
+  int* flags = nullptr;
+  long long* before = nullptr;
+  UWG_HIP(hipMalloc(&flags, sizeof(int) * (P + 1)));
+  UWG_HIP(hipMalloc(&before, sizeof(long long) * (P + 1)));
+  hipLaunchKernelGGL(k_sobol_skip_flags, nblocks(P, 256), 256, 0, s, P, s0, flags);
+  UWG_HIP(hipGetLastError());
+  scan_exclusive_flags(flags, before, P, s);
+  const SobolDirs dirs = sobol_dirs();
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_layout_sobol<3>, nblocks(P, 256), 256, 0, s, pack(g), ppc, dirs, flags, before, coords, P);
+  else
+    hipLaunchKernelGGL(k_layout_sobol<2>, nblocks(P, 256), 256, 0, s, pack(g), ppc, dirs, flags, before, coords, P);
+  UWG_HIP(hipGetLastError());
+  UWG_HIP(hipStreamSynchronize(s));
+  UWG_HIP(hipFree(flags));
+  UWG_HIP(hipFree(before));
+}
+
 template <int DIM>
 __global__ void k_mesh_coords(GeomArgs g, double* out, long N) {
   long n = (long)blockIdx.x * blockDim.x + threadIdx.x;

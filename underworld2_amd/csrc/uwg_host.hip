@@ -444,6 +444,8 @@ static void ensure_reloc(uwg_swarm* s) {
   }
 }
 
+static void populate_tables(uwg_swarm* s, int ppc);
+
 extern "C" int uwg_swarm_populate_random(uwg_swarm* s, int ppc, long seed) {
   if (ppc < 1) return arg_error("particlesPerCell must be >= 1");
   if (s->P != 0) return arg_error("swarm already populated");
@@ -453,6 +455,29 @@ extern "C" int uwg_swarm_populate_random(uwg_swarm* s, int ppc, long seed) {
   swarm_reserve(s, P);
   s->P = P;
   launch_layout_random(g, ppc, seed, s->coords, 0);
+  populate_tables(s, ppc);
+  return 0;
+  API_END
+}
+
+extern "C" int uwg_swarm_populate_sobol(uwg_swarm* s, int ppc, long seed) {
+  if (ppc < 1) return arg_error("particlesPerCell must be >= 1");
+  if (s->P != 0) return arg_error("swarm already populated");
+  const long P = s->mesh->g.E * ppc;
+  if (2 * P + 2 >= (1L << 30)) return arg_error("PerCellSpaceFillerLayout: more particles than the 30-bit Sobol generator serves");
+  API_BEGIN
+  swarm_reserve(s, P);
+  s->P = P;
+  launch_layout_sobol(s->mesh->g, ppc, seed, s->coords, 0);
+  populate_tables(s, ppc);
+  return 0;
+  API_END
+}
+
+// cell tables, owners and materials of a freshly laid-out swarm: particle index = cell * ppc + k
+static void populate_tables(uwg_swarm* s, int ppc) {
+  const Geom& g = s->mesh->g;
+  const long P = s->P;
   std::vector<int> ptr(g.E + 1);
   for (long e = 0; e <= g.E; e++) ptr[e] = (int)(e * ppc);
   UWG_HIP(hipMemcpy(s->cell_ptr, ptr.data(), sizeof(int) * (g.E + 1), hipMemcpyHostToDevice));
@@ -463,8 +488,6 @@ extern "C" int uwg_swarm_populate_random(uwg_swarm* s, int ppc, long seed) {
   UWG_HIP(hipMemcpy(s->owner, ids.data(), sizeof(int) * P, hipMemcpyHostToDevice));
   UWG_HIP(hipMemset(s->mat, 0, sizeof(int) * P));
   s->state++;
-  return 0;
-  API_END
 }
 
 extern "C" int uwg_swarm_add_particles(uwg_swarm* s, long n, const double* coords) {
@@ -2296,21 +2319,33 @@ extern "C" int uwg_stokes_apply(uwg_stokes* st, int which, int level, const doub
   API_BEGIN
   Level& A = st->L[level];
   long nin, nout;
-  if (which == 0 || which == 3 || which == 4 || which == 5) nin = nout = A.n();
+  if (which == 0 || (which >= 3 && which <= 7)) nin = nout = A.n();
   else if (which == 1) { nin = st->g.E; nout = st->L[0].n(); }
   else if (which == 2) { nin = st->L[0].n(); nout = st->g.E; }
-  else return arg_error("which must be 0..5");
+  else return arg_error("which must be 0..7");
   double* dx = dalloc<double>(nin);
   double* dy = dalloc<double>(nout);
   UWG_HIP(hipMemcpy(dx, x, sizeof(double) * nin, hipMemcpyHostToDevice));
-  if (which == 4 || which == 5) {
+  if (which == 6) {
+    // the multigrid residual kernel (apply with the r = b - K x epilogue), b = x
+    launch_residual_K<double>(A.g, A.W, dx, dx, dy, A.bc, 0);
+    UWG_HIP(hipMemcpy(y, dy, sizeof(double) * nout, hipMemcpyDeviceToHost));
+    dfree(dx);
+    dfree(dy);
+    return 0;
+  }
+  if (which == 4 || which == 5 || which == 7) {
     // the FP32 cycle's K (4: with the cycle's weights, bfloat16 on 3D levels when enabled;
-    // 5: FP32 weights) on x rounded to FP32, result widened to FP64
+    // 5: FP32 weights; 7: the cycle's residual kernel r = b - K x with b = x) on x rounded to
+    // FP32, result widened to FP64
     ensure_fp32(st);
     float* fx = dalloc<float>(nin);
     float* fy = dalloc<float>(nout);
     vk_d2f(nin, 1.0, dx, fx, 0);
-    launch_apply_K<float>(A.g, A.W32, fx, fy, A.bc, 0, which == 4 ? A.Wbf : nullptr);
+    if (which == 7)
+      launch_residual_K<float>(A.g, A.W32, fx, fx, fy, A.bc, 0, A.Wbf);
+    else
+      launch_apply_K<float>(A.g, A.W32, fx, fy, A.bc, 0, which == 4 ? A.Wbf : nullptr);
     vk_f2d(nout, 1.0, fy, dy, 0);
     UWG_HIP(hipMemcpy(y, dy, sizeof(double) * nout, hipMemcpyDeviceToHost));
     dfree(fx);

@@ -59,6 +59,12 @@ inline Geom make_geom(int dim, const int* res, const double* lo, const double* h
 // launch wrappers: swarm_kernels.hip (compiled with -ffp-contract=off, bit-exact paths)
 // ---------------------------------------------------------------------------------------
 void launch_layout_random(const Geom& g, int ppc, long seed, double* coords, hipStream_t s);
+struct SobolDirs {
+  unsigned int v[3][30];
+};
+void launch_layout_sobol(const Geom& g, int ppc, long seed, double* coords, hipStream_t s);
+// out[i] = sum of flags[0..i-1] (relocate.hip, hipcub)
+void scan_exclusive_flags(const int* flags, long long* out, long n, hipStream_t s);
 void launch_mesh_coords(const Geom& g, double* out, hipStream_t s);
 void launch_cell_of(const Geom& g, long P, const double* coords, int* owner, hipStream_t s);
 void launch_coincident_map(const Geom& g, long P, const int* slot_cell, const int* cell_pid,

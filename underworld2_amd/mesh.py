@@ -73,6 +73,10 @@ class _SubMesh:
     def nodesGlobal(self):
         return self.parent.elementsGlobal * self._per_element
 
+    def add_variable(self, nodeDofCount, dataType="double", **kwargs):
+        """MeshVariable on this sub-mesh (PY/mesh/_mesh.py FeMesh.add_variable)."""
+        return MeshVariable(self, nodeDofCount, dataType, **kwargs)
+
     @property
     def data(self):
         """Pressure vertex coordinates: element centres (dQ0) or the dPc1 inner vertices."""
@@ -122,6 +126,10 @@ class FeMesh_Cartesian:
         self.subMesh = _SubMesh(self, "DPC1" if self._order == 2 else "DQ0")
         self._data = None
         self._special = None
+
+    def add_variable(self, nodeDofCount, dataType="double", **kwargs):
+        """MeshVariable on this mesh's nodes (PY/mesh/_mesh.py FeMesh.add_variable)."""
+        return MeshVariable(self, nodeDofCount, dataType, **kwargs)
 
     def __del__(self):
         try:

@@ -368,5 +368,21 @@ class _Layouts:
         def _populate(self, swarm):
             L.check(L.lib().uwg_swarm_populate_random(swarm._h, self.particlesPerCell, self.seed), "populate")
 
+    class PerCellSpaceFillerLayout:
+        """``PerCellSpaceFillerLayout(swarm, particlesPerCell)`` (PY/swarm/layouts.py:146-181):
+        per-axis Sobol sequences, with the reference's random skip (seed 13)."""
+
+        def __init__(self, swarm, particlesPerCell, **kwargs):
+            if not isinstance(particlesPerCell, int):
+                raise TypeError("'particlesPerCell' object passed in must be of type 'int'.")
+            if particlesPerCell < 1:
+                raise ValueError("'particlesPerCell' object passed in must take a value greater than zero.")
+            self.swarm = swarm
+            self.particlesPerCell = particlesPerCell
+            self.seed = 13
+
+        def _populate(self, swarm):
+            L.check(L.lib().uwg_swarm_populate_sobol(swarm._h, self.particlesPerCell, self.seed), "populate")
+
 
 layouts = _Layouts()

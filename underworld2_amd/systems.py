@@ -78,6 +78,8 @@ class Stokes:
         mesh = velocityField._base
         self._mesh = mesh
         mask = np.zeros(mesh.nodesGlobal, dtype=np.uint8)
+        if not isinstance(conditions, (list, tuple)):
+            conditions = [conditions]  # a single condition (PY/systems/_stokes.py:190-193)
         for cond in conditions:
             if not isinstance(cond, DirichletCondition):
                 raise TypeError("Provided 'conditions' must be 'DirichletCondition' objects.")
