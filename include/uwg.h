@@ -97,6 +97,14 @@ int uwg_advect_max_dt(uwg_field *vel, double *dt);
 /* interpolate a nodal field of 1-9 components at points, v[n][ncomp]
  * (FeVariable_InterpolateValueAt, FeVariable.c:389-403) */
 int uwg_field_interpolate(uwg_field *f, long n, const double *x, double *v, int *ok);
+/* Allocation-free variant of uwg_field_interpolate (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under graph capture, tight serving loops).
+ * Points, values and flags are staged in a fixed device array of the library (one per GPU,
+ * held until the process ends, about 6.5 MB) instead of per-call allocations; at most
+ * UWG_NOALLOC_MAX_POINTS points per call (more is an argument error).  Calls that overlap on
+ * one GPU share that array and race. */
+#define UWG_NOALLOC_MAX_POINTS 65536
+int uwg_field_interpolate_noalloc(uwg_field *f, long n, const double *x, double *v, int *ok);
 
 /* ---- Stokes: uw.systems.Stokes(vel, p, fn_viscosity, fn_bodyforce, voronoi_swarm, conditions)
  * PY/systems/_stokes.py:106-317.  bcmask[N]: bit i set = component i Dirichlet; NULL means

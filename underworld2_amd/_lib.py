@@ -81,6 +81,7 @@ SIGNATURES = {
     "uwg_advect_rk2": [_vp, _vp, ctypes.c_double, ctypes.c_int, _lp],
     "uwg_advect_max_dt": [_vp, _dp],
     "uwg_field_interpolate": [_vp, ctypes.c_long, _dp, _dp, _ip],
+    "uwg_field_interpolate_noalloc": [_vp, ctypes.c_long, _dp, _dp, _ip],
     "uwg_stokes_create": [_vp, _vp, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(_vp)],
     "uwg_stokes_destroy": [_vp],
     "uwg_stokes_set_materials": [_vp, ctypes.c_int, _dp, _dp],

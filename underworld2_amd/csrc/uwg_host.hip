@@ -9,6 +9,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 
 #include "../../include/uwg.h"
@@ -756,6 +757,64 @@ extern "C" int uwg_field_interpolate(uwg_field* f, long n, const double* x, doub
   dfree(dx);
   dfree(dv);
   dfree(dok);
+  return 0;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free interpolation.  uwg_field_interpolate stages the host points, values and
+// flags in device memory it allocates per call; this variant stages them in a fixed
+// file-scope device array instead, for callers that cannot allocate (code under graph
+// capture, tight serving loops).  Each GPU holds its own copy of the array until the process
+// ends; its device address is looked up once per device (the array's host-side name is not a
+// device address).  Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------------------
+constexpr long kNoAllocMaxPoints = UWG_NOALLOC_MAX_POINTS;
+constexpr size_t kScrX = 0;                                                   // double[max][3]
+constexpr size_t kScrV = kScrX + (size_t)kNoAllocMaxPoints * 3 * sizeof(double);  // double[max][9]
+constexpr size_t kScrOk = kScrV + (size_t)kNoAllocMaxPoints * 9 * sizeof(double);  // int[max]
+constexpr size_t kScrBytes = kScrOk + (size_t)kNoAllocMaxPoints * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[64] = {};
+
+// device address of this device's scratch array (looked up on the device's first call)
+static unsigned char* scratch_on_current_device() {
+  int dev = 0;
+  UWG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    g_scratch_ptr[dev] = (unsigned char*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+extern "C" int uwg_field_interpolate_noalloc(uwg_field* f, long n, const double* x, double* v, int* ok) {
+  if (f->on_elements) return arg_error("nodal field required");
+  if (f->ncomp < 1 || f->ncomp > 9) return arg_error("fields of 1 to 9 components can be interpolated");
+  if (n < 0 || n > kNoAllocMaxPoints)
+    return arg_error("uwg_field_interpolate_noalloc takes at most " + std::to_string(kNoAllocMaxPoints) +
+                     " points per call (got " + std::to_string(n) + "); split the points into batches");
+  if (n == 0) return 0;
+  API_BEGIN
+  const int dim = f->mesh->g.dim, nc = f->ncomp;
+  unsigned char* base = scratch_on_current_device();
+  double* dx = (double*)(base + kScrX);
+  double* dv = (double*)(base + kScrV);
+  int* dok = (int*)(base + kScrOk);
+  UWG_HIP(hipMemcpy(dx, x, sizeof(double) * n * dim, hipMemcpyHostToDevice));
+  if (f->mesh->order == 2)
+    launch_q2_interpolate(f->mesh->g, n, f->d, nc, dx, dv, dok, 0);
+  else
+    launch_field_interpolate(f->mesh->g, n, f->d, nc, dx, dv, dok, 0);
+  UWG_HIP(hipMemcpy(v, dv, sizeof(double) * n * nc, hipMemcpyDeviceToHost));
+  UWG_HIP(hipMemcpy(ok, dok, sizeof(int) * n, hipMemcpyDeviceToHost));
   return 0;
   API_END
 }
