@@ -226,6 +226,35 @@ int uwg_nn_loss_grad(uwg_nn *h, int B, const float *x, const float *t, const flo
 int uwg_nn_set_data(uwg_nn *h, long S, const float *x, const float *t, const float *mask);
 int uwg_nn_train(uwg_nn *h, int steps, int B, float lr, float weight_decay, long seed, float *losses);
 int uwg_nn_eval(uwg_nn *h, int B, float *loss);
+
+/* ---- learned Schur-complement stencil (engine extension, no counterpart in the reference):
+ * a convolutional network on the element grid maps local features of approxS (viscosity
+ * log-ratios to the face neighbours, wall flags) to a 27-point stencil C per element; the
+ * coupled solver's Schur block then applies -D^-1/2 C D^-1/2 (D = approxS) instead of
+ * -approxS^-1 (C = identity is the plain method).  Trained on (features, r^, e^) triples,
+ * r^ = D^-1/2 S D^-1/2 e^ with S = G^T K^-1 G applied by the plain solver's operators. */
+typedef struct uwg_snn uwg_snn;
+int uwg_snn_create(int channels, int layers, long seed, uwg_snn **out);
+int uwg_snn_destroy(uwg_snn *h);
+int uwg_snn_param_count(uwg_snn *h, long *n);
+int uwg_snn_get_params(uwg_snn *h, float *out);
+int uwg_snn_set_params(uwg_snn *h, const float *in);
+/* feat[9][nx*ny*nz] from approxS (host arrays) */
+int uwg_snn_features(int nx, int ny, int nz, const double *approxS, float *feat);
+/* y[B][27][N] = the network's stencil offsets (the stencil is identity + y) */
+int uwg_snn_forward(uwg_snn *h, int nx, int ny, int nz, int B, const float *x, float *y);
+/* batch mean of |C(x) r - t|^2 / |t|^2 and its parameter gradient (grads may be NULL) */
+int uwg_snn_loss_grad(uwg_snn *h, int nx, int ny, int nz, int B, const float *x, const float *r, const float *t,
+                      float *loss, float *grads);
+/* training set held on the device: nsamples triples from an assembled 3D Stokes system */
+int uwg_snn_add_samples(uwg_snn *h, uwg_stokes *st, int nsamples, long seed, double inner_rtol);
+int uwg_snn_sample_count(uwg_snn *h, long *n);
+int uwg_snn_clear_data(uwg_snn *h);
+int uwg_snn_train(uwg_snn *h, int steps, int B, float lr, float weight_decay, long seed, long ntrain, float *losses);
+int uwg_snn_eval(uwg_snn *h, long first, long count, float *loss);
+/* attach a trained stencil network to a 3D Q1/dQ0 Stokes system (used by solves with learned
+ * set, coupled method, precision 0 or 1; coefficients recomputed after every assembly) */
+int uwg_stokes_load_schur_nn(uwg_stokes *st, int channels, int layers, long nparams, const float *params);
 /* kernel-apply tuning knob: node levels per workgroup of the 3D K apply (kz3 > 0 fixes it on
  * every grid; kz3 < 0 restores the default, chosen per grid to fill the GPU's resident
  * workgroup slots evenly), element rows per block of the 2D K apply (kz2 > 0; default 32);

@@ -185,3 +185,103 @@ class Net:
         y, cache = self.forward(p, x)
         L, gy = self.loss(y, t, m)
         return L, self.backward(p, cache, gy), y
+
+
+# ------------------------------------------------------------------------------------------
+# Learned Schur-complement stencil (nn_kernels.hip, SchurNet): 9 features per element (approxS
+# log-ratios to the face neighbours, wall flags), L 3x3x3 convolutions with ReLU (9 -> C, then
+# C -> C), a 1x1x1 head C -> 27;
+# the stencil is identity + head; loss per sample |C r - t|^2 / |t|^2, mean over the batch.
+# Parameter layout: per 3x3x3 layer w[C][ci][27] then b[C]; head w[27][C] then b[27].
+# ------------------------------------------------------------------------------------------
+def snn_features(grid, S):
+    """feat[9][N] from approxS[N] on the element grid (nx, ny, nz), x fastest."""
+    nx, ny, nz = grid
+    t = -np.log(np.asarray(S, dtype=np.float64).reshape(nz, ny, nx))
+    out = []
+    for dz, dy, dx in ((0, 0, -1), (0, 0, 1), (0, -1, 0), (0, 1, 0), (-1, 0, 0), (1, 0, 0)):
+        valid = _shift(np.ones_like(t), dz, dy, dx) > 0
+        out.append(np.where(valid, np.tanh(0.5 * (_shift(t, dz, dy, dx) - t)), 0.0))
+    for ax, n in ((2, nx), (1, ny), (0, nz)):
+        idx = np.arange(n)
+        v = np.where(idx == 0, -1.0, np.where(idx == n - 1, 1.0, 0.0))
+        shape = [1, 1, 1]
+        shape[ax] = n
+        out.append(np.broadcast_to(v.reshape(shape), t.shape))
+    return np.stack(out, 0).reshape(9, -1)
+
+
+class StencilNet:
+    def __init__(self, grid, channels, layers):
+        self.grid = tuple(grid)
+        self.C, self.L = channels, layers
+        self.specs = []
+        off = 0
+        for l in range(layers):
+            ci = 9 if l == 0 else channels
+            self.specs.append((ci, channels, off, off + ci * channels * 27, 27))
+            off += ci * channels * 27 + channels
+        self.specs.append((channels, 27, off, off + channels * 27, 1))
+        off += channels * 27 + 27
+        self.nparams = off
+
+    def unpack(self, p):
+        return [(p[wo:bo].reshape(co, ci, k) if k == 27 else p[wo:bo].reshape(co, ci), p[bo:bo + co])
+                for ci, co, wo, bo, k in self.specs]
+
+    def _shape(self, a):
+        nx, ny, nz = self.grid
+        return a.reshape(a.shape[0], a.shape[1], nz, ny, nx)
+
+    def forward(self, p, x):
+        """x[B][9][N] -> stencil offsets y[B][27][N] and the cache."""
+        W = self.unpack(p)
+        h = self._shape(np.asarray(x, dtype=np.float64))
+        acts = []
+        inp = h
+        for l in range(self.L):
+            a = np.maximum(conv(inp, *W[l]), 0.0)
+            acts.append(a)
+            inp = a
+        wh, bh = W[self.L]
+        y = np.einsum("oc,bczyx->bozyx", wh, inp) + bh[None, :, None, None, None]
+        return y.reshape(y.shape[0], 27, -1), {"x": h, "acts": acts}
+
+    def stencil(self, y, r):
+        """out[b][e] = sum_s (delta_s,13 + y[b][s][e]) r[b][e + off(s)]."""
+        yy = self._shape(y)
+        rr = self._shape(np.asarray(r, dtype=np.float64)[:, None])[:, 0]
+        out = np.zeros_like(rr)
+        for s, (dz, dy, dx) in enumerate(TAPS):
+            out += ((1.0 if s == 13 else 0.0) + yy[:, s]) * _shift(rr, dz, dy, dx)
+        return out.reshape(out.shape[0], -1)
+
+    def loss_grad(self, p, x, r, t):
+        """mean over the batch of |C r - t|^2 / |t|^2, its parameter gradient, and C r."""
+        W = self.unpack(p)
+        y, cache = self.forward(p, x)
+        out = self.stencil(y, r)
+        t = np.asarray(t, dtype=np.float64)
+        B = out.shape[0]
+        den = (t * t).sum(1)
+        L = float(np.mean(((out - t) ** 2).sum(1) / den))
+        gout = 2.0 * (out - t) / den[:, None] / B
+        rr = self._shape(np.asarray(r, dtype=np.float64)[:, None])[:, 0]
+        go = self._shape(gout[:, None])[:, 0]
+        gy = np.stack([go * _shift(rr, dz, dy, dx) for dz, dy, dx in TAPS], 1)  # [B][27][z][y][x]
+        grads = [None] * (self.L + 1)
+        top = cache["acts"][-1]
+        wh, bh = W[self.L]
+        grads[self.L] = (np.einsum("bozyx,bczyx->oc", gy, top), gy.sum(axis=(0, 2, 3, 4)))
+        g = np.einsum("oc,bozyx->bczyx", wh, gy)
+        for l in range(self.L - 1, -1, -1):
+            g = g * (cache["acts"][l] > 0)
+            inp = cache["x"] if l == 0 else cache["acts"][l - 1]
+            gin, dw, db = conv_bwd(inp, W[l][0], g)
+            grads[l] = (dw, db)
+            g = gin
+        flat = np.zeros(self.nparams)
+        for (ci, co, wo, bo, k), (dw, db) in zip(self.specs, grads):
+            flat[wo:bo] = dw.ravel()
+            flat[bo:bo + co] = db
+        return L, flat, out

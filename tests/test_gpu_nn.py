@@ -160,12 +160,13 @@ def test_fallback_monitor_switches_to_multigrid():
     L.check(L.lib().uwg_stokes_set_nn_monitor(st._h, 10, 0.1))
 
 
-def test_trained_model_gain_on_unseen_problems():
-    """What the shipped learned correction gains over the same solver without it, on unseen
-    64^3 problems drawn like the training set (GRF viscosity, buoyant spheres).  Both runs must
-    reach the FP64 tolerance; the iteration ratio and times are printed.  Measured on MI355X
-    when the model was trained: no iteration gain (ratio about 1.0-1.2) and a time loss, which
-    this test bounds so that a regression of the network or of its scaling is caught."""
+def test_coarse_correction_is_experimental_and_bounded():
+    """The learned coarse correction of the velocity cycle is an experimental option: it gains
+    no iterations (the outer count is set by the Schur approximation, whose weak modes are
+    element-scale, out of reach of a coarse-grid correction; the learned Schur stencil, tested
+    in test_gpu_schur_nn.py, is the learned component that gains).  On unseen 64^3 problems
+    drawn like its training set both runs must reach the FP64 tolerance, and the safeguards
+    (step-length damping, fallback monitor) must bound its cost: at most 35 % more iterations."""
     import underworld2_amd as uw
     from underworld2_amd import learned
     model = learned.load_default()

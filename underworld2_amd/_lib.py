@@ -118,6 +118,21 @@ SIGNATURES = {
     "uwg_nn_set_data": [_vp, ctypes.c_long, _fp, _fp, _fp],
     "uwg_nn_train": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_long, _fp],
     "uwg_nn_eval": [_vp, ctypes.c_int, _fp],
+    "uwg_snn_create": [ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.POINTER(_vp)],
+    "uwg_snn_destroy": [_vp],
+    "uwg_snn_param_count": [_vp, ctypes.POINTER(ctypes.c_long)],
+    "uwg_snn_get_params": [_vp, _fp],
+    "uwg_snn_set_params": [_vp, _fp],
+    "uwg_snn_features": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _fp],
+    "uwg_snn_forward": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp],
+    "uwg_snn_loss_grad": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp, _fp, _fp, _fp],
+    "uwg_snn_add_samples": [_vp, _vp, ctypes.c_int, ctypes.c_long, ctypes.c_double],
+    "uwg_snn_sample_count": [_vp, ctypes.POINTER(ctypes.c_long)],
+    "uwg_snn_clear_data": [_vp],
+    "uwg_snn_train": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_long, ctypes.c_long,
+                      _fp],
+    "uwg_snn_eval": [_vp, ctypes.c_long, ctypes.c_long, _fp],
+    "uwg_stokes_load_schur_nn": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_long, _fp],
     "uwg_set_apply_chunk": [ctypes.c_int, ctypes.c_int],
     "uwg_set_cycle_bf16": [ctypes.c_int],
     "uwg_set_krylov_basis_fp32": [ctypes.c_int],

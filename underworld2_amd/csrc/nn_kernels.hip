@@ -747,6 +747,477 @@ void nn_coarse_correction(NNModel* m, const Geom& g, const float* r, float* x, c
 
 long nn_param_count(NNModel* m) { return m->nparams; }
 
+// ==========================================================================================
+// Learned Schur-complement stencil.
+//
+// The Schur complement S = G^T K^-1 G of Q1/dQ0 is preconditioned in the reference by the
+// diagonal approxS.  The modes approxS leaves with small eigenvalues are oscillatory on the
+// element scale, where a 27-point stencil around each element can act on them; so the network
+// maps local features of the element grid to a per-element 27-point stencil C(e), and the
+// preconditioner is M = D^-1/2 C D^-1/2 with D = approxS (C = identity reproduces approxS^-1).
+//
+// Features per element (9 channels) from t = -ln(approxS_e), which is ln(eta_e) up to the
+// element geometry: channels 0..5 tanh((t_nb - t_e) / 2) for the face neighbours x-, x+, y-,
+// y+, z-, z+ (0 where there is none), channels 6..8 -1 on the lower wall layer, +1 on the
+// upper wall layer, 0 elsewhere (x, y, z).  Network: 3x3x3 convolutions with ReLU, C -> C
+// (L layers, the first 9 -> C), then a 1x1x1 head C -> 27; the stencil is identity + head.
+// Trained on (features, r, e) with r = S e (scaled: r^ = D^-1/2 r, e^ = D^1/2 e), loss
+// |C r^ - e^|^2 / |e^|^2.  FP32 storage and arithmetic, deterministic reductions.
+// ==========================================================================================
+__device__ __forceinline__ void grid_ijk(const Grid3& g, long e, int& i, int& j, int& k) {
+  i = (int)(e % g.nx);
+  j = (int)((e / g.nx) % g.ny);
+  k = (int)(e / ((long)g.nx * g.ny));
+}
+
+constexpr int kSnnFeat = 9;  // input channels of the Schur-stencil network
+
+__global__ void k_snn_features(Grid3 g, const double* __restrict__ S, float* __restrict__ feat) {
+  const long N = g.N();
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  int i, j, k;
+  grid_ijk(g, e, i, j, k);
+  const double t = -log(S[e]);
+  const int ii[6] = {i - 1, i + 1, i, i, i, i}, jj[6] = {j, j, j - 1, j + 1, j, j}, kk[6] = {k, k, k, k, k - 1, k + 1};
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    float v = 0.0f;
+    if (ii[c] >= 0 && ii[c] < g.nx && jj[c] >= 0 && jj[c] < g.ny && kk[c] >= 0 && kk[c] < g.nz) {
+      const double tn = -log(S[ii[c] + (long)g.nx * (jj[c] + (long)g.ny * kk[c])]);
+      v = tanhf((float)(0.5 * (tn - t)));
+    }
+    feat[c * N + e] = v;
+  }
+  feat[6 * N + e] = i == 0 ? -1.0f : (i == g.nx - 1 ? 1.0f : 0.0f);
+  feat[7 * N + e] = j == 0 ? -1.0f : (j == g.ny - 1 ? 1.0f : 0.0f);
+  feat[8 * N + e] = k == 0 ? -1.0f : (k == g.nz - 1 ? 1.0f : 0.0f);
+}
+
+// 1x1x1 convolution: out[b][co][p] = bias[co] + sum_ci w[co][ci] in[b][ci][p]; COB output
+// channels per thread (weights wave-uniform)
+template <int COB>
+__global__ __launch_bounds__(256) void k_conv1_fwd(int B, int Ci, int Co, long N, const float* __restrict__ in,
+                                                   const float* __restrict__ w, const float* __restrict__ bias,
+                                                   float* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int co0 = blockIdx.y * COB;
+  if (t >= (long)B * N) return;
+  const long p = t % N;
+  const int b = (int)(t / N);
+  float acc[COB];
+#pragma unroll
+  for (int c = 0; c < COB; c++) acc[c] = bias[co0 + c];
+  for (int ci = 0; ci < Ci; ci++) {
+    const float v = in[((long)b * Ci + ci) * N + p];
+#pragma unroll
+    for (int c = 0; c < COB; c++) acc[c] += w[(long)(co0 + c) * Ci + ci] * v;
+  }
+#pragma unroll
+  for (int c = 0; c < COB; c++) out[((long)b * Co + co0 + c) * N + p] = acc[c];
+}
+
+// gin[b][ci][p] = sum_co w[co][ci] gout[b][co][p]
+__global__ void k_conv1_bwd_data(int B, int Ci, int Co, long N, const float* __restrict__ gout,
+                                 const float* __restrict__ w, float* __restrict__ gin) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * N) return;
+  const long p = t % N;
+  const int b = (int)(t / N);
+  for (int ci = 0; ci < Ci; ci++) {
+    float a = 0.0f;
+    for (int co = 0; co < Co; co++) a += w[(long)co * Ci + ci] * gout[((long)b * Co + co) * N + p];
+    gin[((long)b * Ci + ci) * N + p] = a;
+  }
+}
+
+// dw[co][ci] = sum_b sum_p gout[b][co][p] in[b][ci][p]; one block per (co, ci)
+__global__ __launch_bounds__(256) void k_conv1_bwd_w(int B, int Ci, int Co, long N, const float* __restrict__ gout,
+                                                     const float* __restrict__ in, float* __restrict__ dw) {
+  __shared__ float sm[256];
+  const int co = blockIdx.x / Ci, ci = blockIdx.x % Ci;
+  float a = 0.0f;
+  for (long t = threadIdx.x; t < (long)B * N; t += 256) {
+    const int b = (int)(t / N);
+    const long p = t % N;
+    a += gout[((long)b * Co + co) * N + p] * in[((long)b * Ci + ci) * N + p];
+  }
+  sm[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dw[(long)co * Ci + ci] = sm[0];
+}
+
+// out[b][e] = sum_s (delta_s,13 + y[b][s][e]) r[b][e + off(s)]  (0 outside the grid);
+// diff = out - t; per-block partial sums of diff^2 and t^2 in FP64 (blockIdx.y = sample)
+__global__ __launch_bounds__(256) void k_snn_stencil_diff(Grid3 g, const float* __restrict__ y,
+                                                          const float* __restrict__ r, const float* __restrict__ t,
+                                                          float* __restrict__ diff, double* __restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  const long N = g.N();
+  const int b = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a1 = 0.0, a2 = 0.0;
+  if (e < N) {
+    int i, j, k;
+    grid_ijk(g, e, i, j, k);
+    const float* yb = y + (long)b * 27 * N;
+    const float* rb = r + (long)b * N;
+    float o = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 27; s++) {
+      const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+      if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
+        o += ((s == 13 ? 1.0f : 0.0f) + yb[s * N + e]) * rb[ii + (long)g.nx * (jj + (long)g.ny * kk)];
+    }
+    const float tv = t[(long)b * N + e];
+    const float d = o - tv;
+    diff[(long)b * N + e] = d;
+    a1 = (double)d * d;
+    a2 = (double)tv * tv;
+  }
+  s1[threadIdx.x] = a1;
+  s2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+      s2[threadIdx.x] += s2[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[((long)b * gridDim.x + blockIdx.x) * 2] = s1[0];
+    part[((long)b * gridDim.x + blockIdx.x) * 2 + 1] = s2[0];
+  }
+}
+
+// per sample: loss_b = sum diff^2 / sum t^2; gscale_b = 2 / (sum t^2 B)  (one block per sample)
+__global__ __launch_bounds__(256) void k_snn_loss_reduce(int B, int nblk, const double* __restrict__ part,
+                                                         float* __restrict__ loss_b, float* __restrict__ gscale) {
+  __shared__ double s1[256], s2[256];
+  const int b = blockIdx.x;
+  double a1 = 0.0, a2 = 0.0;
+  for (int q = threadIdx.x; q < nblk; q += 256) {
+    a1 += part[((long)b * nblk + q) * 2];
+    a2 += part[((long)b * nblk + q) * 2 + 1];
+  }
+  s1[threadIdx.x] = a1;
+  s2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+      s2[threadIdx.x] += s2[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double den = s2[0] + 1e-300;
+    loss_b[b] = (float)(s1[0] / den);
+    gscale[b] = (float)(2.0 / (den * B));
+  }
+}
+
+// dL/dy[b][s][e] = gscale_b diff[b][e] r[b][e + off(s)]
+__global__ void k_snn_stencil_grad(int B, Grid3 g, const float* __restrict__ diff, const float* __restrict__ r,
+                                   const float* __restrict__ gscale, float* __restrict__ gy) {
+  const long N = g.N();
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)B * N) return;
+  const long e = t % N;
+  const int b = (int)(t / N);
+  int i, j, k;
+  grid_ijk(g, e, i, j, k);
+  const float d = gscale[b] * diff[t];
+  const float* rb = r + (long)b * N;
+#pragma unroll
+  for (int s = 0; s < 27; s++) {
+    const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+    const bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
+    gy[((long)b * 27 + s) * N + e] = in ? d * rb[ii + (long)g.nx * (jj + (long)g.ny * kk)] : 0.0f;
+  }
+}
+
+// preconditioner coefficients a[s][e] = -(delta_s,13 + y[s][e]) / sqrt(S_e S_e+off(s)) (0 outside):
+// z = sum_s a_s q_e+s is -M q (the Schur block's sign convention, S approximated by -approxS)
+__global__ void k_snn_coef(Grid3 g, const float* __restrict__ y, const double* __restrict__ S, float* __restrict__ a) {
+  const long N = g.N();
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  int i, j, k;
+  grid_ijk(g, e, i, j, k);
+  const double de = 1.0 / sqrt(S[e]);
+#pragma unroll
+  for (int s = 0; s < 27; s++) {
+    const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+    float v = 0.0f;
+    if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz) {
+      const double dn = 1.0 / sqrt(S[ii + (long)g.nx * (jj + (long)g.ny * kk)]);
+      v = (float)(-(double)((s == 13 ? 1.0f : 0.0f) + y[s * N + e]) * de * dn);
+    }
+    a[s * N + e] = v;
+  }
+}
+
+// z_e = sum_s a[s][e] q_e+off(s)  (T = double for the FP64 Krylov iteration, float for FP32)
+template <typename T>
+__global__ __launch_bounds__(256) void k_snn_apply(Grid3 g, const float* __restrict__ a, const T* __restrict__ q,
+                                                   T* __restrict__ z) {
+  const long N = g.N();
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  int i, j, k;
+  grid_ijk(g, e, i, j, k);
+  T acc = (T)0;
+#pragma unroll
+  for (int s = 0; s < 27; s++) {
+    const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+    const bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
+    const long n = in ? ii + (long)g.nx * (jj + (long)g.ny * kk) : e;
+    const T qv = q[n];
+    acc += in ? (T)a[s * N + e] * qv : (T)0;
+  }
+  z[e] = acc;
+}
+
+struct SchurNet {
+  int C = 16, L = 2;
+  long nparams = 0;
+  std::vector<long> woff, boff;  // per layer (L 3x3x3 convs, then the head)
+  float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;
+  long step = 0;
+  // activations for up to bmax samples of n elements
+  long ncap = 0;
+  int bmax = 0;
+  std::vector<float*> act, gact;
+  float *xin = nullptr, *y = nullptr, *gy = nullptr, *diff = nullptr, *lossb = nullptr, *gscale = nullptr;
+  double* part = nullptr;
+  // training set (all samples on one grid)
+  Grid3 dg{0, 0, 0};
+  long nsamples = 0, scap = 0;
+  float *dx = nullptr, *dr = nullptr, *dt = nullptr;
+};
+
+static void snn_specs(SchurNet* m) {
+  m->woff.clear();
+  m->boff.clear();
+  long off = 0;
+  for (int l = 0; l < m->L; l++) {
+    const int ci = l == 0 ? kSnnFeat : m->C;
+    m->woff.push_back(off);
+    off += (long)ci * m->C * 27;
+    m->boff.push_back(off);
+    off += m->C;
+  }
+  m->woff.push_back(off);
+  off += (long)m->C * 27;
+  m->boff.push_back(off);
+  off += 27;
+  m->nparams = off;
+}
+
+static void snn_free_batch(SchurNet* m) {
+  for (auto v : {&m->act, &m->gact})
+    for (auto p : *v) (void)hipFree(p);
+  m->act.clear();
+  m->gact.clear();
+  for (void* p : {(void*)m->xin, (void*)m->y, (void*)m->gy, (void*)m->diff, (void*)m->lossb, (void*)m->gscale,
+                  (void*)m->part})
+    (void)hipFree(p);
+  m->xin = m->y = m->gy = m->diff = m->lossb = m->gscale = nullptr;
+  m->part = nullptr;
+  m->bmax = 0;
+  m->ncap = 0;
+}
+
+static void snn_ensure(SchurNet* m, int B, long N, bool train) {
+  if (B <= m->bmax && N <= m->ncap && (!train || m->gy)) return;
+  snn_free_batch(m);
+  const long n = (long)B * N;
+  for (int l = 0; l < m->L; l++) {
+    m->act.push_back(nalloc<float>(n * m->C));
+    m->gact.push_back(train ? nalloc<float>(n * m->C) : nullptr);
+  }
+  m->xin = nalloc<float>(n * kSnnFeat);
+  m->y = nalloc<float>(n * 27);
+  if (train) {
+    m->gy = nalloc<float>(n * 27);
+    m->diff = nalloc<float>(n);
+    m->part = nalloc<double>((long)B * nbk(N, 256) * 2);
+  }
+  m->lossb = nalloc<float>(B);
+  m->gscale = nalloc<float>(B);
+  m->bmax = B;
+  m->ncap = N;
+}
+
+static void conv3_fwd(int B, int ci, int co, Grid3 g, const float* in, const float* w, const float* bias, float* out,
+                      bool relu, hipStream_t s) {
+  const long nb = (long)B * g.N();
+  if (co % 8 == 0)
+    hipLaunchKernelGGL(k_conv_fwd_blk<8>, dim3(nbk(nb, 256), co / 8), 256, 0, s, B, ci, co, g, in, w, bias, out,
+                       relu ? 1 : 0);
+  else if (co % 4 == 0)
+    hipLaunchKernelGGL(k_conv_fwd_blk<4>, dim3(nbk(nb, 256), co / 4), 256, 0, s, B, ci, co, g, in, w, bias, out,
+                       relu ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_conv_fwd, nbk(nb * co, 256), 256, 0, s, B, ci, co, g, in, w, bias, out, relu ? 1 : 0);
+}
+
+// y[B][27][N] from xin[B][9][N]
+static void snn_forward(SchurNet* m, int B, Grid3 g, hipStream_t s) {
+  const float* in = m->xin;
+  for (int l = 0; l < m->L; l++) {
+    conv3_fwd(B, l == 0 ? kSnnFeat : m->C, m->C, g, in, m->params + m->woff[l], m->params + m->boff[l], m->act[l], true,
+              s);
+    in = m->act[l];
+  }
+  const long nb = (long)B * g.N();
+  hipLaunchKernelGGL(k_conv1_fwd<9>, dim3(nbk(nb, 256), 3), 256, 0, s, B, m->C, 27, g.N(), in,
+                     m->params + m->woff[m->L], m->params + m->boff[m->L], m->y);
+  UWG_HIP(hipGetLastError());
+}
+
+// loss of the batch (xin, r, t) and, if grads, parameter gradients into m->grads
+static void snn_loss_backward(SchurNet* m, int B, Grid3 g, const float* r, const float* t, bool grads,
+                              hipStream_t s) {
+  const long N = g.N(), nb = (long)B * N;
+  const int nblk = (int)nbk(N, 256);
+  hipLaunchKernelGGL(k_snn_stencil_diff, dim3(nblk, B), 256, 0, s, g, m->y, r, t, m->diff, m->part);
+  hipLaunchKernelGGL(k_snn_loss_reduce, B, 256, 0, s, B, nblk, m->part, m->lossb, m->gscale);
+  if (!grads) {
+    UWG_HIP(hipGetLastError());
+    return;
+  }
+  hipLaunchKernelGGL(k_snn_stencil_grad, nbk(nb, 256), 256, 0, s, B, g, m->diff, r, m->gscale, m->gy);
+  // head (1x1x1)
+  const int L = m->L, C = m->C;
+  hipLaunchKernelGGL(k_conv1_bwd_w, 27 * C, 256, 0, s, B, C, 27, N, m->gy, m->act[L - 1], m->grads + m->woff[L]);
+  hipLaunchKernelGGL(k_bias_grad, 27, 256, 0, s, B, 27, N, m->gy, m->grads + m->boff[L]);
+  hipLaunchKernelGGL(k_conv1_bwd_data, nbk(nb, 256), 256, 0, s, B, C, 27, N, m->gy, m->params + m->woff[L],
+                     m->gact[L - 1]);
+  for (int l = L - 1; l >= 0; l--) {
+    const int ci = l == 0 ? kSnnFeat : C;
+    hipLaunchKernelGGL(k_relu_bwd, nbk(nb * C, 256), 256, 0, s, nb * C, m->act[l], m->gact[l]);
+    const float* in = l == 0 ? m->xin : m->act[l - 1];
+    hipLaunchKernelGGL(k_conv_bwd_w, C * ci, 256, 0, s, B, ci, C, g, m->gact[l], in, m->grads + m->woff[l]);
+    hipLaunchKernelGGL(k_bias_grad, C, 256, 0, s, B, C, N, m->gact[l], m->grads + m->boff[l]);
+    if (l > 0) {
+      if (ci % 4 == 0)
+        hipLaunchKernelGGL(k_conv_bwd_data_blk<4>, dim3(nbk(nb, 256), ci / 4), 256, 0, s, B, ci, C, g, m->gact[l],
+                           m->params + m->woff[l], m->gact[l - 1]);
+      else
+        hipLaunchKernelGGL(k_conv_bwd_data, nbk(nb * ci, 256), 256, 0, s, B, ci, C, g, m->gact[l],
+                           m->params + m->woff[l], m->gact[l - 1]);
+    }
+  }
+  UWG_HIP(hipGetLastError());
+}
+
+static SchurNet* snn_alloc(int C, int L) {
+  if (C < 1 || C > 128 || C % 8 != 0 || L < 1 || L > 6) throw std::runtime_error("bad Schur-stencil network shape");
+  SchurNet* m = new SchurNet();
+  m->C = C;
+  m->L = L;
+  snn_specs(m);
+  m->params = nalloc<float>(m->nparams);
+  m->grads = nalloc<float>(m->nparams);
+  m->m1 = nalloc<float>(m->nparams);
+  m->m2 = nalloc<float>(m->nparams);
+  UWG_HIP(hipMemset(m->grads, 0, sizeof(float) * m->nparams));
+  UWG_HIP(hipMemset(m->m1, 0, sizeof(float) * m->nparams));
+  UWG_HIP(hipMemset(m->m2, 0, sizeof(float) * m->nparams));
+  return m;
+}
+
+void schur_net_destroy(SchurNet* m) {
+  if (!m) return;
+  snn_free_batch(m);
+  for (void* p : {(void*)m->params, (void*)m->grads, (void*)m->m1, (void*)m->m2, (void*)m->dx, (void*)m->dr,
+                  (void*)m->dt})
+    (void)hipFree(p);
+  delete m;
+}
+
+SchurNet* schur_net_create(int C, int L, const float* params, long nparams) {
+  SchurNet* m = snn_alloc(C, L);
+  if (nparams != m->nparams) {
+    const long want = m->nparams;
+    schur_net_destroy(m);
+    throw std::runtime_error("Schur-stencil model has " + std::to_string(nparams) + " parameters, the network needs " +
+                             std::to_string(want));
+  }
+  UWG_HIP(hipMemcpy(m->params, params, sizeof(float) * nparams, hipMemcpyHostToDevice));
+  return m;
+}
+
+void schur_net_features(const Geom& g, const double* approxS, float* feat, hipStream_t s) {
+  const Grid3 eg{g.res[0], g.res[1], g.res[2]};
+  hipLaunchKernelGGL(k_snn_features, nbk(eg.N(), 256), 256, 0, s, eg, approxS, feat);
+  UWG_HIP(hipGetLastError());
+}
+
+void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, float* a, hipStream_t s) {
+  if (g.dim != 3) throw std::runtime_error("the learned Schur stencil is provided for 3D meshes");
+  const Grid3 eg{g.res[0], g.res[1], g.res[2]};
+  const long N = eg.N();
+  snn_ensure(m, 1, N, false);
+  schur_net_features(g, approxS, m->xin, s);
+  snn_forward(m, 1, eg, s);
+  hipLaunchKernelGGL(k_snn_coef, nbk(N, 256), 256, 0, s, eg, m->y, approxS, a);
+  UWG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void schur_stencil_apply(const Geom& g, const float* a, const T* q, T* z, hipStream_t s) {
+  const Grid3 eg{g.res[0], g.res[1], g.res[2]};
+  hipLaunchKernelGGL(k_snn_apply<T>, nbk(eg.N(), 256), 256, 0, s, eg, a, q, z);
+  UWG_HIP(hipGetLastError());
+}
+template void schur_stencil_apply<double>(const Geom&, const float*, const double*, double*, hipStream_t);
+template void schur_stencil_apply<float>(const Geom&, const float*, const float*, float*, hipStream_t);
+
+// training set: append one sample (features from approxS, r^ and e^ given on the device, FP64)
+__global__ void k_d2f_copy(long n, const double* __restrict__ a, float* __restrict__ b) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (float)a[t];
+}
+
+void schur_net_append(SchurNet* m, const Geom& g, const double* approxS, const double* rhat, const double* ehat) {
+  const Grid3 eg{g.res[0], g.res[1], g.res[2]};
+  const long N = eg.N();
+  if (m->nsamples > 0 && (eg.nx != m->dg.nx || eg.ny != m->dg.ny || eg.nz != m->dg.nz))
+    throw std::runtime_error("all training samples of a Schur-stencil set must share one element grid");
+  if (m->nsamples == m->scap) {
+    const long cap = std::max<long>(8, m->scap * 2);
+    float* nx_ = nalloc<float>(cap * kSnnFeat * N);
+    float* nr = nalloc<float>(cap * N);
+    float* nt = nalloc<float>(cap * N);
+    if (m->nsamples) {
+      UWG_HIP(hipMemcpy(nx_, m->dx, sizeof(float) * m->nsamples * kSnnFeat * N, hipMemcpyDeviceToDevice));
+      UWG_HIP(hipMemcpy(nr, m->dr, sizeof(float) * m->nsamples * N, hipMemcpyDeviceToDevice));
+      UWG_HIP(hipMemcpy(nt, m->dt, sizeof(float) * m->nsamples * N, hipMemcpyDeviceToDevice));
+    }
+    for (void* p : {(void*)m->dx, (void*)m->dr, (void*)m->dt}) (void)hipFree(p);
+    m->dx = nx_;
+    m->dr = nr;
+    m->dt = nt;
+    m->scap = cap;
+  }
+  m->dg = eg;
+  const long k = m->nsamples;
+  schur_net_features(g, approxS, m->dx + k * kSnnFeat * N, 0);
+  hipLaunchKernelGGL(k_d2f_copy, nbk(N, 256), 256, 0, 0, N, rhat, m->dr + k * N);
+  hipLaunchKernelGGL(k_d2f_copy, nbk(N, 256), 256, 0, 0, N, ehat, m->dt + k * N);
+  UWG_HIP(hipGetLastError());
+  UWG_HIP(hipDeviceSynchronize());
+  m->nsamples++;
+}
+
 }  // namespace uwg
 
 // ------------------------------------------------------------------------------------------
@@ -948,6 +1419,221 @@ extern "C" int uwg_nn_eval(uwg_nn* h, int B, float* loss) {
     for (int b = 0; b < nb; b++) acc += lb[b];
   }
   *loss = (float)(acc / h->nsamples);
+  return 0;
+  NN_API_END
+}
+
+// ------------------------------------------------------------------------------------------
+// C ABI for the learned Schur stencil: training, evaluation and test hooks
+// ------------------------------------------------------------------------------------------
+extern "C" int uwg_snn_create(int channels, int layers, long seed, uwg_snn** out) {
+  if (channels < 8 || channels > 128 || channels % 8 != 0 || layers < 1 || layers > 6) {
+    uwg::set_error("bad Schur-stencil network shape (channels a multiple of 8 in [8, 128], 1-6 layers)");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  uwg_snn* h = new uwg_snn();
+  h->m = snn_alloc(channels, layers);
+  SchurNet* m = h->m;
+  // He-normal 3x3x3 layers from a 64-bit LCG (deterministic for a seed); zero head, so the
+  // untrained network is the identity stencil (the plain approxS preconditioner)
+  std::vector<float> p(m->nparams, 0.0f);
+  unsigned long long st = (unsigned long long)seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  auto unif = [&]() {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return ((st >> 11) + 0.5) / 9007199254740992.0;
+  };
+  for (int l = 0; l < m->L; l++) {
+    const int ci = l == 0 ? kSnnFeat : m->C;
+    const double sd = sqrt(2.0 / (ci * 27.0));
+    for (long k = 0; k < (long)ci * m->C * 27; k++) {
+      const double u1 = unif(), u2 = unif();
+      p[m->woff[l] + k] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+  }
+  UWG_HIP(hipMemcpy(m->params, p.data(), sizeof(float) * p.size(), hipMemcpyHostToDevice));
+  *out = h;
+  return 0;
+  NN_API_END
+}
+
+extern "C" int uwg_snn_destroy(uwg_snn* h) {
+  if (!h) return 0;
+  schur_net_destroy(h->m);
+  delete h;
+  return 0;
+}
+
+extern "C" int uwg_snn_param_count(uwg_snn* h, long* n) {
+  *n = h->m->nparams;
+  return 0;
+}
+
+extern "C" int uwg_snn_get_params(uwg_snn* h, float* out) {
+  NN_API_BEGIN
+  UWG_HIP(hipMemcpy(out, h->m->params, sizeof(float) * h->m->nparams, hipMemcpyDeviceToHost));
+  return 0;
+  NN_API_END
+}
+
+extern "C" int uwg_snn_set_params(uwg_snn* h, const float* in) {
+  NN_API_BEGIN
+  UWG_HIP(hipMemcpy(h->m->params, in, sizeof(float) * h->m->nparams, hipMemcpyHostToDevice));
+  return 0;
+  NN_API_END
+}
+
+// feat[9][N] of an element grid from approxS[N] (host arrays)
+extern "C" int uwg_snn_features(int nx, int ny, int nz, const double* approxS, float* feat) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    uwg::set_error("bad element grid");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  const Grid3 eg{nx, ny, nz};
+  const long N = eg.N();
+  double* dS = nalloc<double>(N);
+  float* df = nalloc<float>(kSnnFeat * N);
+  UWG_HIP(hipMemcpy(dS, approxS, sizeof(double) * N, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_snn_features, nbk(N, 256), 256, 0, 0, eg, dS, df);
+  UWG_HIP(hipGetLastError());
+  UWG_HIP(hipMemcpy(feat, df, sizeof(float) * kSnnFeat * N, hipMemcpyDeviceToHost));
+  (void)hipFree(dS);
+  (void)hipFree(df);
+  return 0;
+  NN_API_END
+}
+
+// y[B][27][N] = network(x[B][9][N]) on an nx x ny x nz element grid (host arrays)
+extern "C" int uwg_snn_forward(uwg_snn* h, int nx, int ny, int nz, int B, const float* x, float* y) {
+  if (nx < 1 || ny < 1 || nz < 1 || B < 1) {
+    uwg::set_error("bad grid or batch");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  SchurNet* m = h->m;
+  const Grid3 g{nx, ny, nz};
+  const long N = g.N();
+  snn_ensure(m, B, N, false);
+  UWG_HIP(hipMemcpy(m->xin, x, sizeof(float) * B * kSnnFeat * N, hipMemcpyHostToDevice));
+  snn_forward(m, B, g, 0);
+  UWG_HIP(hipMemcpy(y, m->y, sizeof(float) * B * 27 * N, hipMemcpyDeviceToHost));
+  return 0;
+  NN_API_END
+}
+
+// mean over the batch of |C(x) r - t|^2 / |t|^2 and its parameter gradient (host arrays)
+extern "C" int uwg_snn_loss_grad(uwg_snn* h, int nx, int ny, int nz, int B, const float* x, const float* r,
+                                 const float* t, float* loss, float* grads) {
+  if (nx < 1 || ny < 1 || nz < 1 || B < 1) {
+    uwg::set_error("bad grid or batch");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  SchurNet* m = h->m;
+  const Grid3 g{nx, ny, nz};
+  const long N = g.N();
+  snn_ensure(m, B, N, true);
+  float* dr = nalloc<float>((long)B * N);
+  float* dt = nalloc<float>((long)B * N);
+  UWG_HIP(hipMemcpy(m->xin, x, sizeof(float) * B * kSnnFeat * N, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(dr, r, sizeof(float) * B * N, hipMemcpyHostToDevice));
+  UWG_HIP(hipMemcpy(dt, t, sizeof(float) * B * N, hipMemcpyHostToDevice));
+  snn_forward(m, B, g, 0);
+  snn_loss_backward(m, B, g, dr, dt, grads != nullptr, 0);
+  std::vector<float> lb(B);
+  UWG_HIP(hipMemcpy(lb.data(), m->lossb, sizeof(float) * B, hipMemcpyDeviceToHost));
+  double acc = 0.0;
+  for (float v : lb) acc += v;
+  *loss = (float)(acc / B);
+  if (grads) UWG_HIP(hipMemcpy(grads, m->grads, sizeof(float) * m->nparams, hipMemcpyDeviceToHost));
+  (void)hipFree(dr);
+  (void)hipFree(dt);
+  return 0;
+  NN_API_END
+}
+
+extern "C" int uwg_snn_sample_count(uwg_snn* h, long* n) {
+  *n = h->m->nsamples;
+  return 0;
+}
+
+extern "C" int uwg_snn_clear_data(uwg_snn* h) {
+  SchurNet* m = h->m;
+  for (void* p : {(void*)m->dx, (void*)m->dr, (void*)m->dt}) (void)hipFree(p);
+  m->dx = m->dr = m->dt = nullptr;
+  m->nsamples = m->scap = 0;
+  return 0;
+}
+
+// Adam on minibatches of B samples drawn (deterministically from seed) from samples [0, ntrain)
+// of the stored set; losses[steps] receives each step's batch loss (may be null)
+extern "C" int uwg_snn_train(uwg_snn* h, int steps, int B, float lr, float weight_decay, long seed, long ntrain,
+                             float* losses) {
+  SchurNet* m = h->m;
+  if (ntrain < 1 || ntrain > m->nsamples || B < 1 || steps < 0) {
+    uwg::set_error("bad training arguments (ntrain must be in [1, stored samples])");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  const Grid3 g = m->dg;
+  const long N = g.N();
+  snn_ensure(m, B, N, true);
+  float* br = nalloc<float>((long)B * N);
+  float* bt = nalloc<float>((long)B * N);
+  unsigned long long st = (unsigned long long)seed * 2862933555777941757ULL + 3037000493ULL;
+  std::vector<float> lb(B);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int it = 0; it < steps; it++) {
+    for (int b = 0; b < B; b++) {
+      st = st * 2862933555777941757ULL + 3037000493ULL;
+      const long k = (long)((st >> 17) % (unsigned long long)ntrain);
+      UWG_HIP(hipMemcpyAsync(m->xin + (long)b * kSnnFeat * N, m->dx + k * kSnnFeat * N, sizeof(float) * kSnnFeat * N,
+                             hipMemcpyDeviceToDevice, 0));
+      UWG_HIP(hipMemcpyAsync(br + (long)b * N, m->dr + k * N, sizeof(float) * N, hipMemcpyDeviceToDevice, 0));
+      UWG_HIP(hipMemcpyAsync(bt + (long)b * N, m->dt + k * N, sizeof(float) * N, hipMemcpyDeviceToDevice, 0));
+    }
+    snn_forward(m, B, g, 0);
+    snn_loss_backward(m, B, g, br, bt, true, 0);
+    m->step++;
+    const float c1 = 1.0f - powf(b1, (float)m->step), c2 = 1.0f - powf(b2, (float)m->step);
+    hipLaunchKernelGGL(k_adam, nbk(m->nparams, 256), 256, 0, 0, m->nparams, m->params, m->grads, m->m1, m->m2, lr, b1,
+                       b2, eps, c1, c2, weight_decay);
+    if (losses) {
+      UWG_HIP(hipMemcpy(lb.data(), m->lossb, sizeof(float) * B, hipMemcpyDeviceToHost));
+      double acc = 0.0;
+      for (float v : lb) acc += v;
+      losses[it] = (float)(acc / B);
+    }
+  }
+  UWG_HIP(hipDeviceSynchronize());
+  (void)hipFree(br);
+  (void)hipFree(bt);
+  return 0;
+  NN_API_END
+}
+
+// mean loss over samples [first, first + count) of the stored set
+extern "C" int uwg_snn_eval(uwg_snn* h, long first, long count, float* loss) {
+  SchurNet* m = h->m;
+  if (first < 0 || count < 1 || first + count > m->nsamples) {
+    uwg::set_error("bad sample range");
+    return UWG_ERR_ARG;
+  }
+  NN_API_BEGIN
+  const Grid3 g = m->dg;
+  const long N = g.N();
+  snn_ensure(m, 1, N, true);
+  double acc = 0.0;
+  float lb = 0.0f;
+  for (long k = first; k < first + count; k++) {
+    UWG_HIP(hipMemcpy(m->xin, m->dx + k * kSnnFeat * N, sizeof(float) * kSnnFeat * N, hipMemcpyDeviceToDevice));
+    snn_forward(m, 1, g, 0);
+    snn_loss_backward(m, 1, g, m->dr + k * N, m->dt + k * N, false, 0);
+    UWG_HIP(hipMemcpy(&lb, m->lossb, sizeof(float), hipMemcpyDeviceToHost));
+    acc += lb;
+  }
+  *loss = (float)(acc / count);
   return 0;
   NN_API_END
 }

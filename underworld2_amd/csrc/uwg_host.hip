@@ -10,6 +10,7 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 
 #include "../../include/uwg.h"
@@ -182,6 +183,11 @@ struct uwg_stokes {
   // learned correction
   std::unique_ptr<NNModel, NNDeleter> nn;
   int nn_level = -1;
+  // learned Schur stencil: network, its coefficients a[27][E] for the current assembly
+  std::unique_ptr<SchurNet, SchurNetDeleter> snn;
+  float* snn_a = nullptr;
+  long snn_stamp = -1;        // assembly_stamp the coefficients were computed for
+  long assembly_stamp = 0;    // bumped by every assembly (approxS changes)
   int vcycles = 0;
   int nn_rejects = 0;
   uint8_t* bc0 = nullptr;     // level-0 Dirichlet mask (owned here, shared with L[0].bc)
@@ -1340,6 +1346,23 @@ static void top_schur(uwg_stokes* st, const T* diagS, const T* r, T* z) {
     vk_xmy_scale<T>(st->np, diagS, r, z, 0);
 }
 
+// Schur block of the coupled preconditioner: the learned stencil while it is enabled for this
+// solve (its coefficients follow the assembly), else the diagonal -approxS^-1
+static bool g_snn_active = false;
+static void ensure_snn_coefficients(uwg_stokes* st) {
+  if (st->snn_stamp == st->assembly_stamp && st->snn_a) return;
+  if (!st->snn_a) st->snn_a = dalloc<float>(27 * st->g.E);
+  schur_net_coefficients(st->snn.get(), st->g, st->approxS, st->snn_a, 0);
+  st->snn_stamp = st->assembly_stamp;
+}
+template <typename T>
+static void schur_prec(uwg_stokes* st, const T* diagS, const T* r, T* z) {
+  if (g_snn_active && st->snn && !st->nn_disabled)
+    schur_stencil_apply<T>(st->g, st->snn_a, r, z, 0);
+  else
+    top_schur<T>(st, diagS, r, z);
+}
+
 // Chebyshev smoothing on the Q2 level: same window and recurrence as the Q1 smoother
 template <typename T>
 static void q2_chebyshev(uwg_stokes* st, const T* b, T* x, int degree, bool zero_guess) {
@@ -1662,6 +1685,7 @@ extern "C" int uwg_stokes_destroy(uwg_stokes* st) {
   delete st->kf;
   delete st->kb32;
   dfree(st->capture_buf);
+  dfree(st->snn_a);
   for (int k = 0; k < WS_COUNT; k++) dfree(st->ws_ptr[k]);
   for (void* p : {(void*)st->fe, (void*)st->f, (void*)st->ubc, (void*)st->approxS, (void*)st->h, (void*)st->eta_tab,
                   (void*)st->force_tab, (void*)st->eta_p, (void*)st->force_p, (void*)st->red_partial,
@@ -1827,6 +1851,7 @@ static void finish_assembly(uwg_stokes* st) {
   UWG_HIP(hipMemset(st->h, 0, sizeof(double) * st->g.E));
   setup_hierarchy(st);
   st->assembled = true;
+  st->assembly_stamp++;
 }
 
 // power iteration for lambda_max(D^-1 K) on the Q2 level
@@ -2122,24 +2147,30 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
     }
     top_GT<double>(st, zz, q, bc);
     vk_axpby<double>(E, 1.0, rr + n, -1.0, q, 0);  // q = r_p - G^T z_u
-    top_schur<double>(st, diagS, q, zz + n);
+    schur_prec<double>(st, diagS, q, zz + n);
   };
   int its = 0;
   double rn;
   // learned-correction safeguard: the FP64 residual must keep falling; if 10 iterations
   // reduce it by less than 10 %, the network is switched off and multigrid takes over
   // (FGMRES admits a preconditioner that changes between iterations)
-  std::vector<double> hist;
+  // hist[k] is the relative residual after iteration k (hist[0] = 1, the start), so the test
+  // compares the residual with the one nn_window iterations earlier
+  std::vector<double> hist{1.0};
   std::function<void(int, double)> mon = [&](int k, double rel) {
     hist.push_back(rel);
     const int wdw = st->nn_window;
-    if (o.learned && st->nn && !st->nn_disabled && k >= wdw && rel > (1.0 - st->nn_min_drop) * hist[k - wdw]) {
+    const bool learned_on = st->nn || st->snn;
+    if (o.learned && learned_on && !st->nn_disabled && k >= wdw && k < (int)hist.size() &&
+        rel > (1.0 - st->nn_min_drop) * hist[k - wdw]) {
       st->nn_disabled = true;
       S->fallback_used = 1;
       if (g_verbose) fprintf(stderr, " learned correction disabled at iteration %d (rel %.3e)\n", k, rel);
     }
   };
   st->nn_disabled = !o.learned;
+  g_snn_active = o.learned && st->snn && !st->q2;
+  if (g_snn_active) ensure_snn_coefficients(st);
   int rc;
   if (o.precision == 1 && g_basis_fp32) {
     // FP32 Arnoldi basis: the FP32 cycle reads the basis vector's velocity part as it is (no
@@ -2158,7 +2189,7 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
       top_GT<double>(st, zz, q, bc);
       vk_f2d(E, 1.0, rr + n, rp, 0);
       vk_axpby<double>(E, 1.0, rp, -1.0, q, 0);  // q = r_p - G^T z_u
-      top_schur<double>(st, diagS, q, zz + n);
+      schur_prec<double>(st, diagS, q, zz + n);
     };
     rc = fgmres<double, float>(st, Kb32, Aop, Mop32, b, x, o.outer_rtol, 1e-300, o.max_outer_its, &its, &rn, true,
                                &mon);
@@ -2272,7 +2303,9 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   memset(S, 0, sizeof(*S));
   if (o.restart < 2 || o.restart > 200) return arg_error("restart must be in [2, 200]");
   if (o.method == 0 && o.learned) return arg_error("the learned correction is only available with method 1");
-  if (o.learned && o.precision != 1) return arg_error("the learned correction runs inside the FP32 cycle (precision 1)");
+  if (o.learned && st->nn && o.precision != 1)
+    return arg_error("the learned coarse correction runs inside the FP32 cycle (precision 1)");
+  if (o.learned && o.precision == 2) return arg_error("the learned preconditioners are not provided with precision 2");
   API_BEGIN
   double t0 = now();
   if (o.mg_levels > 0 && o.mg_levels != (int)st->L.size()) {
@@ -2284,7 +2317,7 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
     }
     if (st->assembled) setup_hierarchy(st);
   }
-  if (o.learned && !st->nn) return arg_error("no learned model loaded");
+  if (o.learned && !st->nn && !st->snn) return arg_error("no learned model loaded");
   if (o.learned && st->q2) return arg_error("the learned correction is provided for Q1/dQ0 systems");
   if (st->swarm && st->assembled_state != st->swarm->state && st->assembled_state != -2) {
     int rc = uwg_stokes_assemble(st, st->swarm);
@@ -2622,6 +2655,87 @@ extern "C" int uwg_stokes_set_nn_monitor(uwg_stokes* st, int window, double min_
   st->nn_window = window;
   st->nn_min_drop = min_drop;
   return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// learned Schur stencil: attach a trained network; generate training triples
+// ---------------------------------------------------------------------------------------
+extern "C" int uwg_stokes_load_schur_nn(uwg_stokes* st, int channels, int layers, long nparams, const float* params) {
+  if (st->q2 || st->g.dim != 3) return arg_error("the learned Schur stencil is provided for 3D Q1/dQ0 systems");
+  API_BEGIN
+  st->snn.reset(schur_net_create(channels, layers, params, nparams));
+  st->snn_stamp = -1;
+  return 0;
+  API_END
+}
+
+// x = K^-1 b: FGMRES(30) preconditioned by one FP64 V-cycle (the Schur path's inner solve)
+static int inner_solve_fp64(uwg_stokes* st, const double* b, double* x, double rtol, int* its) {
+  const long n = st->nu;
+  const uint8_t* bc = top_bc(st);
+  if (!st->kin) st->kin = new KrylovSpace();
+  st->kin->init(n, 30);
+  OpFn Kop = [&](const double* xx, double* yy) { top_K<double>(st, xx, yy, bc); };
+  OpFn Mv = [&](const double* xx, double* yy) {
+    double* b64 = top_buf_b<double>(st);
+    vk_copy<double>(n, xx, b64, 0);
+    top_vcycle<double>(st, b64, yy);
+  };
+  vk_fill<double>(n, 0.0, x, 0);
+  double rn;
+  return fgmres(st, *st->kin, Kop, Mv, b, x, rtol, 1e-300, 10000, its, &rn, true);
+}
+
+// Training triples from the plain solver's operators: e^ white noise (seeded) with the
+// constant-pressure null mode removed, e = D^-1/2 e^ (D = approxS), r = S e = G^T K^-1 G e with
+// K^-1 by the inner solve to rtol, r^ = D^-1/2 r; the sample (features(approxS), r^, e^) is
+// appended to the network's training set.
+extern "C" int uwg_snn_add_samples(uwg_snn* h, uwg_stokes* st, int nsamples, long seed, double rtol) {
+  if (!st->assembled) return arg_error("system not assembled");
+  if (st->q2 || st->g.dim != 3) return arg_error("the learned Schur stencil is provided for 3D Q1/dQ0 systems");
+  if (nsamples < 1) return arg_error("nsamples must be >= 1");
+  API_BEGIN
+  const long E = st->np, n = st->nu;
+  const uint8_t* bc = top_bc(st);
+  std::vector<double> S(E), sq(E), eh(E), e(E), r(E);
+  UWG_HIP(hipMemcpy(S.data(), st->approxS, sizeof(double) * E, hipMemcpyDeviceToHost));
+  double nn2 = 0.0;
+  for (long i = 0; i < E; i++) {
+    sq[i] = sqrt(S[i]);
+    nn2 += S[i];
+  }
+  double* de = dalloc<double>(E);
+  double* deh = dalloc<double>(E);
+  double* dr = dalloc<double>(E);
+  double* t = dalloc<double>(n);
+  double* z = dalloc<double>(n);
+  std::mt19937_64 gen((unsigned long long)seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (int k = 0; k < nsamples; k++) {
+    double proj = 0.0;
+    for (long i = 0; i < E; i++) {
+      eh[i] = nd(gen);
+      proj += eh[i] * sq[i];
+    }
+    proj /= nn2;  // remove the component along D^1/2 1 (the constant pressure)
+    for (long i = 0; i < E; i++) {
+      eh[i] -= proj * sq[i];
+      e[i] = eh[i] / sq[i];
+    }
+    UWG_HIP(hipMemcpy(de, e.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+    top_G<double>(st, de, t, bc);
+    int its = 0;
+    inner_solve_fp64(st, t, z, rtol, &its);
+    top_GT<double>(st, z, dr, bc);
+    UWG_HIP(hipMemcpy(r.data(), dr, sizeof(double) * E, hipMemcpyDeviceToHost));
+    for (long i = 0; i < E; i++) r[i] /= sq[i];
+    UWG_HIP(hipMemcpy(dr, r.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+    UWG_HIP(hipMemcpy(deh, eh.data(), sizeof(double) * E, hipMemcpyHostToDevice));
+    schur_net_append(h->m, st->g, st->approxS, dr, deh);
+  }
+  for (void* p : {(void*)de, (void*)deh, (void*)dr, (void*)t, (void*)z}) dfree(p);
+  return 0;
+  API_END
 }
 
 extern "C" int uwg_stokes_capture(uwg_stokes* st, int level, int kind, int max_samples, int every) {

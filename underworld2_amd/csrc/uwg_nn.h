@@ -22,4 +22,26 @@ struct NNDeleter {
   void operator()(NNModel* m) const { nn_destroy(m); }
 };
 
+// Learned Schur-complement stencil on the element grid (nn_kernels.hip)
+struct SchurNet;
+SchurNet* schur_net_create(int channels, int layers, const float* params, long nparams);
+void schur_net_destroy(SchurNet* m);
+// feat[9][E] from approxS[E] (device arrays)
+void schur_net_features(const Geom& g, const double* approxS, float* feat, hipStream_t s);
+// preconditioner coefficients a[27][E] (FP32): z = sum_s a_s q_e+s approximates -S^-1 q
+void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, float* a, hipStream_t s);
+template <typename T>
+void schur_stencil_apply(const Geom& g, const float* a, const T* q, T* z, hipStream_t s);
+// append one training sample: features of approxS, r^ = D^-1/2 S e, e^ = D^1/2 e (device, FP64)
+void schur_net_append(SchurNet* m, const Geom& g, const double* approxS, const double* rhat, const double* ehat);
+
+struct SchurNetDeleter {
+  void operator()(SchurNet* m) const { schur_net_destroy(m); }
+};
+
 }  // namespace uwg
+
+// training handle of the C ABI
+struct uwg_snn {
+  uwg::SchurNet* m;
+};

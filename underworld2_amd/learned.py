@@ -290,6 +290,23 @@ def gaussian_random_field(nodes, corr_len, rng):
     return np.ascontiguousarray(f.transpose(2, 1, 0))  # [z][y][x]
 
 
+_G3 = np.array([-np.sqrt(0.6), 0.0, np.sqrt(0.6)])
+
+
+def _gauss_offsets(mesh):
+    """Physical offsets of the 27 Gauss points from the element centre [27][3] (q = qx + 3 qy +
+    9 qz) and the element centres [E][3] (x fastest)."""
+    res = np.array(mesh.elementRes)
+    lo, hi = np.array(mesh.minCoord), np.array(mesh.maxCoord)
+    h = (hi - lo) / res
+    q = np.arange(27)
+    off = np.stack([_G3[q % 3], _G3[(q // 3) % 3], _G3[q // 9]], 1) * (0.5 * h)[None, :]
+    k, j, i = np.meshgrid(np.arange(res[2]), np.arange(res[1]), np.arange(res[0]), indexing="ij")
+    cen = np.stack([lo[0] + (i.ravel() + 0.5) * h[0], lo[1] + (j.ravel() + 0.5) * h[1],
+                    lo[2] + (k.ravel() + 0.5) * h[2]], 1)
+    return off, cen
+
+
 class NodalField(Function):
     """A scalar field given by its values on the node grid of the unit box, evaluated by
     trilinear interpolation (a host-side helper for building training problems)."""
@@ -297,6 +314,29 @@ class NodalField(Function):
     def __init__(self, values_zyx, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
         self.v = np.asarray(values_zyx, dtype=np.float64)
         self.lo, self.hi = np.asarray(lo, float), np.asarray(hi, float)
+
+    def _gauss_values(self, mesh):
+        """[E][27] values at the Gauss points of a 3D Q1 mesh whose nodes are this field's nodes:
+        the trilinear interpolant at each point is a fixed combination of the element's corners."""
+        nz, ny, nx = self.v.shape
+        if mesh.dim != 3 or mesh._order != 1 or tuple(mesh.elementRes) != (nx - 1, ny - 1, nz - 1) or \
+                not np.allclose(mesh.minCoord, self.lo) or not np.allclose(mesh.maxCoord, self.hi):
+            return self._fn_eval(type("C", (), {"coords": _gauss_points_of(mesh)})())[:, 0]
+        t = 0.5 * (1.0 + _G3)  # position of the Gauss points in [0, 1] along an element edge
+        out = np.empty((nx - 1) * (ny - 1) * (nz - 1) * 27).reshape(-1, 27)
+        v = self.v
+        for q in range(27):
+            tx, ty, tz = t[q % 3], t[(q // 3) % 3], t[q // 9]
+            acc = 0.0
+            for dz in (0, 1):
+                wz = tz if dz else 1.0 - tz
+                for dy in (0, 1):
+                    wy = ty if dy else 1.0 - ty
+                    for dx in (0, 1):
+                        wx = tx if dx else 1.0 - tx
+                        acc = acc + (wx * wy * wz) * v[dz:nz - 1 + dz, dy:ny - 1 + dy, dx:nx - 1 + dx]
+            out[:, q] = acc.reshape(-1)
+        return out
 
     def _fn_eval(self, ctx):
         x = ctx.coords
@@ -316,9 +356,25 @@ class NodalField(Function):
         return out[:, None]
 
 
+def _gauss_points_of(mesh):
+    off, cen = _gauss_offsets(mesh)
+    return (cen[:, None, :] + off[None, :, :]).reshape(-1, 3)
+
+
 class _SpheresForce(Function):
     def __init__(self, centres, radii):
         self.c, self.r = np.asarray(centres, float), np.asarray(radii, float)
+
+    def _gauss_values(self, mesh):
+        off, cen = _gauss_offsets(mesh)
+        out = np.zeros((cen.shape[0], 27, 3))
+        for q in range(27):
+            x = cen + off[q]
+            rho = np.zeros(x.shape[0])
+            for c, r in zip(self.c, self.r):
+                rho = np.maximum(rho, (np.sum((x - c) ** 2, axis=1) < r * r).astype(float))
+            out[:, q, 2] = -rho
+        return out
 
     def _fn_eval(self, ctx):
         x = ctx.coords
@@ -374,3 +430,183 @@ def build_training_set(res, level_res, kind, seeds, samples_per_problem=16, ever
             M.append(m)
         del solver, st
     return np.concatenate(X), np.concatenate(T), np.concatenate(M)
+
+
+# ------------------------------------------------------------------------------------------
+# Learned Schur-complement stencil
+# ------------------------------------------------------------------------------------------
+SCHUR_PATH = os.path.join(_HERE, "trained", "schur_stencil.npz")
+
+
+class StencilNet:
+    """The Schur-stencil network of the library (uwg_snn_*): 9 input features per element,
+    ``layers`` 3x3x3 convolutions of ``channels`` channels with ReLU, a 1x1x1 head to the 27
+    offsets of the element's stencil (identity + head).  Holds its training set on the device."""
+
+    def __init__(self, channels=16, layers=2, seed=0, params=None):
+        self.channels, self.layers = int(channels), int(layers)
+        h = ctypes.c_void_p()
+        L.check(L.lib().uwg_snn_create(self.channels, self.layers, int(seed), ctypes.byref(h)), "StencilNet")
+        self._h = h
+        n = ctypes.c_long()
+        L.check(L.lib().uwg_snn_param_count(self._h, ctypes.byref(n)))
+        self.nparams = n.value
+        if params is not None:
+            self.set_params(params)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.lib().uwg_snn_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        L.check(L.lib().uwg_snn_get_params(self._h, L.fptr(p)), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {p.size}")
+        L.check(L.lib().uwg_snn_set_params(self._h, L.fptr(p)), "set_params")
+
+    NFEAT = 9
+
+    @staticmethod
+    def features(grid, approxS):
+        """[9][N] input features of an element grid (nx, ny, nz) from approxS [N]."""
+        N = int(np.prod(grid))
+        S = np.ascontiguousarray(approxS, dtype=np.float64).reshape(N)
+        f = np.zeros((9, N), dtype=np.float32)
+        L.check(L.lib().uwg_snn_features(*[int(v) for v in grid], L.dptr(S), L.fptr(f)), "features")
+        return f
+
+    def forward(self, grid, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        N = int(np.prod(grid))
+        y = np.zeros((x.shape[0], 27, N), dtype=np.float32)
+        L.check(L.lib().uwg_snn_forward(self._h, *[int(v) for v in grid], x.shape[0], L.fptr(x), L.fptr(y)),
+                "forward")
+        return y
+
+    def loss_grad(self, grid, x, r, t):
+        x, r, t = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, r, t))
+        loss = ctypes.c_float()
+        g = np.zeros(self.nparams, dtype=np.float32)
+        L.check(L.lib().uwg_snn_loss_grad(self._h, *[int(v) for v in grid], x.shape[0], L.fptr(x), L.fptr(r),
+                                          L.fptr(t), ctypes.byref(loss), L.fptr(g)), "loss_grad")
+        return loss.value, g
+
+    def add_samples(self, stokes, nsamples=2, seed=0, inner_rtol=1e-6):
+        """(features, r^, e^) triples from an assembled 3D Stokes system, stored on the device:
+        e^ random, r^ = D^-1/2 S D^-1/2 e^ applied by the plain solver (S = G^T K^-1 G)."""
+        stokes._prepare_values()
+        L.check(L.lib().uwg_snn_add_samples(self._h, stokes._h, int(nsamples), int(seed), float(inner_rtol)),
+                "add_samples")
+
+    @property
+    def sample_count(self):
+        n = ctypes.c_long()
+        L.check(L.lib().uwg_snn_sample_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def clear_data(self):
+        L.check(L.lib().uwg_snn_clear_data(self._h))
+
+    def fit(self, steps, batch=4, lr=1e-3, weight_decay=0.0, seed=0, ntrain=None):
+        """Adam on minibatches of the first ``ntrain`` stored samples; per-step batch losses."""
+        ntrain = self.sample_count if ntrain is None else int(ntrain)
+        losses = np.zeros(steps, dtype=np.float32)
+        L.check(L.lib().uwg_snn_train(self._h, int(steps), int(batch), float(lr), float(weight_decay), int(seed),
+                                      ntrain, L.fptr(losses)), "fit")
+        return losses
+
+    def evaluate(self, first, count):
+        loss = ctypes.c_float()
+        L.check(L.lib().uwg_snn_eval(self._h, int(first), int(count), ctypes.byref(loss)), "evaluate")
+        return loss.value
+
+
+class LearnedSchurStencil:
+    """Trained parameters of the Schur-stencil network.  ``attach(stokes)`` makes the coupled
+    solver's Schur block use the learned stencil whenever the solve runs with the learned
+    option (``Solver.set_learned``); its coefficients are recomputed after every assembly."""
+
+    def __init__(self, params, channels, layers, meta=None):
+        self.params = np.ascontiguousarray(params, dtype=np.float32)
+        self.channels, self.layers = int(channels), int(layers)
+        self.meta = dict(meta or {})
+
+    def attach(self, stokes):
+        L.check(L.lib().uwg_stokes_load_schur_nn(stokes._h, self.channels, self.layers, self.params.size,
+                                                 L.fptr(self.params)), "load learned Schur stencil")
+        stokes._snn_model = self
+
+    def save(self, path):
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        np.savez_compressed(path, params=self.params, channels=self.channels, layers=self.layers,
+                            meta=np.array([f"{k}={v}" for k, v in sorted(self.meta.items())]))
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            meta = {}
+            for item in z["meta"]:
+                k, _, v = str(item).partition("=")
+                meta[k] = v
+            return cls(z["params"], int(z["channels"]), int(z["layers"]), meta)
+
+
+def load_default_schur():
+    """The learned Schur stencil shipped with the package (or None if none has been trained)."""
+    if not os.path.exists(SCHUR_PATH):
+        return None
+    return LearnedSchurStencil.load(SCHUR_PATH)
+
+
+def inclusion_problem(res, seed):
+    """A free-slip unit-cube problem with 1-5 spherical inclusions of viscosity 10^c
+    (c ~ U(-4, 4) per inclusion) in a background of viscosity 1, density 1 in the inclusions,
+    assembled by Gauss quadrature: sharp viscosity interfaces, as in sinker models."""
+    from . import mesh as _mesh, conditions as _cond, systems as _sys
+
+    rng = np.random.default_rng(seed)
+    ns = int(rng.integers(1, 6))
+    centres = rng.uniform(0.2, 0.8, size=(ns, 3))
+    radii = rng.uniform(0.06, 0.2, size=ns)
+    logeta = rng.uniform(-4.0, 4.0, size=ns)
+    m = _mesh.FeMesh_Cartesian("Q1/dQ0", (res,) * 3, (0.0,) * 3, (1.0,) * 3)
+    v = _mesh.MeshVariable(m, 3)
+    p = _mesh.MeshVariable(m.subMesh, 1)
+    walls = tuple(m.specialSets[f"Min{a}_VertexSet"] + m.specialSets[f"Max{a}_VertexSet"] for a in "IJK")
+    bc = _cond.DirichletCondition(v, walls)
+    st = _sys.Stokes(v, p, fn_viscosity=_SpheresViscosity(centres, radii, logeta),
+                     fn_bodyforce=_SpheresForce(centres, radii), conditions=[bc])
+    return st, v, p, {"seed": seed, "inclusions": ns}
+
+
+class _SpheresViscosity(Function):
+    def __init__(self, centres, radii, logeta):
+        self.c, self.r, self.l = np.asarray(centres, float), np.asarray(radii, float), np.asarray(logeta, float)
+
+    def _gauss_values(self, mesh):
+        off, cen = _gauss_offsets(mesh)
+        out = np.zeros((cen.shape[0], 27))
+        for q in range(27):
+            x = cen + off[q]
+            le = np.zeros(x.shape[0])
+            for c, r, lv in zip(self.c, self.r, self.l):
+                le = np.where(np.sum((x - c) ** 2, axis=1) < r * r, lv, le)
+            out[:, q] = 10.0 ** le
+        return out
+
+    def _fn_eval(self, ctx):
+        x = ctx.coords
+        le = np.zeros(x.shape[0])
+        for c, r, lv in zip(self.c, self.r, self.l):
+            le = np.where(np.sum((x - c) ** 2, axis=1) < r * r, lv, le)
+        return (10.0 ** le)[:, None]
+

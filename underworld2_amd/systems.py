@@ -145,11 +145,21 @@ class Stokes:
         lib = L.lib()
         d = self._mesh.dim
         if self._swarm is None:
-            pts = _gauss_points(self._mesh)
-            E, Q = pts.shape[0], pts.shape[1]
-            flat = pts.reshape(-1, d)
-            eta = np.asarray(self._fn_viscosity.evaluate(flat), dtype=np.float64).reshape(E, Q)
-            force = np.asarray(self._fn_bodyforce.evaluate(flat), dtype=np.float64).reshape(E, Q, -1)
+            # functions may provide their values at the Gauss points directly (_gauss_values)
+            flat = None
+            E = self._mesh.elementsGlobal
+            Q = (5 if self._mesh._order == 2 else 3) ** d
+
+            def at_gauss(f):
+                nonlocal flat
+                if hasattr(f, "_gauss_values"):
+                    return np.asarray(f._gauss_values(self._mesh), dtype=np.float64)
+                if flat is None:
+                    flat = _gauss_points(self._mesh).reshape(-1, d)
+                return np.asarray(f.evaluate(flat), dtype=np.float64)
+
+            eta = at_gauss(self._fn_viscosity).reshape(E, Q)
+            force = at_gauss(self._fn_bodyforce).reshape(E, Q, -1)
             if force.shape[2] != d:
                 raise ValueError("fn_bodyforce must be a vector of the mesh dimension")
             if not np.all(eta > 0):
