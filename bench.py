@@ -6,13 +6,16 @@ integration (A6-A9: moments, approxS, right-hand side), multigrid hierarchy setu
 Krylov solve to outer rtol 1e-6 -- the same work the reference's ``StokesSolver.solve`` does
 with ``reinitialise=True`` (the voronoi swarm is unchanged, so DVC is not re-run).
 
-The headline ``value`` is the learned+mixed configuration when a trained model is present,
-otherwise the mixed-precision coupled solver.  Every mode's own timing, iteration counts and
-FP64 residuals are reported in ``modes``; ``advection`` holds the RK2 step and DVC pass times.
+The headline ``value`` is the learned+mixed configuration (the learned Schur stencil in the
+coupled solver's Schur block, FP32 multigrid cycle, FP64 Krylov) when a trained model is
+present, otherwise the mixed-precision coupled solver.  Every mode's own timing, iteration
+counts, FP64 residuals and error against an FP64 reference solution (coupled FP64 solve to
+rtol 1e-10) are reported in ``modes``, with a per-solve roofline from the logged counts;
+``advection`` holds the RK2 step and DVC pass times.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 128] [--ppc 30]
-                       [--mode M] [--modes fp64,mixed,learned[,schur]] [--no-cpu-baseline]
-                       [--no-advection] [--apply-only]
+                       [--mode M] [--modes fp64,mixed,learned[,learned-coarse,schur]]
+                       [--no-cpu-baseline] [--no-advection] [--no-reference] [--apply-only]
        python bench.py --q2 --res 64 --ppc 20 [--modes fp64,mixed]
        python bench.py --convection --res 96 --ppc 25 --steps 40 [--modes fp64,mixed,learned]
            [--series-out F.json] [--compare-series F.json]
@@ -76,9 +79,9 @@ def make_solver(P, mode, rtol):
         s.set_method("block")
         s.set_precision({"fp64": "fp64", "mixed-cycle": "mixed-cycle"}.get(mode, "mixed"))
         s.set_outer_rtol(rtol)
-        if mode == "learned":
+        if mode in ("learned", "learned-coarse"):
             from underworld2_amd import learned
-            model = learned.load_default(P["stokes"])
+            model = learned.load_default_schur() if mode == "learned" else learned.load_default(P["stokes"])
             if model is None:
                 return None
             s.set_learned(model)
@@ -101,10 +104,71 @@ def time_mode(P, mode, steps, warmup, rtol):
     t = time.perf_counter() - t0
     st = stats[-1]
     rel = float(np.hypot(st["res_momentum"], st["res_continuity"]) / st["norm_f"])
-    return dict(mode=mode, seconds_per_solve=t / steps, solves_per_sec=steps / t, outer_its=st["outer_its"],
-                pressure_its=st["pressure_its"], velocity_its=st["velocity_total_its"], vcycles=st["vcycles"],
-                levels=st["levels"], setup_s=st["setup_time"], solve_s=st["solve_time"], rel_residual_fp64=rel,
-                converged=bool(st["converged"]))
+    out = dict(mode=mode, seconds_per_solve=t / steps, solves_per_sec=steps / t, outer_its=st["outer_its"],
+               pressure_its=st["pressure_its"], velocity_its=st["velocity_total_its"], vcycles=st["vcycles"],
+               levels=st["levels"], setup_s=st["setup_time"], solve_s=st["solve_time"], rel_residual_fp64=rel,
+               converged=bool(st["converged"]), fallback_used=int(st.get("fallback_used", 0)))
+    sol = (np.array(P["vel"].data), np.array(P["p"].data[:, 0]))
+    return out, sol
+
+
+def reference_solution(P, rtol=1e-10):
+    """FP64 reference: the coupled FP64 solve to rtol 1e-10 (the FP64 path is tied to the
+    oracle's solution to 1e-8 by tests/test_gpu_sinker32.py at 32^3)."""
+    s = make_solver(P, "fp64", rtol)
+    s.solve()
+    st = s.get_stats()
+    return (np.array(P["vel"].data), np.array(P["p"].data[:, 0])), {
+        "rtol": rtol, "outer_its": int(st.outer_its),
+        "rel_residual_fp64": float(np.hypot(st.res_momentum, st.res_continuity) / st.norm_f)}
+
+
+def solution_error(sol, ref):
+    (u, p), (ur, pr) = sol, ref
+    pm, prm = p - p.mean(), pr - pr.mean()
+    return {"velocity_rel_l2": float(np.linalg.norm(u - ur) / np.linalg.norm(ur)),
+            "pressure_rel_l2_mean_free": float(np.linalg.norm(pm - prm) / np.linalg.norm(prm))}
+
+
+def per_solve_roofline(P, mode, r, restart=60):
+    """Algorithmic HBM bytes of one solve from its logged counts, divided by the solve time and
+    the 8 TB/s peak (the per-solve roofline of the path).  Per FGMRES iteration: the outer FP64
+    operator (K: E 27 8 + 2 3 N 8 bytes; G and G^T: E 8 + 3 N 8 each), one V-cycle (every
+    level: 5 K applies -- Chebyshev(2) before and after the coarse correction and one residual
+    -- with bf16 weights and FP32 vectors in the mixed modes, FP64 otherwise, plus restriction
+    and prolongation of FP32/FP64 vectors), the Schur block (diagonal: 3 E s; learned stencil:
+    27 E 4 + 2 E 8), and classical Gram-Schmidt against the j + 1 basis vectors of the cycle's
+    step j (read twice: dots and update; FP32 basis in the mixed modes) plus w and the new basis
+    vector.  Setup: the particle-to-weights assembly (P (3 8 + 8 + 4) + E (27 + 8 + 1) 8)."""
+    mesh = P["mesh"]
+    res = np.array(mesh.elementRes)
+    mixed = mode not in ("fp64", "schur")
+    s_v, s_w = (4, 2) if mixed else (8, 8)
+    E, N = int(np.prod(res)), int(np.prod(res + 1))
+    n = 3 * N + E
+    kb = lambda e, nn, sw, sv: e * 27 * sw + 2 * 3 * nn * sv
+    outer = kb(E, N, 8, 8) + 2 * (E * 8 + 3 * N * 8)
+    vc = 0
+    lev = res.copy()
+    for l in range(int(r["levels"])):
+        e, nn = int(np.prod(lev)), int(np.prod(lev + 1))
+        if l < int(r["levels"]) - 1:
+            vc += 5 * kb(e, nn, s_w, s_v)
+            nc = int(np.prod(lev // 2 + 1))
+            vc += 2 * (3 * nn * s_v + 3 * nc * s_v)
+        lev = lev // 2
+    schur = 27 * E * 4 + 2 * E * 8 if mode == "learned" else 3 * E * 8
+    its = int(r["outer_its"])
+    s_b = 4 if mixed else 8
+    gs = 0
+    for k in range(its):
+        j = k % restart
+        gs += 2 * (j + 1) * n * s_b + 3 * n * 8
+    setup = int(P["swarm"].particleLocalCount) * (3 * 8 + 8 + 4) + E * (27 + 8 + 1) * 8
+    total = its * (outer + vc + schur) + gs + setup
+    t = r["seconds_per_solve"]
+    return {"bytes": total, "bytes_per_iteration": (total - setup) / max(1, its), "achieved_GB_per_s": total / t / 1e9,
+            "frac": total / t / HBM_PEAK, "gram_schmidt_share_of_bytes": gs / total}
 
 
 def roofline(P, precision, reps=50):
@@ -140,13 +204,16 @@ def roofline(P, precision, reps=50):
             "traffic": traffic, "kernel": kernel, "kernel_ms": ms.value, "algorithmic_bytes": bytes_}
 
 
-def cpu_baseline(res_sample=24, ppc=30, rtol=1e-6):
+def cpu_baseline(res_sample=48, ppc=30, rtol=1e-6, max_outer=40, schur_its_128=4635):
     """The CPU restatement of the reference's solve (oracle/cpu_baseline.c: assembled CSR K, G,
     approxS from the particle quadrature; Schur FGMRES with the approxS preconditioner; inner
     FGMRES + Galerkin multigrid with the SROpGenerator prolongation; the same rtols), OpenMP on
-    every core this process may use, on a bounded sample of the same sinker.  Timed: element
-    integration + CSR assembly + solve (the per-solve work; layout and DVC excluded).  Scaled to
-    128^3 by the DOF ratio only (iteration growth ignored: an upper bound for the CPU)."""
+    every core this process may use, on a bounded sample of the same sinker: the 48^3 sinker
+    (30 ppc), element integration + CSR assembly timed in full, the Schur solve for its first
+    ``max_outer`` outer iterations.  Scaled to 128^3 by the element ratio (assembly) and by the
+    DOF ratio times the outer iteration count of the same algorithm at 128^3 (``schur_its_128``:
+    4,635 pressure iterations, measured with the GPU Schur path, which runs the reference's
+    algorithm) -- the per-iteration cost of the inner multigrid solves grows with the DOFs."""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from sinker import sphere_material
@@ -163,22 +230,59 @@ def cpu_baseline(res_sample=24, ppc=30, rtol=1e-6):
     Ke, fe, Ge, S = O.assemble_elements(mesh, ptr, xw, w, eta, force)
     sysm = O.System(mesh, mesh.freeslip_mask(), Ke, fe, Ge, S)
     t1 = time.perf_counter()
-    u, p, st = sysm.bsscr_solve(rtol, rtol)
+    u, p, st = sysm.bsscr_solve(rtol, rtol, max_outer=max_outer)
     t2 = time.perf_counter()
     dof = sysm.nfree + mesh.E
     dof128 = 6340221 + 128 ** 3
-    scale = dof128 / dof
-    t = t2 - t0
+    per_outer = (t2 - t1) / max(1, st["outer_its"])
+    t128 = (t1 - t0) * (128 ** 3 / mesh.E) + per_outer * (dof128 / dof) * schur_its_128
     try:
         cpu = [l for l in open("/proc/cpuinfo") if l.startswith("model name")][0].split(":", 1)[1].strip()
     except Exception:
         cpu = "unknown"
-    return {"value": 1.0 / (t * scale), "unit": "solves/s", "cores": st["threads"], "kind": "port",
+    return {"value": 1.0 / t128, "unit": "solves/s", "cores": st["threads"], "kind": "port",
             "sample": f"CPU restatement (not PETSc) of the reference solve on the {res_sample}^3 sinker, {ppc} ppc: "
-                      f"assembly {t1 - t0:.2f}s + Schur/multigrid solve {t2 - t1:.2f}s ({st['outer_its']} outer, "
-                      f"{st['inner_its']} inner its, {st['levels']} levels) on {st['threads']} OpenMP threads "
-                      f"({cpu}); scaled x{scale:.0f} by DOF count to 128^3, iteration growth ignored",
-            "seconds_per_solve_sample": t, "outer_its_sample": st["outer_its"]}
+                      f"assembly {t1 - t0:.2f}s + the first {st['outer_its']} Schur iterations {t2 - t1:.2f}s "
+                      f"({st['inner_its']} inner its, {st['levels']} levels) on {st['threads']} OpenMP threads "
+                      f"({cpu}); scaled to 128^3 by the element ratio (assembly) and the DOF ratio x "
+                      f"{schur_its_128} Schur iterations (the same algorithm's count at 128^3)",
+            "seconds_per_solve_128_estimate": t128, "seconds_per_outer_iteration_sample": per_outer}
+
+
+def config1(ppc=20, rtol=1e-5):
+    """The 2D 64^2 sinker of the plain-method configuration (20 ppc, seed 0, inclusion r 0.1 at
+    (0.5, 0.6): viscosity 100, density 1; background 1 / 0; gravity (0, -1)): the Schur solve to
+    rtol 1e-5 and one RK2 step at dt = 0.5 get_max_dt(), timed on the GPU (FP64, the reference's
+    algorithm) and in the CPU restatement (element integration + assembly + Schur solve + RK2;
+    layout and DVC excluded on both)."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from sinker import build_oracle, build_uw
+    g = build_uw((64, 64), ppc, 0, (0.5, 0.6), 0.1)
+    solver = g["uw"].systems.Solver(g["stokes"])
+    solver.set_outer_rtol(rtol)
+    g["swarm"].integration_points(dvc=True)
+    solver.solve()  # warm-up (first-call set-up of the handle)
+    adv = g["uw"].systems.SwarmAdvector(g["vel"], g["swarm"], order=2)
+    t0 = time.perf_counter()
+    solver.solve()
+    dt = 0.5 * adv.get_max_dt()
+    adv.integrate(dt)
+    g["swarm"].particleCoordinates.data  # noqa: B018 (sync)
+    tg = time.perf_counter() - t0
+    gst = solver.get_stats()
+    o = build_oracle((64, 64), ppc, 0, (0.5, 0.6), 0.1, solve=False)
+    m = o["mesh"]
+    t0 = time.perf_counter()
+    Ke, fe, Ge, S = O.assemble_elements(m, o["ptr"], o["xi"], o["w"], o["eta"], o["force"])
+    sysm = O.System(m, m.freeslip_mask(), Ke, fe, Ge, S)
+    u, p, st = sysm.bsscr_solve(rtol, 1e-6)
+    dtc = 0.5 * O.max_dt(m, u)
+    O.advect_rk2(m, u, o["coords"], dtc)
+    tc = time.perf_counter() - t0
+    return {"workload": "2D 64^2 sinker, 20 ppc, Schur solve rtol 1e-5 + one RK2 step",
+            "gpu_seconds": tg, "gpu_pressure_its": int(gst.pressure_its), "cpu_seconds": tc,
+            "cpu_outer_its": st["outer_its"], "cpu_threads": st["threads"], "cpu_kind": "port"}
 
 
 def time_advection(P, steps=10):
@@ -319,10 +423,12 @@ def main():
     ap.add_argument("--mode", default="auto", help="mode whose throughput is the headline value")
     ap.add_argument("--modes", default="fp64,mixed,learned",
                     help="comma list of fp64 (coupled FGMRES, FP64), mixed (FP32 cycle in FP64 Krylov), "
-                         "learned (mixed + learned coarse correction), schur (the reference's Schur "
-                         "complement algorithm in FP64; minutes at 128^3)")
+                         "learned (mixed + learned Schur stencil), learned-coarse (mixed + the experimental "
+                         "learned coarse correction), schur (the reference's Schur complement algorithm in "
+                         "FP64; minutes at 128^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-advection", action="store_true")
+    ap.add_argument("--no-reference", action="store_true", help="skip the FP64 reference solve and error report")
     ap.add_argument("--apply-only", action="store_true")
     ap.add_argument("--q2", action="store_true",
                     help="Q2/dPc1 3D sinker: K apply (matrix cores / vector) and solves at --res, --ppc")
@@ -353,6 +459,7 @@ def main():
         apply_only(P)
         return
     modes = {}
+    sols = {}
     for m in args.modes.split(","):
         m = m.strip()
         if not m:
@@ -360,8 +467,16 @@ def main():
         log(f"timing mode {m} ...")
         r = time_mode(P, m, args.steps, args.warmup, args.rtol)
         if r is not None:
-            modes[m] = r
-            log(json.dumps(r))
+            modes[m], sols[m] = r
+            modes[m]["per_solve_roofline"] = per_solve_roofline(P, m, modes[m])
+            log(json.dumps(modes[m]))
+    reference = None
+    if not args.no_reference:
+        log("FP64 reference solution (coupled FP64, rtol 1e-10) ...")
+        ref, reference = reference_solution(P)
+        for m in modes:
+            modes[m]["error_vs_fp64_reference"] = solution_error(sols[m], ref)
+        log(json.dumps(reference))
     head = args.mode
     if head == "auto":
         head = "learned" if "learned" in modes else ("mixed" if "mixed" in modes else next(iter(modes)))
@@ -372,11 +487,16 @@ def main():
     others = {"fp32_weights": roofline(P, 2), "fp64": roofline(P, 0)} if head not in ("fp64", "schur") else {}
     adv = None if args.no_advection else time_advection(P)
     cpu = None
+    cfg1 = None
     if not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline()
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": str(e)}
+        try:
+            cfg1 = config1()
+        except Exception as e:
+            cfg1 = {"error": str(e)}
     dtype = "f64" if head in ("fp64", "schur") else "f32+f64"
     line = {
         "metric": METRIC, "value": H["solves_per_sec"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -387,7 +507,8 @@ def main():
                                f"end-to-end solve() on a fixed swarm (particle-quadrature assembly + solve), "
                                f"mode={head}" + ("" if head in ("fp64", "schur") else
                                                  " (FP32 cycle with bf16 element weights, FP64 Krylov)")},
-        "roofline": roof, "cpu_baseline": cpu, "modes": modes, "advection": adv,
+        "roofline": roof, "cpu_baseline": cpu, "modes": modes, "advection": adv, "fp64_reference": reference,
+        "config1_plain_method": cfg1,
     }
     if others:
         line["roofline_other_applies"] = others

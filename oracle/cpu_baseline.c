@@ -455,8 +455,8 @@ static void Sp_op(void *c, const double *r, double *z)
    stats[0..3] = outer its, total inner its, V-cycles, levels; times[0..1] = setup, solve (s). */
 int orc_bsscr_solve(int dim, const int *res, long nu, const long *Kptr, const int *Kcol, const double *Kval, long np,
                     const long *Gptr, const int *Gcol, const double *Gval, const double *approxS, const double *f,
-                    double outer_rtol, double inner_rtol, int coarse_max, double *u, double *p, long *stats,
-                    double *times)
+                    double outer_rtol, double inner_rtol, int coarse_max, int max_outer, double *u, double *p,
+                    long *stats, double *times)
 {
     double t0 = omp_get_wtime();
     mg_t M;
@@ -528,7 +528,7 @@ int orc_bsscr_solve(int dim, const int *res, long nu, const long *Kptr, const in
     spmv(&Gt, ustar, hh);
     memset(p, 0, sizeof(double) * np);
     int outer = 0;
-    fgmres(np, S_op, &s, Sp_op, &s, hh, p, outer_rtol, 5000, 30, &outer);
+    fgmres(np, S_op, &s, Sp_op, &s, hh, p, outer_rtol, max_outer, 30, &outer);
     /* back-solve u = K^{-1} (f - G p) */
     spmv(&G, p, s.t3);
     for (long i = 0; i < nu; i++) s.t3[i] = f[i] - s.t3[i];

@@ -403,10 +403,10 @@ class System:
         p = x[self.nfree:self.nfree + E]
         return u.reshape(self.mesh.N, self.mesh.dim), p
 
-    def bsscr_solve(self, outer_rtol=1e-6, inner_rtol=1e-6, coarse_max=800):
+    def bsscr_solve(self, outer_rtol=1e-6, inner_rtol=1e-6, coarse_max=800, max_outer=5000):
         """The reference's iterative solve restated on the CPU (cpu_baseline.c): Schur FGMRES
-        with the approxS preconditioner around FGMRES + Galerkin multigrid inner solves.
-        Free-slip walls on every face.  Returns (u, p, stats dict)."""
+        with the approxS preconditioner around FGMRES + Galerkin multigrid inner solves, at most
+        max_outer outer iterations.  Free-slip walls on every face.  Returns (u, p, stats dict)."""
         K = self.K.tocsr()
         G = self.G.tocsr()
         kp, kc, kv = (np.ascontiguousarray(K.indptr, np.int64), np.ascontiguousarray(K.indices, np.int32),
@@ -424,7 +424,7 @@ class System:
         rc = lib().orc_bsscr_solve(self.mesh.dim, _i(res), ctypes.c_long(self.nfree), _l(kp), _i(kc), _d(kv),
                                    ctypes.c_long(E), _l(gp), _i(gc), _d(gv), _d(S), _d(f),
                                    ctypes.c_double(outer_rtol), ctypes.c_double(inner_rtol), ctypes.c_int(coarse_max),
-                                   _d(uf), _d(p), _l(st), _d(tm))
+                                   ctypes.c_int(max_outer), _d(uf), _d(p), _l(st), _d(tm))
         if rc < 0:
             raise RuntimeError("bsscr_solve: free-DOF numbering does not match free-slip walls")
         u = np.zeros(self.ndof)

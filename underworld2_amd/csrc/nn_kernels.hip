@@ -942,9 +942,18 @@ __global__ void k_snn_stencil_grad(int B, Grid3 g, const float* __restrict__ dif
   }
 }
 
-// preconditioner coefficients a[s][e] = -(delta_s,13 + y[s][e]) / sqrt(S_e S_e+off(s)) (0 outside):
-// z = sum_s a_s q_e+s is -M q (the Schur block's sign convention, S approximated by -approxS)
-__global__ void k_snn_coef(Grid3 g, const float* __restrict__ y, const double* __restrict__ S, float* __restrict__ a) {
+// bfloat16 (round to nearest, ties to even) of a finite float, and back
+__device__ __forceinline__ uint16_t snn_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float snn_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// preconditioner coefficients a[s][e] = -(delta_s,13 + y[s][e]) / sqrt(S_e S_e+off(s)) (0 outside),
+// stored as bfloat16: z = sum_s a_s q_e+s is -M q (the Schur block's sign convention, S
+// approximated by -approxS); a preconditioner's coefficients need no more than bf16's 8 bits
+__global__ void k_snn_coef(Grid3 g, const float* __restrict__ y, const double* __restrict__ S,
+                           uint16_t* __restrict__ a) {
   const long N = g.N();
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N) return;
@@ -959,28 +968,33 @@ __global__ void k_snn_coef(Grid3 g, const float* __restrict__ y, const double* _
       const double dn = 1.0 / sqrt(S[ii + (long)g.nx * (jj + (long)g.ny * kk)]);
       v = (float)(-(double)((s == 13 ? 1.0f : 0.0f) + y[s * N + e]) * de * dn);
     }
-    a[s * N + e] = v;
+    a[s * N + e] = snn_bf16(v);
   }
 }
 
-// z_e = sum_s a[s][e] q_e+off(s)  (T = double for the FP64 Krylov iteration, float for FP32)
+// z_e = sum_s a[s][e] q_e+off(s)  (T = double for the FP64 Krylov iteration, float for FP32).
+// One thread per element, x fastest: the 27 coefficient planes stream coalesced; the
+// neighbour values come through the caches.  Out-of-grid taps have zero coefficients and read
+// the element itself (a valid address), so no tap is branched on.  32-bit indices (the element
+// count is below 2^31 on every grid this is used on).
 template <typename T>
-__global__ __launch_bounds__(256) void k_snn_apply(Grid3 g, const float* __restrict__ a, const T* __restrict__ q,
+__global__ __launch_bounds__(256) void k_snn_apply(Grid3 g, const uint16_t* __restrict__ a, const T* __restrict__ q,
                                                    T* __restrict__ z) {
-  const long N = g.N();
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = (int)g.N();
+  const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (e >= N) return;
-  int i, j, k;
-  grid_ijk(g, e, i, j, k);
+  const int nx = g.nx, nxy = g.nx * g.ny;
+  const int i = e % nx, j = (e / nx) % g.ny, k = e / nxy;
+  const int dxm = i > 0 ? -1 : 0, dxp = i < nx - 1 ? 1 : 0;
+  const int dym = j > 0 ? -nx : 0, dyp = j < g.ny - 1 ? nx : 0;
+  const int dzm = k > 0 ? -nxy : 0, dzp = k < g.nz - 1 ? nxy : 0;
+  const int ox[3] = {dxm, 0, dxp}, oy[3] = {dym, 0, dyp}, oz[3] = {dzm, 0, dzp};
+  uint16_t c[27];
+#pragma unroll
+  for (int s = 0; s < 27; s++) c[s] = a[s * N + e];
   T acc = (T)0;
 #pragma unroll
-  for (int s = 0; s < 27; s++) {
-    const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
-    const bool in = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
-    const long n = in ? ii + (long)g.nx * (jj + (long)g.ny * kk) : e;
-    const T qv = q[n];
-    acc += in ? (T)a[s * N + e] * qv : (T)0;
-  }
+  for (int s = 0; s < 27; s++) acc += (T)snn_f32(c[s]) * q[e + ox[s % 3] + oy[(s / 3) % 3] + oz[s / 9]];
   z[e] = acc;
 }
 
@@ -1058,7 +1072,10 @@ static void snn_ensure(SchurNet* m, int B, long N, bool train) {
 static void conv3_fwd(int B, int ci, int co, Grid3 g, const float* in, const float* w, const float* bias, float* out,
                       bool relu, hipStream_t s) {
   const long nb = (long)B * g.N();
-  if (co % 8 == 0)
+  if (co % 16 == 0)
+    hipLaunchKernelGGL(k_conv_fwd_blk<16>, dim3(nbk(nb, 256), co / 16), 256, 0, s, B, ci, co, g, in, w, bias, out,
+                       relu ? 1 : 0);
+  else if (co % 8 == 0)
     hipLaunchKernelGGL(k_conv_fwd_blk<8>, dim3(nbk(nb, 256), co / 8), 256, 0, s, B, ci, co, g, in, w, bias, out,
                        relu ? 1 : 0);
   else if (co % 4 == 0)
@@ -1161,7 +1178,7 @@ void schur_net_features(const Geom& g, const double* approxS, float* feat, hipSt
   UWG_HIP(hipGetLastError());
 }
 
-void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, float* a, hipStream_t s) {
+void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, uint16_t* a, hipStream_t s) {
   if (g.dim != 3) throw std::runtime_error("the learned Schur stencil is provided for 3D meshes");
   const Grid3 eg{g.res[0], g.res[1], g.res[2]};
   const long N = eg.N();
@@ -1173,13 +1190,14 @@ void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, f
 }
 
 template <typename T>
-void schur_stencil_apply(const Geom& g, const float* a, const T* q, T* z, hipStream_t s) {
+void schur_stencil_apply(const Geom& g, const uint16_t* a, const T* q, T* z, hipStream_t s) {
   const Grid3 eg{g.res[0], g.res[1], g.res[2]};
+  if (eg.N() >= (1L << 31) / 27) throw std::runtime_error("element grid too large for the stencil apply");
   hipLaunchKernelGGL(k_snn_apply<T>, nbk(eg.N(), 256), 256, 0, s, eg, a, q, z);
   UWG_HIP(hipGetLastError());
 }
-template void schur_stencil_apply<double>(const Geom&, const float*, const double*, double*, hipStream_t);
-template void schur_stencil_apply<float>(const Geom&, const float*, const float*, float*, hipStream_t);
+template void schur_stencil_apply<double>(const Geom&, const uint16_t*, const double*, double*, hipStream_t);
+template void schur_stencil_apply<float>(const Geom&, const uint16_t*, const float*, float*, hipStream_t);
 
 // training set: append one sample (features from approxS, r^ and e^ given on the device, FP64)
 __global__ void k_d2f_copy(long n, const double* __restrict__ a, float* __restrict__ b) {

@@ -185,7 +185,7 @@ struct uwg_stokes {
   int nn_level = -1;
   // learned Schur stencil: network, its coefficients a[27][E] for the current assembly
   std::unique_ptr<SchurNet, SchurNetDeleter> snn;
-  float* snn_a = nullptr;
+  uint16_t* snn_a = nullptr;  // bfloat16
   long snn_stamp = -1;        // assembly_stamp the coefficients were computed for
   long assembly_stamp = 0;    // bumped by every assembly (approxS changes)
   int vcycles = 0;
@@ -198,7 +198,7 @@ struct uwg_stokes {
   KSpaceT<double, float>* kb32 = nullptr;  // FP64 Krylov space with an FP32 Arnoldi basis
   // learned-correction support
   bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
-  int nn_window = 10;            // fallback monitor: over nn_window iterations the residual must
+  int nn_window = 20;            // fallback monitor: over nn_window iterations the residual must
   double nn_min_drop = 0.1;      // fall by at least this fraction while the network is in use
   int nn_mode = 0;               // 0: replace the coarse cycle, 1: add to it
   int capture_level = -1;        // record coarse vectors seen by the cycle
@@ -1351,7 +1351,7 @@ static void top_schur(uwg_stokes* st, const T* diagS, const T* r, T* z) {
 static bool g_snn_active = false;
 static void ensure_snn_coefficients(uwg_stokes* st) {
   if (st->snn_stamp == st->assembly_stamp && st->snn_a) return;
-  if (!st->snn_a) st->snn_a = dalloc<float>(27 * st->g.E);
+  if (!st->snn_a) st->snn_a = dalloc<uint16_t>(27 * st->g.E);
   schur_net_coefficients(st->snn.get(), st->g, st->approxS, st->snn_a, 0);
   st->snn_stamp = st->assembly_stamp;
 }
@@ -2151,8 +2151,9 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   };
   int its = 0;
   double rn;
-  // learned-correction safeguard: the FP64 residual must keep falling; if 10 iterations
-  // reduce it by less than 10 %, the network is switched off and multigrid takes over
+  // learned-correction safeguard: the FP64 residual must keep falling; if 20 iterations
+  // reduce it by less than 10 %, the learned component is switched off and the plain
+  // preconditioner takes over (a shorter window trips on the plateaus after restarts)
   // (FGMRES admits a preconditioner that changes between iterations)
   // hist[k] is the relative residual after iteration k (hist[0] = 1, the start), so the test
   // compares the residual with the one nn_window iterations earlier

@@ -28,10 +28,10 @@ SchurNet* schur_net_create(int channels, int layers, const float* params, long n
 void schur_net_destroy(SchurNet* m);
 // feat[9][E] from approxS[E] (device arrays)
 void schur_net_features(const Geom& g, const double* approxS, float* feat, hipStream_t s);
-// preconditioner coefficients a[27][E] (FP32): z = sum_s a_s q_e+s approximates -S^-1 q
-void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, float* a, hipStream_t s);
+// preconditioner coefficients a[27][E] (bfloat16 bits): z = sum_s a_s q_e+s approximates -S^-1 q
+void schur_net_coefficients(SchurNet* m, const Geom& g, const double* approxS, uint16_t* a, hipStream_t s);
 template <typename T>
-void schur_stencil_apply(const Geom& g, const float* a, const T* q, T* z, hipStream_t s);
+void schur_stencil_apply(const Geom& g, const uint16_t* a, const T* q, T* z, hipStream_t s);
 // append one training sample: features of approxS, r^ = D^-1/2 S e, e^ = D^1/2 e (device, FP64)
 void schur_net_append(SchurNet* m, const Geom& g, const double* approxS, const double* rhat, const double* ehat);
 

@@ -348,7 +348,7 @@ class StokesSolver:
         self._method = 0
         self._precision = 0
         self._learned = False
-        self._restart = 30
+        self._restart = None  # Krylov restart length: 30 (Schur path) / 60 (coupled) unless set
         self._smooth_degree = 2
         self._stats = L.SolveStats()
 
@@ -406,6 +406,13 @@ class StokesSolver:
         the stopping test stay FP64) or "fp64".  Process-wide; takes effect at the next solve."""
         L.check(L.lib().uwg_set_krylov_basis_fp32({"fp32": 1, "fp64": 0}[fmt]), "set_krylov_basis")
 
+    def set_restart(self, restart):
+        """FGMRES restart length (engine-specific): the outer Schur iteration's (default 30, the
+        reference's PETSc default) or the coupled iteration's (default 60)."""
+        if not isinstance(restart, int) or not 2 <= restart <= 200:
+            raise ValueError("restart must be an int in [2, 200]")
+        self._restart = restart
+
     def set_learned(self, model):
         """Attach a trained learned coarse correction (``underworld2_amd.learned``)."""
         if model is None:
@@ -423,7 +430,7 @@ class StokesSolver:
         o.outer_rtol = float(self.options.scr.ksp_rtol)
         o.inner_rtol = float(self.options.A11.ksp_rtol)
         o.mg_levels = int(self.options.mg.levels)
-        o.restart = self._restart if self._method == 0 else max(self._restart, 60)
+        o.restart = self._restart if self._restart is not None else (30 if self._method == 0 else 60)
         o.smooth_degree = self._smooth_degree
         return o
 
