@@ -1606,6 +1606,16 @@ INST(double)
 INST(float)
 #undef INST
 
+// y = x / sqrt(ss[0]) with the squared norm ss[0] read on the device (no host round trip)
+__global__ void k_scale_inv_sqrt(long n, const double* __restrict__ ss, const double* __restrict__ x,
+                                 double* __restrict__ y) {
+  const double f = 1.0 / sqrt(ss[0]);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = f * x[i];
+}
+void vk_scale_inv_sqrt(long n, const double* ss, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_inv_sqrt, vgrid(n), 256, 0, s, n, ss, x, y);
+}
+
 void vk_d2f(long n, double scale, const double* x, float* y, hipStream_t s) {
   hipLaunchKernelGGL(k_d2f, vgrid(n), 256, 0, s, n, scale, x, y);
 }
@@ -1849,6 +1859,99 @@ __global__ __launch_bounds__(256) void k_dense_mv(int n, const double* A, const 
   }
   if (threadIdx.x == 0) y[r] = sm[0];
 }
+// Dense operator of the coarsest multigrid level, one thread per entry (r, c), r = node_r d + i,
+// c = node_c d + j: K[r][c] = sum over the elements holding both nodes (fixed order) and their
+// 3^d Gauss points of W_q (delta_ij dN_a . dN_b + dN_a/dx_j dN_b/dx_i), physical derivatives;
+// identity rows and columns on Dirichlet dofs.  A gather, so the sums are deterministic.
+__global__ void k_coarse_dense(Geom g, const double* __restrict__ W, const uint8_t* __restrict__ bc,
+                               double* __restrict__ K) {
+  const int dim = g.dim, n = (int)(g.N * dim);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n * n) return;
+  const int r = (int)(t / n), c = (int)(t % n);
+  const int nr = r / dim, i = r % dim, nc = c / dim, j = c % dim;
+  if (((bc[nr] >> i) & 1) || ((bc[nc] >> j) & 1)) {
+    K[t] = r == c ? 1.0 : 0.0;
+    return;
+  }
+  const int nx = g.nn[0], ny = g.nn[1];
+  const int rx = nr % nx, ry = (nr / nx) % ny, rz = dim == 3 ? nr / (nx * ny) : 0;
+  const int cx = nc % nx, cy = (nc / nx) % ny, cz = dim == 3 ? nc / (nx * ny) : 0;
+  if (abs(rx - cx) > 1 || abs(ry - cy) > 1 || abs(rz - cz) > 1) {
+    K[t] = 0.0;
+    return;
+  }
+  double h[3] = {1.0, 1.0, 1.0};
+  for (int d = 0; d < dim; d++) h[d] = (g.hi[d] - g.lo[d]) / g.res[d];
+  const double ga = sqrt(0.6);
+  double acc = 0.0;
+  // elements around node_r: lower corner (rx - 1 + ex, ...) for ex, ey, ez in {0, 1}
+  for (int ez = 0; ez < (dim == 3 ? 2 : 1); ez++)
+    for (int ey = 0; ey < 2; ey++)
+      for (int ex = 0; ex < 2; ex++) {
+        const int ei = rx - 1 + ex, ej = ry - 1 + ey, ek = dim == 3 ? rz - 1 + ez : 0;
+        if (ei < 0 || ei >= g.res[0] || ej < 0 || ej >= g.res[1] || ek < 0 || ek >= (dim == 3 ? g.res[2] : 1)) continue;
+        // local corners of node_r and node_c in this element
+        const int ax = rx - ei, ay = ry - ej, az = dim == 3 ? rz - ek : 0;
+        const int bx = cx - ei, by = cy - ej, bz = dim == 3 ? cz - ek : 0;
+        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+        const long e = ei + (long)g.res[0] * (ej + (long)g.res[1] * ek);
+        const double sa[3] = {ax ? 1.0 : -1.0, ay ? 1.0 : -1.0, az ? 1.0 : -1.0};
+        const double sb[3] = {bx ? 1.0 : -1.0, by ? 1.0 : -1.0, bz ? 1.0 : -1.0};
+        const int Q = dim == 3 ? 27 : 9;
+        for (int q = 0; q < Q; q++) {
+          const double X[3] = {(q % 3 - 1) * ga, ((q / 3) % 3 - 1) * ga, (q / 9 - 1) * ga};
+          double da[3], db[3];
+          if (dim == 2) {
+            da[0] = 0.25 * sa[0] * (1 + sa[1] * X[1]) * 2.0 / h[0];
+            da[1] = 0.25 * sa[1] * (1 + sa[0] * X[0]) * 2.0 / h[1];
+            db[0] = 0.25 * sb[0] * (1 + sb[1] * X[1]) * 2.0 / h[0];
+            db[1] = 0.25 * sb[1] * (1 + sb[0] * X[0]) * 2.0 / h[1];
+          } else {
+            da[0] = 0.125 * sa[0] * (1 + sa[1] * X[1]) * (1 + sa[2] * X[2]) * 2.0 / h[0];
+            da[1] = 0.125 * sa[1] * (1 + sa[0] * X[0]) * (1 + sa[2] * X[2]) * 2.0 / h[1];
+            da[2] = 0.125 * sa[2] * (1 + sa[0] * X[0]) * (1 + sa[1] * X[1]) * 2.0 / h[2];
+            db[0] = 0.125 * sb[0] * (1 + sb[1] * X[1]) * (1 + sb[2] * X[2]) * 2.0 / h[0];
+            db[1] = 0.125 * sb[1] * (1 + sb[0] * X[0]) * (1 + sb[2] * X[2]) * 2.0 / h[1];
+            db[2] = 0.125 * sb[2] * (1 + sb[0] * X[0]) * (1 + sb[1] * X[1]) * 2.0 / h[2];
+          }
+          double dot = 0.0;
+          for (int kk = 0; kk < dim; kk++) dot += da[kk] * db[kk];
+          acc += W[(long)q * g.Es + e] * ((i == j ? dot : 0.0) + da[j] * db[i]);
+        }
+      }
+  K[t] = acc;
+}
+
+// One Gauss-Jordan step of the inverse of a symmetric positive definite n x n matrix
+// (row-major), pivot k, from A into B (two buffers, so no entry is read after another
+// workgroup overwrote it): with row = A[k][:] / p, row[k] = 1 / p, col = A[:][k],
+//   B[k][j] = row[j],   B[i][j] = (j == k ? 0 : A[i][j]) - col[i] row[j]  (i != k).
+// After steps k = 0 .. n-1 the last buffer holds the inverse (no pivoting, which SPD matrices
+// do not need).  flag[0] = 1 on a non-positive pivot.
+__global__ void k_gj_step(int n, int k, const double* __restrict__ A, double* __restrict__ B, int* __restrict__ flag) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n * n) return;
+  const double p = A[(long)k * n + k];
+  if (t == 0 && !(p > 0.0)) flag[0] = 1;
+  const int i = (int)(t / n), j = (int)(t % n);
+  const double ip = 1.0 / p;
+  const double rj = j == k ? ip : A[(long)k * n + j] * ip;
+  B[t] = i == k ? rj : (j == k ? 0.0 : A[t]) - A[(long)i * n + k] * rj;
+}
+
+// Ainv (n x n) = inverse of the coarsest level's operator; work: a second n x n buffer
+void launch_coarse_inverse(const Geom& g, const double* W, const uint8_t* bc, double* Ainv, double* work, int* flag,
+                           hipStream_t s) {
+  const long n = g.N * g.dim;
+  // n steps alternate between the two buffers: start in the one that ends in Ainv
+  double* bufs[2] = {(n % 2) ? work : Ainv, (n % 2) ? Ainv : work};
+  hipLaunchKernelGGL(k_coarse_dense, nblocks(n * n, 256), 256, 0, s, g, W, bc, bufs[0]);
+  for (int k = 0; k < (int)n; k++)
+    hipLaunchKernelGGL(k_gj_step, nblocks(n * n, 256), 256, 0, s, (int)n, k, bufs[k & 1], bufs[(k + 1) & 1], flag);
+  UWG_HIP(hipGetLastError());
+}
+
 void launch_dense_mv(int n, const double* A, const double* x, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_dense_mv, n, 256, 0, s, n, A, x, y);
   UWG_HIP(hipGetLastError());

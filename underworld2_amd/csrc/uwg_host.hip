@@ -42,7 +42,7 @@ enum WsSlot {
   WS_BLK_B, WS_BLK_X, WS_BLK_DS, WS_BLK_Q, WS_BLK_GP, WS_BLK_RP,
   WS_SCH_USTAR, WS_SCH_HHAT, WS_SCH_T, WS_SCH_Z, WS_SCH_DS,
   WS_MIX_B, WS_MIX_X, WS_MIX_R, WS_MIX_GP, WS_MIX_R32, WS_MIX_D32, WS_MIX_DS32, WS_MIX_Q32, WS_MIX_GP32,
-  WS_Q2_ETAG, WS_Q2_FORCEG, WS_Q2_BAD,
+  WS_Q2_ETAG, WS_Q2_FORCEG, WS_Q2_BAD, WS_COARSE_WORK,
   WS_COUNT
 };
 static double now() {
@@ -116,6 +116,13 @@ struct Level {
 struct Coarse {
   int n = 0;
   double* Ainv = nullptr;  // device, n x n
+  long cap = 0;            // doubles allocated at Ainv
+  void reserve(long nn) {
+    if (nn <= cap && Ainv) return;
+    dfree(Ainv);
+    Ainv = dalloc<double>(nn);
+    cap = nn;
+  }
 };
 
 // Q2/dPc1 finest level: quadratic velocity on the vertex grid gn, d+1 pressure values per
@@ -966,6 +973,19 @@ static void build_coarse(uwg_stokes* st) {
   if (n > 6000) {
     dfree(st->coarse.Ainv);
     st->coarse.Ainv = nullptr;
+    st->coarse.cap = 0;
+    return;
+  }
+  if (n <= 2048) {
+    // on the device: dense operator by a deterministic gather, Gauss-Jordan (one launch per pivot)
+    st->coarse.reserve((long)n * n);
+    int* flag = wsbuf<int>(st, WS_UBC_FLAG, 1);
+    double* work = wsbuf<double>(st, WS_COARSE_WORK, (long)n * n);
+    UWG_HIP(hipMemsetAsync(flag, 0, sizeof(int), 0));
+    launch_coarse_inverse(g, A.W, A.bc, st->coarse.Ainv, work, flag, 0);
+    int bad = 0;
+    UWG_HIP(hipMemcpy(&bad, flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) throw std::runtime_error("coarse operator is not positive definite (check viscosity/BCs)");
     return;
   }
   std::vector<double> W((size_t)Q * g.E);
@@ -1049,34 +1069,28 @@ static void build_coarse(uwg_stokes* st) {
     }
     for (int r = 0; r < n; r++) Inv[(size_t)c * n + r] = col[r];
   }
-  if (!st->coarse.Ainv) st->coarse.Ainv = dalloc<double>((long)n * n);
+  st->coarse.reserve((long)n * n);
   UWG_HIP(hipMemcpy(st->coarse.Ainv, Inv.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
 }
 
 // power iteration for lambda_max(D^-1 K) on each level
 static void estimate_lmax(uwg_stokes* st, Level& A) {
+  // 12 power iterations x <- D^-1 K x / |D^-1 K x| from a fixed pseudo-random start; the norm
+  // stays on the device (one host read per level): after the first normalisation |x| = 1, so
+  // lambda = |D^-1 K x| of the last iteration
   long n = A.n();
   launch_fill_hash(n, A.x, 0);
-  double lam = 0.0;
-  const double* ptr[1];
+  const double* ptr[1] = {A.d};
+  UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
   for (int it = 0; it < 12; it++) {
     launch_apply_K<double>(A.g, A.W, A.x, A.r, A.bc, 0);
     vk_xmy_scale<double>(n, A.dinv, A.r, A.d, 0);
-    // norms
-    ptr[0] = A.d;
-    UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
     vk_multi_dot(n, 1, A.d, st->dptrs, st->red_partial, st->red_out, 0);
-    double nd;
-    UWG_HIP(hipMemcpy(&nd, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
-    ptr[0] = A.x;
-    UWG_HIP(hipMemcpy(st->dptrs, ptr, sizeof(double*), hipMemcpyHostToDevice));
-    vk_multi_dot(n, 1, A.x, st->dptrs, st->red_partial, st->red_out, 0);
-    double nx2;
-    UWG_HIP(hipMemcpy(&nx2, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
-    lam = sqrt(nd / nx2);
-    vk_scale<double>(n, 1.0 / sqrt(nd), A.d, A.x, 0);
+    if (it < 11) vk_scale_inv_sqrt(n, st->red_out, A.d, A.x, 0);
   }
-  A.lmax = lam;
+  double nd;
+  UWG_HIP(hipMemcpy(&nd, st->red_out, sizeof(double), hipMemcpyDeviceToHost));
+  A.lmax = sqrt(nd);
 }
 
 // Log-viscosity per node of a level: element mean viscosity sum_g W_g / volume (W reproduces

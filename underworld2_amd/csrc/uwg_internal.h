@@ -195,6 +195,7 @@ template <typename T> void vk_axpby(long n, T a, const T* x, T b, T* y, hipStrea
 template <typename T> void vk_scale(long n, T a, const T* x, T* y, hipStream_t s);          // y = a x (y may alias x)
 template <typename T> void vk_xmy_scale(long n, const T* d, const T* x, T* y, hipStream_t s); // y = d .* x
 void vk_d2f(long n, double scale, const double* x, float* y, hipStream_t s);
+void vk_scale_inv_sqrt(long n, const double* ss, const double* x, double* y, hipStream_t s);  // y = x / sqrt(ss[0])
 void vk_f2d_axpy(long n, double scale, const float* x, double* y, hipStream_t s);  // y += scale*x
 void vk_f2d(long n, double scale, const float* x, double* y, hipStream_t s);        // y = scale*x
 void vk_d2f_vec(long n, const double* x, float* y, hipStream_t s);
@@ -220,6 +221,10 @@ void vk_cheb_first(long n, T c, const T* dinv, const T* b, T* d, T* x, hipStream
 template <typename T> void vk_residual(long n, const T* b, T* r, hipStream_t s);
 // dense coarse solve y = Ainv x for a small dense inverse (double)
 void launch_dense_mv(int n, const double* A, const double* x, double* y, hipStream_t s);
+// dense inverse of the coarsest level's operator (identity Dirichlet rows), n = N dim; work:
+// a second n x n buffer
+void launch_coarse_inverse(const Geom& g, const double* W, const uint8_t* bc, double* Ainv, double* work, int* flag,
+                           hipStream_t s);
 // max-reduction partials -> value
 double finish_max(const double* d_partial, int nblk);
 
