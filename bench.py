@@ -9,14 +9,17 @@ with ``reinitialise=True`` (the voronoi swarm is unchanged, so DVC is not re-run
 The headline ``value`` is the learned+mixed configuration (the learned Schur stencil in the
 coupled solver's Schur block, FP32 multigrid cycle, FP64 Krylov) when a trained model is
 present, otherwise the mixed-precision coupled solver.  Every mode's own timing, iteration
-counts, FP64 residuals and error against an FP64 reference solution (coupled FP64 solve to
-rtol 1e-10) are reported in ``modes``, with a per-solve roofline from the logged counts;
+counts, FP64 residuals and error against an FP64 reference solution (coupled solve in FP64
+throughout, learned stencil, rtol 1e-11) are reported in ``modes``, with a per-solve roofline
+from the logged counts;
 ``advection`` holds the RK2 step and DVC pass times.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res 128] [--ppc 30]
                        [--mode M] [--modes fp64,mixed,learned[,learned-coarse,schur]]
-                       [--no-cpu-baseline] [--no-advection] [--no-reference] [--apply-only]
+                       [--no-cpu-baseline] [--no-advection] [--no-reference] [--no-floor]
+                       [--apply-only] [--advection-only]
        python bench.py --q2 --res 64 --ppc 20 [--modes fp64,mixed]
+       python bench.py --learned-eval 64 --res 128
        python bench.py --convection --res 96 --ppc 25 --steps 40 [--modes fp64,mixed,learned]
            [--series-out F.json] [--compare-series F.json]
 """
@@ -26,6 +29,7 @@ import json
 import os
 import sys
 import time
+import warnings
 
 import numpy as np
 
@@ -112,15 +116,64 @@ def time_mode(P, mode, steps, warmup, rtol):
     return out, sol
 
 
-def reference_solution(P, rtol=1e-10):
-    """FP64 reference: the coupled FP64 solve to rtol 1e-10 (the FP64 path is tied to the
-    oracle's solution to 1e-8 by tests/test_gpu_sinker32.py at 32^3)."""
-    s = make_solver(P, "fp64", rtol)
-    s.solve()
+def reference_solution(P, rtol=1e-11, restart=200, max_its=6000):
+    """FP64 reference: the coupled solve in FP64 throughout (FP64 cycle, FP64 Krylov, restart
+    200) with the learned Schur stencil, to rtol 1e-11.  At 128^3 the plain approxS FP64 solve
+    stagnates near 1e-8 (a few percent per 250 iterations), the learned one keeps a steady rate
+    (about 3x per 250 iterations near 1e-10).  Q1/dQ0 pressure is ill-conditioned: solutions
+    at residual 1e-8 still differ by ~1e-1 in pressure, so the reference must be tight.  Without
+    a trained model the plain FP64 solve is used with the same settings.  Both FP64 paths are
+    tied to the oracle's converged solution by tests/test_gpu_sinker32.py at 32^3."""
+    uw = P["uw"]
+    from underworld2_amd import learned
+    model = learned.load_default_schur()
+    s = uw.systems.Solver(P["stokes"])
+    s.set_method("block")
+    s.set_precision("fp64")
+    s.set_outer_rtol(rtol)
+    s.set_restart(restart)
+    s.options.scr.ksp_max_it = max_its
+    s.set_learned(model)
+    t0 = time.perf_counter()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        s.solve()
+    t = time.perf_counter() - t0
     st = s.get_stats()
+    rel = float(np.hypot(st.res_momentum, st.res_continuity) / st.norm_f)
     return (np.array(P["vel"].data), np.array(P["p"].data[:, 0])), {
-        "rtol": rtol, "outer_its": int(st.outer_its),
-        "rel_residual_fp64": float(np.hypot(st.res_momentum, st.res_continuity) / st.norm_f)}
+        "solver": "coupled FP64, " + ("learned stencil" if model is not None else "approxS"), "restart": restart,
+        "rtol": rtol, "outer_its": int(st.outer_its), "converged": bool(st.converged), "rel_residual_fp64": rel,
+        "fallback_used": int(st.fallback_used), "seconds": t}
+
+
+def iteration_floor(P, rtol=1e-6, vel_rtol=1e-8):
+    """Outer FGMRES count of the coupled FP64 solve when the preconditioner's velocity block is
+    solved (inner FGMRES + V-cycle to vel_rtol) instead of one V-cycle: the count that the Schur
+    block alone sets, for the plain approxS diagonal and for the learned stencil.  When it sits
+    near the V-cycle count, the Schur block, not the velocity cycle, is what sets the iterations."""
+    from underworld2_amd import learned
+    out = {"velocity_block_rtol": vel_rtol}
+    model = learned.load_default_schur()
+    for name, m in (("approxS", None), ("learned_stencil", model)):
+        if name == "learned_stencil" and model is None:
+            continue
+        s = P["uw"].systems.Solver(P["stokes"])
+        s.set_method("block")
+        s.set_precision("fp64")
+        s.set_outer_rtol(rtol)
+        s.set_learned(m)
+        s.set_velocity_block_solve(vel_rtol)
+        try:
+            t0 = time.perf_counter()
+            s.solve()
+            t = time.perf_counter() - t0
+        finally:
+            s.set_velocity_block_solve(0.0)
+        st = s.get_stats()
+        out[name] = {"outer_its": int(st.outer_its), "vcycles": int(st.vcycles), "seconds": t,
+                     "converged": bool(st.converged), "fallback_used": int(st.fallback_used)}
+    return out
 
 
 def solution_error(sol, ref):
@@ -412,6 +465,16 @@ def apply_only(P):
     print(json.dumps(out), flush=True)
 
 
+def advection_only(P):
+    """One mixed-precision solve for the velocity, then the RK2 + relocation steps and the DVC
+    pass of time_advection, plus the library's plain copy kernel of known size, for kernel-trace
+    and hardware counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) of the advection step."""
+    s = make_solver(P, "mixed", 1e-6)
+    s.solve()
+    out = {"advection": time_advection(P), "copy": stream_copy(1 << 27, 5)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,9 +492,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-advection", action="store_true")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 reference solve and error report")
+    ap.add_argument("--no-floor", action="store_true",
+                    help="skip the iteration-floor solves (velocity block solved instead of one V-cycle)")
     ap.add_argument("--apply-only", action="store_true")
+    ap.add_argument("--advection-only", action="store_true",
+                    help="one solve, then 10 RK2 steps + a DVC pass and a copy kernel (for counter runs)")
     ap.add_argument("--q2", action="store_true",
                     help="Q2/dPc1 3D sinker: K apply (matrix cores / vector) and solves at --res, --ppc")
+    ap.add_argument("--learned-eval", type=int, default=0, metavar="N",
+                    help="evaluate the learned Schur stencil on N unseen random --res^3 problems (FP64, mixed, "
+                         "learned + mixed per field) instead")
+    ap.add_argument("--learned-eval-start", type=int, default=0, metavar="K",
+                    help="learned-eval: first field index (seed 700000 + K), to split the fields over runs")
+    ap.add_argument("--floor-fields", type=int, default=8,
+                    help="learned-eval: fields on which the iteration floor is also measured")
     ap.add_argument("--convection", action="store_true",
                     help="run the 3D thermal-convection time series (--res, --ppc, --steps, --modes) instead")
     ap.add_argument("--series-out", default=None,
@@ -452,11 +526,17 @@ def main():
     if args.q2:
         q2_main(args)
         return
+    if args.learned_eval:
+        learned_eval_main(args)
+        return
     res = (args.res,) * 3
     P = build_problem(res, args.ppc)
     if args.apply_only:
         P["stokes"]._prepare_values()
         apply_only(P)
+        return
+    if args.advection_only:
+        advection_only(P)
         return
     modes = {}
     sols = {}
@@ -472,11 +552,16 @@ def main():
             log(json.dumps(modes[m]))
     reference = None
     if not args.no_reference:
-        log("FP64 reference solution (coupled FP64, rtol 1e-10) ...")
+        log("FP64 reference solution (coupled FP64 with the learned stencil, rtol 1e-11) ...")
         ref, reference = reference_solution(P)
         for m in modes:
             modes[m]["error_vs_fp64_reference"] = solution_error(sols[m], ref)
         log(json.dumps(reference))
+    floor = None
+    if not args.no_floor:
+        log("iteration floor (coupled FP64, velocity block solved to 1e-8) ...")
+        floor = iteration_floor(P, args.rtol)
+        log(json.dumps(floor))
     head = args.mode
     if head == "auto":
         head = "learned" if "learned" in modes else ("mixed" if "mixed" in modes else next(iter(modes)))
@@ -508,20 +593,103 @@ def main():
                                f"mode={head}" + ("" if head in ("fp64", "schur") else
                                                  " (FP32 cycle with bf16 element weights, FP64 Krylov)")},
         "roofline": roof, "cpu_baseline": cpu, "modes": modes, "advection": adv, "fp64_reference": reference,
+        "iteration_floor": floor,
         "config1_plain_method": cfg1,
     }
     if others:
         line["roofline_other_applies"] = others
-    for plain in ("fp64", "schur"):
-        if plain in modes:
+    for plain in ("fp64", "schur", "mixed"):
+        if plain in modes and plain != head:
             line[f"speedup_vs_{plain}"] = H["solves_per_sec"] / modes[plain]["solves_per_sec"]
     print(json.dumps(line), flush=True)
 
 
+def learned_eval_main(args):
+    """The learned Schur stencil on unseen random problems: GRF viscosity 10^(c g), c ~ U(0, 6),
+    correlation length ~ U(0.1, 0.3), density 1 in 1-5 random spheres (learned.random_problem,
+    seeds 700000 + k, disjoint from the training seeds 5000-5511), at --res.  Each field is
+    assembled once (Gauss quadrature; the host evaluation of the random fields is not timed) and
+    solved to outer rtol --rtol by the plain FP64 coupled solver, the mixed-precision one, and the
+    mixed one with the learned stencil (with the plain approxS preconditioner as its fallback);
+    the solver time is the library's own (set-up + Krylov solve).  On the first --floor-fields
+    fields the coupled FP64 solve is repeated with the velocity block solved (inner solve to
+    1e-8), with and without the stencil: the iteration floor the pressure block sets."""
+    from underworld2_amd import learned
+    model = learned.load_default_schur()
+    if model is None:
+        raise SystemExit("no trained Schur-stencil model")
+    rows = []
+    for k in range(args.learned_eval_start, args.learned_eval_start + args.learned_eval):
+        seed = 700000 + k
+        t0 = time.perf_counter()
+        st, v, p, desc = learned.random_problem(args.res, seed)
+        st._prepare_values()
+        row = dict(desc, build_s=time.perf_counter() - t0)
+        for name, prec, m in (("fp64", "fp64", None), ("mixed", "mixed", None), ("learned", "mixed", model)):
+            s = P_solver(st, prec, args.rtol, m)
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                s.solve(reinitialise=False)
+            stt = s.get_stats()
+            row[name] = {"its": int(stt.outer_its), "s": float(stt.setup_time + stt.solve_time),
+                         "converged": bool(stt.converged),
+                         "rel_residual_fp64": float(np.hypot(stt.res_momentum, stt.res_continuity) / stt.norm_f),
+                         "fallback_used": int(stt.fallback_used)}
+        if k < args.floor_fields:
+            for name, m in (("floor_approxS", None), ("floor_learned", model)):
+                s = P_solver(st, "fp64", args.rtol, m)
+                s.set_velocity_block_solve(1e-8)
+                try:
+                    s.solve(reinitialise=False)
+                finally:
+                    s.set_velocity_block_solve(0.0)
+                row[name] = int(s.get_stats().outer_its)
+        rows.append(row)
+        fl = f", floor {row['floor_approxS']} / {row['floor_learned']}" if "floor_approxS" in row else ""
+        log(f"field {k} seed {seed} c={desc['log10_contrast']:.2f} ell={desc['corr_len']:.2f} "
+            f"spheres={desc['spheres']}: its fp64 {row['fp64']['its']} mixed {row['mixed']['its']} "
+            f"learned {row['learned']['its']} (fallback {row['learned']['fallback_used']}){fl}, solver s "
+            f"{row['fp64']['s']:.3f} / {row['mixed']['s']:.3f} / {row['learned']['s']:.3f}")
+        del st, v, p
+    its_ratio = np.array([r["learned"]["its"] / r["mixed"]["its"] for r in rows])
+    c = np.array([r["log10_contrast"] for r in rows])
+    summ = {
+        "fields": len(rows), "res": args.res, "rtol": args.rtol,
+        "all_converged": all(r[m]["converged"] for r in rows for m in ("fp64", "mixed", "learned")),
+        "max_rel_residual_fp64": max(r[m]["rel_residual_fp64"] for r in rows for m in ("fp64", "mixed", "learned")),
+        "fallbacks": int(sum(r["learned"]["fallback_used"] for r in rows)),
+        "mean_its": {m: float(np.mean([r[m]["its"] for r in rows])) for m in ("fp64", "mixed", "learned")},
+        "its_ratio_learned_over_mixed": {"median": float(np.median(its_ratio)), "mean": float(its_ratio.mean()),
+                                         "min": float(its_ratio.min()), "max": float(its_ratio.max()),
+                                         "mean_log10_contrast_ge_4": float(its_ratio[c >= 4].mean())
+                                         if np.any(c >= 4) else None},
+        "solver_s": {m: float(sum(r[m]["s"] for r in rows)) for m in ("fp64", "mixed", "learned")},
+    }
+    fl = [r for r in rows if "floor_approxS" in r]
+    if fl:
+        summ["floor_fields"] = len(fl)
+        summ["floor_mean_its"] = {"vcycle_approxS_fp64": float(np.mean([r["fp64"]["its"] for r in fl])),
+                                  "solved_velocity_approxS": float(np.mean([r["floor_approxS"] for r in fl])),
+                                  "solved_velocity_learned": float(np.mean([r["floor_learned"] for r in fl]))}
+    summ["speedup_learned_vs_fp64"] = summ["solver_s"]["fp64"] / summ["solver_s"]["learned"]
+    summ["speedup_learned_vs_mixed"] = summ["solver_s"]["mixed"] / summ["solver_s"]["learned"]
+    print(json.dumps({"learned_eval": summ, "fields": rows}), flush=True)
+
+
+def P_solver(st, precision, rtol, model):
+    import underworld2_amd as uw
+    s = uw.systems.Solver(st)
+    s.set_method("block")
+    s.set_precision(precision)
+    s.set_outer_rtol(rtol)
+    s.set_learned(model)
+    return s
+
+
 def q2_main(args):
-    """Q2/dPc1 3D sinker (sphere r 0.1, eta 1e3, particle quadrature, free slip): the Q2 K apply on
-    the matrix cores and on the vector kernel (FP64 and FP32, HIP-event timed), and one coupled
-    solve per precision mode."""
+    """Q2/dPc1 3D sinker (sphere r 0.1, eta 1e3, particle quadrature, free slip): the Q2 K apply by
+    the sum-factorised, matrix-core and dense vector kernels (FP64 and FP32, HIP-event timed), and
+    one coupled solve per precision mode with the default (sum-factorised) kernel."""
     import underworld2_amd as uw
     import underworld2_amd._lib as L
     fn = uw.function
@@ -541,14 +709,18 @@ def q2_main(args):
     stokes._prepare_values()
     E = mesh.elementsGlobal
     out = {"workload": f"3D sinker {args.res}^3 Q2/dPc1, {args.ppc} ppc, particle quadrature", "apply": {}, "solve": {}}
-    for mf in (1, 0):
-        L.lib().uwg_set_q2_mfma(mf)
+    # flops per element of each kernel's algorithm: the sum-factorised passes (12,420
+    # multiply-adds) plus the stress (18 per point); the dense contractions
+    kflop = {2: 27.09, 1: 121.5, 0: 121.5}
+    for kind, name in ((2, "sumfact"), (1, "mfma"), (0, "vector")):
+        L.check(L.lib().uwg_set_q2_apply_kernel(kind), "kernel")
         for prec in (0, 1):
             ms = ctypes.c_double()
             L.check(L.lib().uwg_stokes_time_apply(stokes._h, prec, 0, 20, ctypes.byref(ms)), "time_apply")
-            out["apply"][f"{'mfma' if mf else 'vector'}_{'fp32' if prec else 'fp64'}"] = {
-                "ms": ms.value, "TFLOP_per_s": E * 121.5e3 / (ms.value * 1e-3) / 1e12}
-    L.lib().uwg_set_q2_mfma(1)
+            out["apply"][f"{name}_{'fp32' if prec else 'fp64'}"] = {
+                "ms": ms.value, "kflop_per_element": kflop[kind],
+                "TFLOP_per_s": E * kflop[kind] * 1e3 / (ms.value * 1e-3) / 1e12}
+    L.check(L.lib().uwg_set_q2_apply_kernel(2), "kernel")
     for m in [x.strip() for x in args.modes.split(",") if x.strip() in ("fp64", "mixed")]:
         s = make_solver({"uw": uw, "stokes": stokes}, m, args.rtol)
         s.solve()

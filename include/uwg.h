@@ -182,8 +182,8 @@ int uwg_stokes_residuals(uwg_stokes *st, double *out3);
  * 5 = K in FP32 with FP32 weights (4 and 5: Q1 levels; x rounded to FP32, y widened),
  * 6 = the multigrid residual kernel y = b - K x with b = x (FP64), 7 = the same kernel as the
  * FP32 cycle runs it (cycle weights; x rounded to FP32, y widened).
- * Q2/dPc1 systems: level 0 is the Q2 level (pressure vectors [E][dim+1]), levels 1.. are the
- * Q1 levels of the multigrid hierarchy below it (K only) */
+ * Q2/dPc1 systems: level 0 is the Q2 level (pressure vectors [E][dim+1]; which 0..3, and 4 =
+ * its FP32 K), levels 1.. are the Q1 levels of the multigrid hierarchy below it (K only) */
 int uwg_stokes_apply(uwg_stokes *st, int which, int level, const double *x, double *y);
 int uwg_stokes_levels(uwg_stokes *st, int *nlev, int *res_out /*[nlev][3]*/);
 /* per-element quadrature weights W[3^dim][E] of a level, element force fe[E][2^dim*dim],
@@ -198,9 +198,14 @@ int uwg_stokes_get_rhs(uwg_stokes *st, double *f);
 int uwg_stokes_load_nn(uwg_stokes *st, int level, int nparams, const float *params, int channels, int depth);
 int uwg_stokes_set_nn_mode(uwg_stokes *st, int mode);
 /* fallback monitor: while the network is in use the FP64 outer residual must fall by at least
- * min_drop (fraction) over every `window` iterations, else multigrid alone takes over
- * (defaults 10 and 0.1) */
+ * min_drop (fraction) over every `window` iterations, else the plain preconditioner takes over
+ * (defaults 20 and 0.01) */
 int uwg_stokes_set_nn_monitor(uwg_stokes *st, int window, double min_drop);
+/* diagnostic, no counterpart in the reference: with rtol > 0 the coupled FP64 solve (method 1,
+ * precision 0) applies K^-1 in its block preconditioner by an inner FGMRES + V-cycle solve to
+ * rtol instead of one V-cycle, so its outer count is the floor set by the Schur-block choice
+ * (approxS or the learned stencil); 0 restores the V-cycle */
+int uwg_stokes_set_velocity_block_solve(uwg_stokes *st, double rtol);
 /* record up to max_samples vectors of `level` (every `every`-th visit) during multigrid
  * cycles: kind 0 the right-hand side restricted to the level, kind 1 the residual that the
  * cycle below the level leaves (b - K x after it); retrieved as float[count][N_level*dim] */
@@ -260,7 +265,9 @@ int uwg_stokes_load_schur_nn(uwg_stokes *st, int channels, int layers, long npar
  * workgroup slots evenly), element rows per block of the 2D K apply (kz2 > 0; default 32);
  * zero leaves a setting unchanged */
 int uwg_set_apply_chunk(int kz3, int kz2);
-/* 3D Q2 K apply on the matrix cores (1, default) or the vector kernel (0) */
+/* 3D Q2 K apply kernel: sum-factorised (2, default), matrix cores (1) or the dense vector
+ * kernel (0); uwg_set_q2_mfma(on) selects 1 or 0 */
+int uwg_set_q2_apply_kernel(int kind);
 int uwg_set_q2_mfma(int on);
 /* element weights of the 3D FP32 multigrid cycle (precision 1): bfloat16 (1, default; the
  * element contractions take bf16 inputs, FP32 arithmetic and accumulation) or FP32 (0).

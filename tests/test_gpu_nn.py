@@ -157,7 +157,7 @@ def test_fallback_monitor_switches_to_multigrid():
     assert s.converged and s.fallback_used == 1
     assert s.res_momentum < 1e-5 * s.norm_f
     # the monitor is reset for the next solve on the handle
-    L.check(L.lib().uwg_stokes_set_nn_monitor(st._h, 10, 0.1))
+    L.check(L.lib().uwg_stokes_set_nn_monitor(st._h, 20, 0.01))  # the defaults
 
 
 def test_coarse_correction_is_experimental_and_bounded():

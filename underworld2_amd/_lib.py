@@ -48,6 +48,7 @@ SIGNATURES = {
     "uwg_mesh_create": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
     "uwg_mesh_create_q2": [ctypes.c_int, _ip, _dp, _dp, ctypes.POINTER(_vp)],
     "uwg_set_q2_mfma": [ctypes.c_int],
+    "uwg_set_q2_apply_kernel": [ctypes.c_int],
     "uwg_swarm_population_control": [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                      ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)],
     "uwg_advdiff_create": [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(_vp)],
@@ -102,6 +103,7 @@ SIGNATURES = {
     "uwg_stokes_load_nn": [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int],
     "uwg_stokes_set_nn_mode": [_vp, ctypes.c_int],
     "uwg_stokes_set_nn_monitor": [_vp, ctypes.c_int, ctypes.c_double],
+    "uwg_stokes_set_velocity_block_solve": [_vp, ctypes.c_double],
     "uwg_stokes_capture": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "uwg_stokes_get_captured": [_vp, _fp, _ip],
     "uwg_stokes_level_solve": [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, _ip],

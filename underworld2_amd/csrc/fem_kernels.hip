@@ -89,10 +89,60 @@ __device__ inline double kdiag_entry(const double* Wg, int a, int i, const doubl
   return acc;
 }
 
+// Element stiffness diagonal from the raw moments (before the conversion to Gauss weights):
+// dN_a/dxi_j = 2^-d s_j prod_{k != j} (1 + s_k xi_k) (s = the node's corner signs), so
+//   sum_p om_p (dN_a/dxi_j)^2 = 4^-d sum over the other axes of c_p(s_k) c_q(s_l) M[...]
+// with c_0 = 1, c_1 = 2 s, c_2 = 1 and exponent 0 on axis j; K[(a,i),(a,i)] =
+// sc_i^2 T_i + sum_j sc_j^2 T_j (sc = 2/h).  Equal to kdiag_entry of the converted weights: the
+// integrand has degree <= 2 per axis, which the Gauss weights reproduce exactly.
+template <int DIM>
+__device__ __forceinline__ void kdiag_from_moments(const double* M, int a, const double* sc, double* kd) {
+  const double sgn[3] = {(a & 1) ? 1.0 : -1.0, (a & 2) ? 1.0 : -1.0, (a & 4) ? 1.0 : -1.0};
+  double T[3];
+  if (DIM == 3) {
+    const int str[3] = {1, 3, 9};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int k = j == 0 ? 1 : 0, l = j == 2 ? 1 : 2;
+      const double ck[3] = {1.0, 2.0 * sgn[k], 1.0}, cl[3] = {1.0, 2.0 * sgn[l], 1.0};
+      double t = 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        double u = 0.0;
+#pragma unroll
+        for (int p = 0; p < 3; p++) u += ck[p] * M[p * str[k] + q * str[l]];
+        t += cl[q] * u;
+      }
+      T[j] = (1.0 / 64.0) * sc[j] * sc[j] * t;
+    }
+    const double sum = T[0] + T[1] + T[2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) kd[i] = T[i] + sum;
+  } else {
+    const int str[2] = {1, 3};
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int k = 1 - j;
+      const double ck[3] = {1.0, 2.0 * sgn[k], 1.0};
+      double t = 0.0;
+#pragma unroll
+      for (int p = 0; p < 3; p++) t += ck[p] * M[p * str[k]];
+      T[j] = (1.0 / 16.0) * sc[j] * sc[j] * t;
+    }
+    const double sum = T[0] + T[1];
+#pragma unroll
+    for (int i = 0; i < 2; i++) kd[i] = T[i] + sum;
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // A6/A7/A9 particle -> element moments, element force vector, approxS diagonal.
-// One thread per element, particles in cell-table (slot) order.  Function values by
-// particle id: material tables (mat != null) or per-particle arrays.
+// Eight lanes per element: lane l takes the element's slots l, l + 8, ... (consecutive lanes
+// read consecutive slots, and the dependent loads -- particle id, material, table -- of four
+// slots per lane are issued together), then a three-step butterfly over the eight lanes sums the 3^d
+// moments and the force terms (every lane ends with the same sums; fixed order, so the result
+// is deterministic).  Each lane converts the moments and writes every eighth output value.
+// Function values by particle id: material tables (mat != null) or per-particle arrays.
 //   W[g*Es+e] (Es: padded stride), fe[e*NN*DIM + a*DIM + i], approxS[e]
 // f_e: VectorAssemblyTerm_NA__Fn.cpp:143-202; approxS: UzawaPreconditionerTerm.c:124-209
 // ---------------------------------------------------------------------------------------
@@ -104,77 +154,133 @@ struct MomArgs {
   double Ge[24];  // constant element gradient matrix, passed by value
 };
 
+constexpr int kMomLanes = 8;
+constexpr int kMomUnroll = 4;
+
 template <int DIM>
-__global__ void k_moments(MomArgs A, const int* __restrict__ cell_ptr, const int* __restrict__ cell_pid,
-                          const double* __restrict__ xi_slot, const double* __restrict__ w_slot,
-                          const int* __restrict__ mat, const double* __restrict__ eta_tab,
-                          const double* __restrict__ force_tab, const double* __restrict__ eta_p,
-                          const double* __restrict__ force_p, double* __restrict__ W, double* __restrict__ fe,
-                          double* __restrict__ approxS) {
-  constexpr int NN = DIM == 3 ? 8 : 4, Q = DIM == 3 ? 27 : 9;
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= A.E) return;
-  double M[27], F[NN * DIM];
+__global__ __launch_bounds__(256) void k_moments(MomArgs A, const int* __restrict__ cell_ptr,
+                                                 const int* __restrict__ cell_pid, const double* __restrict__ xi_slot,
+                                                 const double* __restrict__ w_slot, const int* __restrict__ mat,
+                                                 const double* __restrict__ eta_tab,
+                                                 const double* __restrict__ force_tab, const double* __restrict__ eta_p,
+                                                 const double* __restrict__ force_p, double* __restrict__ W,
+                                                 double* __restrict__ fe, double* __restrict__ approxS) {
+  constexpr int NN = DIM == 3 ? 8 : 4, Q = DIM == 3 ? 27 : 9, NF = NN * DIM;
+  const int lane = threadIdx.x & (kMomLanes - 1);
+  const long e = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kMomLanes;
+  const bool live = e < A.E;
+  double M[Q], F[NF];
+#pragma unroll
   for (int q = 0; q < Q; q++) M[q] = 0.0;
-  for (int q = 0; q < NN * DIM; q++) F[q] = 0.0;
-  const int s0 = cell_ptr[e], s1 = cell_ptr[e + 1];
-  for (int s = s0; s < s1; s++) {
-    const int p = cell_pid[s];
-    double xi[3];
-    for (int d = 0; d < DIM; d++) xi[d] = xi_slot[(long)s * DIM + d];
-    const double w = w_slot[s] * A.detJ;
-    double eta, f[3];
-    if (mat) {
-      int m = mat[p];
-      if (m < 0 || m >= A.nmat) {  // index outside the table: poison the element
-        m = 0;
-        eta = __builtin_nan("");
-      } else {
-        eta = eta_tab[m];
-      }
-      for (int d = 0; d < DIM; d++) f[d] = force_tab[m * DIM + d];
-    } else {
-      eta = eta_p[p];
-      for (int d = 0; d < DIM; d++) f[d] = force_p[(long)p * DIM + d];
+#pragma unroll
+  for (int q = 0; q < NF; q++) F[q] = 0.0;
+  const int s0 = live ? cell_ptr[e] : 0, s1 = live ? cell_ptr[e + 1] : 0;
+  // kMomUnroll slots per lane and pass: the dependent loads (particle id, then its material,
+  // then the table entries) of the batch are issued together
+  for (int sb = s0 + lane; sb < s1; sb += kMomLanes * kMomUnroll) {
+    int pid[kMomUnroll];
+    bool ok[kMomUnroll];
+    double xi[kMomUnroll][3], w[kMomUnroll];
+#pragma unroll
+    for (int u = 0; u < kMomUnroll; u++) {
+      const int s = sb + u * kMomLanes;
+      ok[u] = s < s1;
+      const int sc = ok[u] ? s : sb;
+      pid[u] = cell_pid[sc];
+      for (int d = 0; d < DIM; d++) xi[u][d] = xi_slot[(long)sc * DIM + d];
+      w[u] = w_slot[sc] * A.detJ;
     }
-    const double om = eta * w;
-    double px[3] = {1.0, xi[0], xi[0] * xi[0]}, py[3] = {1.0, xi[1], xi[1] * xi[1]};
-    if (DIM == 3) {
-      double pz[3] = {1.0, xi[2], xi[2] * xi[2]};
-      for (int kz = 0; kz < 3; kz++)
-        for (int ky = 0; ky < 3; ky++) {
-          double t = om * pz[kz] * py[ky];
-          for (int kx = 0; kx < 3; kx++) M[kz * 9 + ky * 3 + kx] += t * px[kx];
+    double eta[kMomUnroll], f[kMomUnroll][3];
+#pragma unroll
+    for (int u = 0; u < kMomUnroll; u++) {
+      const int p = pid[u];
+      if (mat) {
+        int m = mat[p];
+        if (m < 0 || m >= A.nmat) {  // index outside the table: poison the element
+          m = 0;
+          eta[u] = __builtin_nan("");
+        } else {
+          eta[u] = eta_tab[m];
         }
-    } else {
-      for (int ky = 0; ky < 3; ky++) {
-        double t = om * py[ky];
-        for (int kx = 0; kx < 3; kx++) M[ky * 3 + kx] += t * px[kx];
+        for (int d = 0; d < DIM; d++) f[u][d] = force_tab[m * DIM + d];
+      } else {
+        eta[u] = eta_p[p];
+        for (int d = 0; d < DIM; d++) f[u][d] = force_p[(long)p * DIM + d];
       }
     }
-    // f_e[a][i] += w detJ f_i N_a(xi)
-    double lx[2] = {0.5 * (1.0 - xi[0]), 0.5 * (1.0 + xi[0])};
-    double ly[2] = {0.5 * (1.0 - xi[1]), 0.5 * (1.0 + xi[1])};
-    double lz[2] = {1.0, 1.0};
-    if (DIM == 3) { lz[0] = 0.5 * (1.0 - xi[2]); lz[1] = 0.5 * (1.0 + xi[2]); }
-    for (int a = 0; a < NN; a++) {
-      double Na = lx[a & 1] * ly[(a >> 1) & 1] * lz[(a >> 2) & 1];
-      for (int d = 0; d < DIM; d++) F[a * DIM + d] += w * f[d] * Na;
+#pragma unroll
+    for (int u = 0; u < kMomUnroll; u++) {
+      if (!ok[u]) continue;
+      const double om = eta[u] * w[u];
+      const double px[3] = {1.0, xi[u][0], xi[u][0] * xi[u][0]}, py[3] = {1.0, xi[u][1], xi[u][1] * xi[u][1]};
+      if (DIM == 3) {
+        const double pz[3] = {1.0, xi[u][2], xi[u][2] * xi[u][2]};
+#pragma unroll
+        for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+          for (int ky = 0; ky < 3; ky++) {
+            const double t = om * pz[kz] * py[ky];
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) M[kz * 9 + ky * 3 + kx] += t * px[kx];
+          }
+      } else {
+#pragma unroll
+        for (int ky = 0; ky < 3; ky++) {
+          const double t = om * py[ky];
+#pragma unroll
+          for (int kx = 0; kx < 3; kx++) M[ky * 3 + kx] += t * px[kx];
+        }
+      }
+      // f_e[a][i] += w detJ f_i N_a(xi)
+      const double lx[2] = {0.5 * (1.0 - xi[u][0]), 0.5 * (1.0 + xi[u][0])};
+      const double ly[2] = {0.5 * (1.0 - xi[u][1]), 0.5 * (1.0 + xi[u][1])};
+      double lz[2] = {1.0, 1.0};
+      if (DIM == 3) {
+        lz[0] = 0.5 * (1.0 - xi[u][2]);
+        lz[1] = 0.5 * (1.0 + xi[u][2]);
+      }
+      double wf[3];
+      for (int d = 0; d < DIM; d++) wf[d] = w[u] * f[u][d];
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        const double Na = lx[a & 1] * ly[(a >> 1) & 1] * lz[(a >> 2) & 1];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) F[a * DIM + d] += wf[d] * Na;
+      }
     }
   }
-  moments_to_weights<DIM>(M);
-  for (int q = 0; q < Q; q++) W[(long)q * A.Es + e] = M[q];
-  for (int q = 0; q < NN * DIM; q++) fe[e * NN * DIM + q] = F[q];
-  // approxS_e = sum_v Ge[v]^2 / Ke[v][v]
-  double sc[3] = {2.0 / A.h[0], 2.0 / A.h[1], DIM == 3 ? 2.0 / A.h[2] : 1.0};
-  double S = 0.0;
-  for (int a = 0; a < NN; a++)
-    for (int i = 0; i < DIM; i++) {
-      double kd = kdiag_entry<DIM>(M, a, i, sc);
-      double g = A.Ge[a * DIM + i];
-      S += g * (g / kd);
+  // butterfly over the element's eight lanes (all lanes take part; dead elements add zeros)
+#pragma unroll
+  for (int m = 1; m < kMomLanes; m <<= 1) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) M[q] += __shfl_xor(M[q], m, kMomLanes);
+#pragma unroll
+    for (int q = 0; q < NF; q++) F[q] += __shfl_xor(F[q], m, kMomLanes);
+  }
+  if (!live) return;
+  // approxS_e = sum_v Ge[v]^2 / Ke[v][v], Ke's diagonal from the raw moments
+  const double sc[3] = {2.0 / A.h[0], 2.0 / A.h[1], DIM == 3 ? 2.0 / A.h[2] : 1.0};
+  if (lane == 0) {
+    double S = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; a++) {
+      double kd[3];
+      kdiag_from_moments<DIM>(M, a, sc, kd);
+#pragma unroll
+      for (int i = 0; i < DIM; i++) {
+        const double g = A.Ge[a * DIM + i];
+        S += g * (g / kd[i]);
+      }
     }
-  approxS[e] = S;
+    approxS[e] = S;
+  }
+  moments_to_weights<DIM>(M);
+#pragma unroll
+  for (int q = 0; q < Q; q++)
+    if ((q & (kMomLanes - 1)) == lane) W[(long)q * A.Es + e] = M[q];
+#pragma unroll
+  for (int q = 0; q < NF; q++)
+    if ((q & (kMomLanes - 1)) == lane) fe[e * NF + q] = F[q];
 }
 
 static MomArgs mom_args(const Geom& g) {
@@ -199,12 +305,13 @@ void launch_moments(const Geom& g, const int* cell_ptr, const int* cell_pid, con
   MomArgs A = mom_args(g);
   A.nmat = nmat;
   for (int q = 0; q < 24; q++) A.Ge[q] = q < (1 << g.dim) * g.dim ? Ge[q] : 0.0;
+  const unsigned nb = nblocks(g.E * kMomLanes, 256);
   if (g.dim == 3)
-    hipLaunchKernelGGL(k_moments<3>, nblocks(g.E, 128), 128, 0, s, A, cell_ptr, cell_pid, xi_slot, w_slot, mat,
-                       eta_tab, force_tab, eta_p, force_p, W, fe, approxS);
+    hipLaunchKernelGGL(k_moments<3>, nb, 256, 0, s, A, cell_ptr, cell_pid, xi_slot, w_slot, mat, eta_tab, force_tab,
+                       eta_p, force_p, W, fe, approxS);
   else
-    hipLaunchKernelGGL(k_moments<2>, nblocks(g.E, 128), 128, 0, s, A, cell_ptr, cell_pid, xi_slot, w_slot, mat,
-                       eta_tab, force_tab, eta_p, force_p, W, fe, approxS);
+    hipLaunchKernelGGL(k_moments<2>, nb, 256, 0, s, A, cell_ptr, cell_pid, xi_slot, w_slot, mat, eta_tab, force_tab,
+                       eta_p, force_p, W, fe, approxS);
   UWG_HIP(hipGetLastError());
 }
 

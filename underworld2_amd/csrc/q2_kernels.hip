@@ -408,8 +408,167 @@ void q2_mfma_matrix(const Geom& g, std::vector<double>& D) {
   }
 }
 
-static int g_q2_mfma = 1;
-void set_q2_mfma(int on) { g_q2_mfma = on ? 1 : 0; }
+// ---------------------------------------------------------------------------------------
+// Sum-factorised 3D apply.  On the tensor 3 x 3 x 3 element and the 5 x 5 x 5 Gauss rule every
+// shape-function derivative factorises into 1D tables, B[m][a] = phi_a(g_m) and
+// D[m][a] = phi_a'(g_m) (times 2/h on its axis), so the gradient at the 125 points is three
+// passes of 1D contractions (x, then y, then z; 2,070 multiply-adds per velocity component
+// instead of 27 x 125 x 3) and the transposed contraction the same passes in reverse: 12,420
+// multiply-adds, about 27 kflop per element with the pointwise stress, against 121.5 kflop for
+// the dense contraction.  Five elements per 128-lane workgroup; the x and y passes and their
+// transposes are spread over the lanes through LDS, each lane (element, qx, qy) runs the
+// z pass, the stress at its five points and the transposed z pass in registers.  Elements of
+// one parity class share no vertex, so the colour launches add into y without atomics.
+// ---------------------------------------------------------------------------------------
+constexpr int Q2S_E = 5;
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_q2_apply_sf(Q2G G, int cx, int cy, int cz, int ncx, int ncy, long nce,
+                                                     const T* __restrict__ W, const T* __restrict__ x,
+                                                     T* __restrict__ y, const uint8_t* __restrict__ bc) {
+  __shared__ T xs[Q2S_E][81];        // u[(ax + 3 ay + 9 az) * 3 + i]
+  __shared__ T U[Q2S_E][2][5][27];   // x pass [B|D][qx][(ay + 3 az) * 3 + i]; reused for the y^T pass
+  __shared__ T A[Q2S_E][25][27];     // after the z^T pass, per (qx, qy): [path * 9 + az * 3 + i]
+  __shared__ T tb[3][5][3];          // B, D * (2/h_x), D * (2/h_y); D * (2/h_z) is formed in the z pass
+  const int t = threadIdx.x;
+  const long ce0 = (long)blockIdx.x * Q2S_E;
+  auto elem_of = [&](long ce) -> long {
+    const int i = cx + 2 * (int)(ce % ncx), j = cy + 2 * (int)((ce / ncx) % ncy);
+    const int k = cz + 2 * (int)(ce / ((long)ncx * ncy));
+    return i + (long)G.rx * (j + (long)G.ry * k);
+  };
+  if (t < 15) {
+    const int m = t / 3, a = t % 3;
+    tb[0][m][a] = (T)ph(a, g5(m));
+    tb[1][m][a] = (T)dph(a, g5(m)) * (T)G.s[0];
+    tb[2][m][a] = (T)dph(a, g5(m)) * (T)G.s[1];
+  }
+  // 1. element dofs (Dirichlet dofs read as zero)
+  for (int idx = t; idx < Q2S_E * 81; idx += 128) {
+    const int le = idx / 81, r = idx % 81;
+    T v = (T)0;
+    if (ce0 + le < nce) {
+      const long n = q2_vertex<3>(G, elem_of(ce0 + le), r / 3);
+      const int i = r % 3;
+      const uint8_t m = bc[n];
+      const T xv = x[n * 3 + i];  // loaded unconditionally: the mask does not gate the load
+      v = ((m >> i) & 1) ? (T)0 : xv;
+    }
+    xs[le][r] = v;
+  }
+  __syncthreads();
+  // 2. x pass: U[v][qx][L*3+i] = sum_ax (B|D)[qx][ax] u[(ax + 3 L) * 3 + i], L = ay + 3 az
+  for (int idx = t; idx < Q2S_E * 270; idx += 128) {
+    const int le = idx / 270, r = idx % 270, v = r / 135, qx = (r % 135) / 27, ln = r % 27;
+    const int L = ln / 3, i = ln % 3;
+    T acc = (T)0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) acc += tb[v][qx][ax] * xs[le][(ax + 3 * L) * 3 + i];
+    U[le][v][qx][ln] = acc;
+  }
+  __syncthreads();
+  // 3. per (element, qx, qy): y pass, z pass, stress at the five points, z^T pass
+  {
+    const int le = t / 25, qq = t % 25, qx = qq % 5, qy = qq / 5;
+    if (t < 125 && ce0 + le < nce) {
+      const long e = elem_of(ce0 + le);
+      T Vbb[3][3], Vdb[3][3], Vbd[3][3];  // [az][i]: z-derivative, x-derivative, y-derivative paths
+#pragma unroll
+      for (int az = 0; az < 3; az++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          T bb = (T)0, db = (T)0, bd = (T)0;
+#pragma unroll
+          for (int ay = 0; ay < 3; ay++) {
+            const int ln = (ay + 3 * az) * 3 + i;
+            bb += tb[0][qy][ay] * U[le][0][qx][ln];
+            db += tb[0][qy][ay] * U[le][1][qx][ln];
+            bd += tb[2][qy][ay] * U[le][0][qx][ln];
+          }
+          Vbb[az][i] = bb;
+          Vdb[az][i] = db;
+          Vbd[az][i] = bd;
+        }
+      T A1[3][3], A2[3][3], A3[3][3];
+#pragma unroll
+      for (int az = 0; az < 3; az++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) A1[az][i] = A2[az][i] = A3[az][i] = (T)0;
+#pragma unroll
+      for (int qz = 0; qz < 5; qz++) {
+        T bz[3], dz[3];
+#pragma unroll
+        for (int az = 0; az < 3; az++) {
+          bz[az] = tb[0][qz][az];
+          dz[az] = (T)dph(az, g5(qz)) * (T)G.s[2];
+        }
+        T g[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          g[i][0] = bz[0] * Vdb[0][i] + bz[1] * Vdb[1][i] + bz[2] * Vdb[2][i];
+          g[i][1] = bz[0] * Vbd[0][i] + bz[1] * Vbd[1][i] + bz[2] * Vbd[2][i];
+          g[i][2] = dz[0] * Vbb[0][i] + dz[1] * Vbb[1][i] + dz[2] * Vbb[2][i];
+        }
+        const T w = W[e * 125 + qx + 5 * qy + 25 * qz];
+        T tau[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) tau[i][j] = w * (g[i][j] + g[j][i]);
+#pragma unroll
+        for (int az = 0; az < 3; az++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            A1[az][i] += bz[az] * tau[i][0];
+            A2[az][i] += bz[az] * tau[i][1];
+            A3[az][i] += dz[az] * tau[i][2];
+          }
+      }
+#pragma unroll
+      for (int az = 0; az < 3; az++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          A[le][qq][0 * 9 + az * 3 + i] = A1[az][i];
+          A[le][qq][1 * 9 + az * 3 + i] = A2[az][i];
+          A[le][qq][2 * 9 + az * 3 + i] = A3[az][i];
+        }
+    }
+  }
+  __syncthreads();
+  // 4. y^T pass: C1[qx][L*3+i] = sum_qy B[qy][ay] A1, C2 = sum_qy (D[qy][ay] A2 + B[qy][ay] A3)
+  for (int idx = t; idx < Q2S_E * 270; idx += 128) {
+    const int le = idx / 270, r = idx % 270, v = r / 135, qx = (r % 135) / 27, ln = r % 27;
+    const int L = ln / 3, i = ln % 3, ay = L % 3, az = L / 3;
+    T acc = (T)0;
+#pragma unroll
+    for (int qy = 0; qy < 5; qy++) {
+      const T* a = A[le][qx + 5 * qy];
+      if (v == 0)
+        acc += tb[0][qy][ay] * a[0 * 9 + az * 3 + i];
+      else
+        acc += tb[2][qy][ay] * a[1 * 9 + az * 3 + i] + tb[0][qy][ay] * a[2 * 9 + az * 3 + i];
+    }
+    U[le][v][qx][ln] = acc;
+  }
+  __syncthreads();
+  // 5. x^T pass and scatter: y_ai += sum_qx D[qx][ax] C1 + B[qx][ax] C2
+  for (int idx = t; idx < Q2S_E * 81; idx += 128) {
+    const int le = idx / 81, r = idx % 81;
+    if (ce0 + le >= nce) continue;
+    const int a = r / 3, i = r % 3, ax = a % 3, L = a / 3;
+    T acc = (T)0;
+#pragma unroll
+    for (int qx = 0; qx < 5; qx++)
+      acc += tb[1][qx][ax] * U[le][0][qx][L * 3 + i] + tb[0][qx][ax] * U[le][1][qx][L * 3 + i];
+    const long n = q2_vertex<3>(G, elem_of(ce0 + le), a);
+    if (!((bc[n] >> i) & 1)) y[n * 3 + i] += acc;
+  }
+}
+
+// 3D apply kernel: 2 sum-factorised (default), 1 matrix cores, 0 dense vector kernel
+static int g_q2_kernel = 2;
+void set_q2_mfma(int on) { g_q2_kernel = on ? 1 : 0; }
+void set_q2_apply_kernel(int kind) { g_q2_kernel = kind; }
 
 // ---------------------------------------------------------------------------------------
 // matrix-free K apply
@@ -561,7 +720,10 @@ void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t*
     const int ncx = (G.rx - cx + 1) / 2, ncy = (G.ry - cy + 1) / 2, ncz = g.dim == 3 ? (G.rz - cz + 1) / 2 : 1;
     const long nce = (long)ncx * ncy * ncz;
     if (nce == 0) continue;
-    if (g.dim == 3 && Dm && g_q2_mfma) {
+    if (g.dim == 3 && g_q2_kernel == 2) {
+      hipLaunchKernelGGL((k_q2_apply_sf<T>), q2_blocks(nce, Q2S_E), 128, 0, s, G, cx, cy, cz, ncx, ncy, nce, W, x, y,
+                         bc);
+    } else if (g.dim == 3 && Dm && g_q2_kernel == 1) {
       const long nbatch = (nce + Q2M_E - 1) / Q2M_E;
       if (sizeof(T) == 4 && nbatch >= 2L * 2048) {
         // persistent blocks (2+ batches each) keep their slice of D in registers

@@ -206,7 +206,9 @@ struct uwg_stokes {
   // learned-correction support
   bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
   int nn_window = 20;            // fallback monitor: over nn_window iterations the residual must
-  double nn_min_drop = 0.1;      // fall by at least this fraction while the network is in use
+  double nn_min_drop = 0.01;     // fall by at least this fraction while the network is in use
+  double exact_vel_rtol = 0.0;   // > 0: the coupled FP64 preconditioner's velocity block is an
+                                 // inner solve to this rtol instead of one V-cycle (diagnostic)
   int nn_mode = 0;               // 0: replace the coarse cycle, 1: add to it
   int capture_level = -1;        // record coarse vectors seen by the cycle
   int capture_kind = 0;          // 0: right-hand side reaching the level, 1: residual left by its cycle
@@ -2125,6 +2127,8 @@ static int solve_schur(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
 
 // Coupled solve: FGMRES on [[K, G],[G^T, 0]] with the block lower-triangular right
 // preconditioner  z_u = V-cycle(r_u), z_p = -approxS^-1 (r_p - G^T z_u).
+static int inner_solve_fp64(uwg_stokes* st, const double* b, double* x, double rtol, int* its);
+
 static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats* S, const double* fu,
                        const double* hp, double* u, double* p) {
   const long n = st->nu, E = st->np, nt = n + E;
@@ -2154,6 +2158,10 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
       vk_d2f(n, 1.0, rr, b32, 0);
       top_vcycle<float>(st, b32, x32);
       vk_f2d(n, 1.0, x32, zz, 0);
+    } else if (st->exact_vel_rtol > 0.0) {
+      int vits = 0;  // K z_u = r_u solved (the iteration floor of the Schur-block choice)
+      inner_solve_fp64(st, rr, zz, st->exact_vel_rtol, &vits);
+      st->vcycles += vits - 1;
     } else {
       double* b64 = top_buf_b<double>(st);
       vk_copy<double>(n, rr, b64, 0);
@@ -2166,8 +2174,10 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   int its = 0;
   double rn;
   // learned-correction safeguard: the FP64 residual must keep falling; if 20 iterations
-  // reduce it by less than 10 %, the learned component is switched off and the plain
-  // preconditioner takes over (a shorter window trips on the plateaus after restarts)
+  // reduce it by less than 1 %, the learned component is switched off and the plain
+  // preconditioner takes over (a shorter window trips on the plateaus after restarts; a
+  // larger required drop trips in the slow tail of tight tolerances, where the plain
+  // preconditioner is slower still)
   // (FGMRES admits a preconditioner that changes between iterations)
   // hist[k] is the relative residual after iteration k (hist[0] = 1, the start), so the test
   // compares the residual with the one nn_window iterations earlier
@@ -2321,6 +2331,8 @@ extern "C" int uwg_stokes_solve(uwg_stokes* st, const uwg_solve_opts* opts, uwg_
   if (o.learned && st->nn && o.precision != 1)
     return arg_error("the learned coarse correction runs inside the FP32 cycle (precision 1)");
   if (o.learned && o.precision == 2) return arg_error("the learned preconditioners are not provided with precision 2");
+  if (st->exact_vel_rtol > 0.0 && (o.method != 1 || o.precision != 0))
+    return arg_error("the solved velocity block is provided for the coupled FP64 solve (method 1, precision 0)");
   API_BEGIN
   double t0 = now();
   if (o.mg_levels > 0 && o.mg_levels != (int)st->L.size()) {
@@ -2397,13 +2409,23 @@ extern "C" int uwg_stokes_apply(uwg_stokes* st, int which, int level, const doub
     // level 0 is the Q2/dPc1 level, level l >= 1 the Q1 level l-1 below it
     if (level < 0 || level > (int)st->L.size()) return arg_error("bad level");
     if (level == 0) {
-      if (which < 0 || which > 3) return arg_error("which must be 0..3");
+      if (which < 0 || which > 4) return arg_error("which must be 0..4");
       API_BEGIN
       const long nin = which == 1 ? st->np : st->nu, nout = which == 2 ? st->np : st->nu;
       double* dx = dalloc<double>(nin);
       double* dy = dalloc<double>(nout);
       UWG_HIP(hipMemcpy(dx, x, sizeof(double) * nin, hipMemcpyHostToDevice));
-      if (which == 0)
+      if (which == 4) {
+        // the FP32 K of the mixed cycle on x rounded to FP32, widened to FP64
+        ensure_fp32(st);
+        float* fx = dalloc<float>(nin);
+        float* fy = dalloc<float>(nout);
+        vk_d2f(nin, 1.0, dx, fx, 0);
+        top_K<float>(st, fx, fy, st->q2->bc);
+        vk_f2d(nout, 1.0, fy, dy, 0);
+        dfree(fx);
+        dfree(fy);
+      } else if (which == 0)
         top_K<double>(st, dx, dy, st->q2->bc);
       else if (which == 1)
         top_G<double>(st, dx, dy, st->q2->bc);
@@ -2536,6 +2558,12 @@ extern "C" int uwg_set_q2_mfma(int on) {
   return 0;
 }
 
+extern "C" int uwg_set_q2_apply_kernel(int kind) {
+  if (kind < 0 || kind > 2) return arg_error("kind must be 0 (vector), 1 (matrix cores) or 2 (sum-factorised)");
+  set_q2_apply_kernel(kind);
+  return 0;
+}
+
 extern "C" int uwg_set_cycle_bf16(int on) {
   g_cycle_bf16 = on != 0;
   return 0;
@@ -2662,6 +2690,12 @@ extern "C" int uwg_stokes_load_nn(uwg_stokes* st, int level, int nparams, const 
 extern "C" int uwg_stokes_set_nn_mode(uwg_stokes* st, int mode) {
   if (mode != 0 && mode != 1) return arg_error("mode must be 0 (replace) or 1 (add)");
   st->nn_mode = mode;
+  return 0;
+}
+
+extern "C" int uwg_stokes_set_velocity_block_solve(uwg_stokes* st, double rtol) {
+  if (!(rtol >= 0.0 && rtol < 1.0)) return arg_error("rtol must be in [0, 1)");
+  st->exact_vel_rtol = rtol;
   return 0;
 }
 

@@ -162,6 +162,7 @@ void launch_q2_apply(const Geom& g, const T* W, const T* x, T* y, const uint8_t*
                      const T* Dm = nullptr);  // Dm: [32][384] derivative matrix -> matrix-core path (3D)
 void q2_mfma_matrix(const Geom& g, std::vector<double>& D);
 void set_q2_mfma(int on);
+void set_q2_apply_kernel(int kind);
 void launch_q2_diag(const Geom& g, const double* W, const uint8_t* bc, double* dinv, hipStream_t s);
 template <typename T>
 void launch_q2_apply_G(const Geom& g, const double* Gr, const T* p, const uint8_t* bc, T* y, hipStream_t s);

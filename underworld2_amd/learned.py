@@ -14,7 +14,7 @@ Training data comes from the plain solver itself: ``collect`` records vectors re
 network's level during ordinary solves, and the level operator's own FGMRES/multigrid solve
 (FP64, tight tolerance) provides the target correction.  Training runs on the GPU through the
 library's forward/backward/Adam kernels (``ConvNet.fit``).  The true FP64 residual of the
-outer FGMRES checks the result: when 20 iterations fail to reduce it by 10 % the solver drops
+outer FGMRES checks the result: when 20 iterations fail to reduce it by 1 % the solver drops
 the network and continues with plain multigrid (``SolveStats.fallback_used``).
 
 The reference has no learned component; this module has no counterpart there.

@@ -413,8 +413,16 @@ class StokesSolver:
             raise ValueError("restart must be an int in [2, 200]")
         self._restart = restart
 
+    def set_velocity_block_solve(self, rtol):
+        """Diagnostic (engine-specific): with ``rtol`` > 0 the coupled FP64 solve applies K^-1 in
+        its block preconditioner by an inner FGMRES + V-cycle solve to ``rtol`` instead of one
+        V-cycle, so the outer count is the floor set by the Schur block (approxS or the learned
+        stencil) alone; 0 restores the V-cycle.  The setting stays on the system."""
+        L.check(L.lib().uwg_stokes_set_velocity_block_solve(self._stokesSLE._h, float(rtol)))
+
     def set_learned(self, model):
-        """Attach a trained learned coarse correction (``underworld2_amd.learned``)."""
+        """Attach a trained learned preconditioner component (``underworld2_amd.learned``): the
+        learned Schur stencil, or the experimental learned coarse correction."""
         if model is None:
             self._learned = False
             return
@@ -431,6 +439,9 @@ class StokesSolver:
         o.inner_rtol = float(self.options.A11.ksp_rtol)
         o.mg_levels = int(self.options.mg.levels)
         o.restart = self._restart if self._restart is not None else (30 if self._method == 0 else 60)
+        max_it = getattr(self.options.scr, "ksp_max_it", None)  # PETSc's option name; default 10,000
+        if max_it is not None:
+            o.max_outer_its = int(max_it)
         o.smooth_degree = self._smooth_degree
         return o
 

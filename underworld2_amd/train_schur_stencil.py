@@ -1,7 +1,7 @@
 """Train the learned Schur stencil on the GPU and save it for ``learned.load_default_schur``.
 
     python -m underworld2_amd.train_schur_stencil [--problems 512] [--res 64] [--inclusions 0.25]
-        [--samples 2] [--channels 16] [--layers 2] [--steps 3000] [--out PATH]
+        [--samples 2] [--channels 16] [--layers 2] [--steps 3000] [--corr-len LO HI] [--out PATH]
 
 Training problems are free-slip unit cubes: a fraction with eta = 10^(c * GRF) (c ~ U(0, 6),
 correlation length ~ U(0.1, 0.3)) and 1-5 buoyant spheres, the rest (``--inclusions``) with 1-5
@@ -32,6 +32,9 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--seed0", type=int, default=5000)
     ap.add_argument("--inner-rtol", type=float, default=1e-6)
+    ap.add_argument("--corr-len", type=float, nargs=2, default=None, metavar=("LO", "HI"),
+                    help="GRF correlation length range (default 0.1 0.3); longer lengths at a coarse training "
+                         "resolution give the element-scale contrast of finer grids")
     ap.add_argument("--out", default=learned.SCHUR_PATH)
     a = ap.parse_args(argv)
 
@@ -46,6 +49,9 @@ def main(argv=None):
         # interleave the two families so the held-out tail holds both
         if n_incl and (k % max(1, int(round(a.problems / n_incl)))) == 0:
             st, v, p, desc = learned.inclusion_problem(a.res, seed)
+        elif a.corr_len is not None:
+            ell = np.random.default_rng(seed + 10**6).uniform(*a.corr_len)
+            st, v, p, desc = learned.random_problem(a.res, seed, corr_len=ell)
         else:
             st, v, p, desc = learned.random_problem(a.res, seed)
         net.add_samples(st, a.samples, seed=seed, inner_rtol=a.inner_rtol)
@@ -68,7 +74,8 @@ def main(argv=None):
         net.get_params(), a.channels, a.layers,
         meta=dict(problems=a.problems, res=a.res, inclusions=n_incl, samples=n, held_out=nval, steps=a.steps,
                   batch=a.batch, heldout_loss=f"{val:.4f}", untrained_loss=f"{base:.4f}",
-                  seeds=f"{a.seed0}-{a.seed0 + a.problems - 1}"))
+                  seeds=f"{a.seed0}-{a.seed0 + a.problems - 1}",
+                  corr_len="0.1-0.3" if a.corr_len is None else f"{a.corr_len[0]}-{a.corr_len[1]}"))
     model.save(a.out)
     log(f"saved {a.out}")
     return 0
