@@ -188,7 +188,8 @@ def _oracle_convection(res, ppc, steps, T0, Ra=1.0e6):
 def test_config_thermal_convection_small():
     """The 3D thermal-convection configuration (Ra 1e6, eta = exp(-ln(1e4) T), Stokes + SUPG +
     RK2 per step) at 8^3 with 8 particles per cell: two FP64 steps against the CPU restatement of
-    the whole loop, then FP64 against mixed precision over eight steps on Nu and v_rms."""
+    the whole loop, then FP64 against mixed precision and against the learned stencil with mixed
+    precision (the configuration's learned mode) over eight steps on Nu and v_rms."""
     import bench
     res, ppc = (8, 8, 8), 8
     print(f"thermal convection {res}, {ppc} ppc: GPU FP64 loop against the oracle", flush=True)
@@ -210,14 +211,17 @@ def test_config_thermal_convection_small():
         # the GPU solve stops at rtol 1e-11 (the oracle solves exactly); T and the particles inherit
         # the velocity's error through dt and the advection
         assert eT < 1e-8 and eu < 1e-8 and ec < 1e-8
-    print("FP64 against mixed precision over 8 steps", flush=True)
+    print("FP64 against mixed precision and against learned + mixed over 8 steps", flush=True)
     runs = {}
-    for m in ("fp64", "mixed"):
+    for m in ("fp64", "mixed", "learned"):
         C = bench.build_convection(res, ppc)
         runs[m] = bench.run_convection(C, 8, m, rtol=1e-8)
-    nu = np.array(runs["fp64"]["nu"]), np.array(runs["mixed"]["nu"])
-    vr = np.array(runs["fp64"]["vrms"]), np.array(runs["mixed"]["vrms"])
-    print(f"Nu {nu[0][-1]:.6f} / {nu[1][-1]:.6f}, v_rms {vr[0][-1]:.4f} / {vr[1][-1]:.4f}", flush=True)
-    assert np.max(np.abs(nu[1] - nu[0]) / nu[0]) < 1e-6
-    assert np.max(np.abs(vr[1] - vr[0]) / vr[0]) < 1e-6
-    assert nu[0][-1] > 1.0 and vr[0][-1] > 0.0
+        assert runs[m] is not None
+    nu0, vr0 = np.array(runs["fp64"]["nu"]), np.array(runs["fp64"]["vrms"])
+    for m in ("mixed", "learned"):
+        nu, vr = np.array(runs[m]["nu"]), np.array(runs[m]["vrms"])
+        print(f"{m}: Nu {nu0[-1]:.6f} / {nu[-1]:.6f}, v_rms {vr0[-1]:.4f} / {vr[-1]:.4f}, Stokes iterations "
+              f"{sum(runs['fp64']['its'])} / {sum(runs[m]['its'])}", flush=True)
+        assert np.max(np.abs(nu - nu0) / nu0) < 1e-6
+        assert np.max(np.abs(vr - vr0) / vr0) < 1e-6
+    assert nu0[-1] > 1.0 and vr0[-1] > 0.0

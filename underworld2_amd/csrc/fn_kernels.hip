@@ -62,13 +62,13 @@ constexpr int FN_MAXREG = 48;
 
 template <int DIM>
 __device__ bool field_at(const FnGeom& g, const double* f, int nc, const double* x, double* v) {
-  long el;
-  if (!search_element<DIM>(g.res, g.lo, g.hi, x, &el)) return false;
+  int ijk[3];
+  if (!search_element_ijk<DIM>(g.res, g.lo, g.hi, x, ijk)) return false;
   double verts[24], xi[3], N[8];
-  element_verts<DIM>(g.res, g.lo, g.hi, el, verts);
+  element_verts_ijk<DIM>(g.res, g.lo, g.hi, ijk, verts);
   newton_local<DIM>(verts, x, xi);
   shape<DIM>(xi, N);
-  long b = element_base_node<DIM>(g.res, el);
+  long b = element_base_node_ijk<DIM>(g.res, ijk);
   long nx = g.res[0] + 1, nxy = nx * (g.res[1] + 1);
   for (int c = 0; c < nc; c++) v[c] = 0.0;
   for (int a = 0; a < (DIM == 3 ? 8 : 4); a++) {

@@ -70,40 +70,62 @@ __global__ void k_newcnt(long E, const int* ptr, const int* ca, const int* cr, i
     tc[e] = old + ca[e];
   }
 }
-// one thread per cell: replay removals/additions in particle-index order
-__global__ void k_replay(long E, const int* ptr, const int* pid, const int* off_a, const int* cnt_a, const int* val_a,
-                         const int* off_r, const int* cnt_r, const int* val_r, const int* tmp_ptr, int* tmp,
-                         const int* new_ptr, int* new_pid) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  int s0 = ptr[e], n = ptr[e + 1] - s0;
-  int na = cnt_a[e], nr = cnt_r[e];
-  int* out = new_pid + new_ptr[e];
-  if (na == 0 && nr == 0) {
-    for (int k = 0; k < n; k++) out[k] = pid[s0 + k];
-    return;
+// Per cell, replay removals/additions in particle-index order.  Eight lanes per cell copy the
+// cell's list in and out (consecutive lanes, consecutive entries); the edits themselves are
+// sequential (a removal moves the list's last entry into the hole) and run on the cell's first
+// lane between two workgroup barriers, which also make the lanes' global writes of the list
+// visible to each other.
+constexpr int kReplayLanes = 8, kReplayBlock = 256;
+__global__ __launch_bounds__(kReplayBlock) void k_replay(long E, const int* ptr, const int* pid, const int* off_a,
+                                                         const int* cnt_a, const int* val_a, const int* off_r,
+                                                         const int* cnt_r, const int* val_r, const int* tmp_ptr,
+                                                         int* tmp, const int* new_ptr, int* new_pid) {
+  __shared__ int nfin[kReplayBlock / kReplayLanes];
+  const int lane = threadIdx.x & (kReplayLanes - 1), c = threadIdx.x / kReplayLanes;
+  const long e = ((long)blockIdx.x * blockDim.x + threadIdx.x) / kReplayLanes;
+  const bool live = e < E;
+  int s0 = 0, n = 0, na = 0, nr = 0;
+  int* out = nullptr;
+  int* L = nullptr;
+  if (live) {
+    s0 = ptr[e];
+    n = ptr[e + 1] - s0;
+    na = cnt_a[e];
+    nr = cnt_r[e];
+    out = new_pid + new_ptr[e];
+    L = tmp + tmp_ptr[e];
   }
-  int* L = tmp + tmp_ptr[e];
-  for (int k = 0; k < n; k++) L[k] = pid[s0 + k];
-  const int* A = val_a + off_a[e];
-  const int* R = val_r + off_r[e];
-  int ia = 0, ir = 0;
-  while (ia < na || ir < nr) {
-    bool take_r = ir < nr && (ia >= na || R[ir] < A[ia]);
-    if (take_r) {
-      int p = R[ir++];
-      int k = 0;
-      while (k < n && L[k] != p) k++;
-      if (k < n) {
-        L[k] = L[n - 1];
-        n--;
+  const bool edit = live && (na != 0 || nr != 0);
+  if (live) {
+    int* dst = edit ? L : out;
+    for (int k = lane; k < n; k += kReplayLanes) dst[k] = pid[s0 + k];
+  }
+  __syncthreads();
+  if (edit && lane == 0) {
+    const int* A = val_a + off_a[e];
+    const int* R = val_r + off_r[e];
+    int ia = 0, ir = 0, m = n;
+    while (ia < na || ir < nr) {
+      bool take_r = ir < nr && (ia >= na || R[ir] < A[ia]);
+      if (take_r) {
+        int p = R[ir++];
+        int k = 0;
+        while (k < m && L[k] != p) k++;
+        if (k < m) {
+          L[k] = L[m - 1];
+          m--;
+        }
+      } else {
+        L[m++] = A[ia++];
       }
-    } else {
-      L[n++] = A[ia++];
     }
+    nfin[c] = m;
   }
-  const int nmax = new_ptr[e + 1] - new_ptr[e];
-  for (int k = 0; k < n && k < nmax; k++) out[k] = L[k];
+  __syncthreads();
+  if (edit) {
+    const int m = nfin[c], nmax = new_ptr[e + 1] - new_ptr[e];
+    for (int k = lane; k < m && k < nmax; k += kReplayLanes) out[k] = L[k];
+  }
 }
 __global__ void k_count_outside(long P, const int* owner, long E, int* out) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,7 +214,8 @@ long relocate(const Geom& g, long P, const double* coords, int* owner, int* cell
   UWG_HIP(hipcub::DeviceScan::ExclusiveSum(w->cub_tmp, bytes, w->new_cnt, w->new_ptr, (int)(E + 1), s));
   bytes = w->cub_bytes;
   UWG_HIP(hipcub::DeviceScan::ExclusiveSum(w->cub_tmp, bytes, w->tmp_cnt, w->tmp_ptr, (int)(E + 1), s));
-  hipLaunchKernelGGL(k_replay, nb(E, 128), 128, 0, s, E, cell_ptr, cell_pid, w->off_a, w->cnt_a, w->val_a_s, w->off_r,
+  hipLaunchKernelGGL(k_replay, nb(E * kReplayLanes, kReplayBlock), kReplayBlock, 0, s, E, cell_ptr, cell_pid, w->off_a,
+                     w->cnt_a, w->val_a_s, w->off_r,
                      w->cnt_r, w->val_r_s, w->tmp_ptr, w->tmp, w->new_ptr, w->new_pid);
   UWG_HIP(hipGetLastError());
   UWG_HIP(hipMemcpyAsync(cell_ptr, w->new_ptr, sizeof(int) * (E + 1), hipMemcpyDeviceToDevice, s));

@@ -655,13 +655,13 @@ void launch_dvc_list(int dim, int num, long E0, long nE, const int* elist, const
 // ---------------------------------------------------------------------------------------
 template <int DIM>
 __device__ bool velocity_at(const GeomArgs& g, const double* vel, const double* x, double* v) {
-  long el;
-  if (!search_element<DIM>(g.res, g.lo, g.hi, x, &el)) return false;
+  int ijk[3];
+  if (!search_element_ijk<DIM>(g.res, g.lo, g.hi, x, ijk)) return false;
   double verts[24], xi[3], N[8];
-  element_verts<DIM>(g.res, g.lo, g.hi, el, verts);
+  element_verts_ijk<DIM>(g.res, g.lo, g.hi, ijk, verts);
   newton_local<DIM>(verts, x, xi);
   shape<DIM>(xi, N);
-  long b = element_base_node<DIM>(g.res, el);
+  long b = element_base_node_ijk<DIM>(g.res, ijk);
   long nx = g.res[0] + 1, nxy = nx * (g.res[1] + 1);
   for (int d = 0; d < DIM; d++) v[d] = 0.0;
   for (int a = 0; a < (DIM == 3 ? 8 : 4); a++) {
@@ -679,14 +679,14 @@ __device__ bool velocity_at(const GeomArgs& g, const double* vel, const double* 
 // incidence order.
 template <int DIM>
 __device__ bool velocity_at_q2(const GeomArgs& g, const double* vel, const double* x, double* v) {
-  long el;
-  if (!search_element<DIM>(g.res, g.lo, g.hi, x, &el)) return false;
+  int ijk[3];
+  if (!search_element_ijk<DIM>(g.res, g.lo, g.hi, x, ijk)) return false;
   constexpr int NN = DIM == 3 ? 27 : 9;
   double verts[NN * DIM], xi[3], N[NN];
-  q2_element_verts<DIM>(g.res, g.lo, g.hi, el, verts);
+  q2_element_verts_ijk<DIM>(g.res, g.lo, g.hi, ijk, verts);
   q2_newton_local<DIM>(verts, x, xi);
   q2_basis<DIM>(xi, N);
-  const long i = el % g.res[0], j = (el / g.res[0]) % g.res[1], k = DIM == 3 ? el / ((long)g.res[0] * g.res[1]) : 0;
+  const long i = ijk[0], j = ijk[1], k = ijk[2];
   const long nx = 2L * g.res[0] + 1, ny = 2L * g.res[1] + 1;
   for (int d = 0; d < DIM; d++) v[d] = 0.0;
   for (int a = 0; a < NN; a++) {
@@ -707,7 +707,8 @@ __device__ __forceinline__ bool velocity_lookup(const GeomArgs& g, const double*
 }
 
 template <int DIM, int ORDER>
-__global__ void k_advect_rk2(GeomArgs g, long P, const double* vel, double* coords, double dt, int* counts) {
+__global__ __launch_bounds__(256, 2) void k_advect_rk2(GeomArgs g, long P, const double* vel, double* coords, double dt,
+                                                    int* counts) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   double x[3], start[3], v0[3], v1[3];

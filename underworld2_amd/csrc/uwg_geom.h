@@ -18,12 +18,22 @@ UWG_HD double node_coord_1d(int i, int nnodes, double lo, double hi) {
   return (lo + 0.0) + ((double)i / (double)(nnodes - 1)) * (steps - 0.0 * 0.0);
 }
 
-// Element-local vertex coordinates, local order x fastest (TrilinearElementType.c:216-233)
+// Element (i, j, k) of cell index e (x fastest).  Cell indices are below 2^31 (owners are
+// int), so the decomposition uses 32-bit unsigned division.
 template <int DIM>
-UWG_HD void element_verts(const int* res, const double* lo, const double* hi, long e, double* v) {
-  long i = e % res[0];
-  long j = (e / res[0]) % res[1];
-  long k = DIM == 3 ? e / ((long)res[0] * res[1]) : 0;
+UWG_HD void element_ijk(const int* res, long e, int* ijk) {
+  const unsigned u = (unsigned)e, rx = (unsigned)res[0], ry = (unsigned)res[1];
+  const unsigned q = u / rx;
+  ijk[0] = (int)(u - q * rx);
+  ijk[1] = (int)(q % ry);
+  ijk[2] = DIM == 3 ? (int)(q / ry) : 0;
+}
+
+// Element-local vertex coordinates of element (i, j, k), local order x fastest
+// (TrilinearElementType.c:216-233)
+template <int DIM>
+UWG_HD void element_verts_ijk(const int* res, const double* lo, const double* hi, const int* ijk, double* v) {
+  const int i = ijk[0], j = ijk[1], k = ijk[2];
   double X[2] = {node_coord_1d((int)i, res[0] + 1, lo[0], hi[0]), node_coord_1d((int)i + 1, res[0] + 1, lo[0], hi[0])};
   double Y[2] = {node_coord_1d((int)j, res[1] + 1, lo[1], hi[1]), node_coord_1d((int)j + 1, res[1] + 1, lo[1], hi[1])};
   if (DIM == 2) {
@@ -42,12 +52,23 @@ UWG_HD void element_verts(const int* res, const double* lo, const double* hi, lo
 }
 
 template <int DIM>
+UWG_HD void element_verts(const int* res, const double* lo, const double* hi, long e, double* v) {
+  int ijk[3];
+  element_ijk<DIM>(res, e, ijk);
+  element_verts_ijk<DIM>(res, lo, hi, ijk, v);
+}
+
+template <int DIM>
+UWG_HD long element_base_node_ijk(const int* res, const int* ijk) {
+  const long nx = res[0] + 1, ny = res[1] + 1;
+  return ijk[0] + nx * (ijk[1] + ny * (long)ijk[2]);
+}
+
+template <int DIM>
 UWG_HD long element_base_node(const int* res, long e) {
-  long nx = res[0] + 1, ny = res[1] + 1;
-  long i = e % res[0];
-  long j = (e / res[0]) % res[1];
-  long k = DIM == 3 ? e / ((long)res[0] * res[1]) : 0;
-  return i + nx * (j + ny * k);
+  int ijk[3];
+  element_ijk<DIM>(res, e, ijk);
+  return element_base_node_ijk<DIM>(res, ijk);
 }
 
 // Shape functions exactly as written in Bilinear/TrilinearElementType.c
@@ -182,10 +203,8 @@ UWG_HD void newton_local(const double* v, const double* x, double* xi) {
 // (2 res + 1 vertices per direction), the C2Generator incidence order.
 // ---------------------------------------------------------------------------------------
 template <int DIM>
-UWG_HD void q2_element_verts(const int* res, const double* lo, const double* hi, long e, double* v) {
-  const long i = e % res[0];
-  const long j = (e / res[0]) % res[1];
-  const long k = DIM == 3 ? e / ((long)res[0] * res[1]) : 0;
+UWG_HD void q2_element_verts_ijk(const int* res, const double* lo, const double* hi, const int* ijk, double* v) {
+  const long i = ijk[0], j = ijk[1], k = ijk[2];
   double X[3], Y[3], Z[3];
   for (int t = 0; t < 3; t++) {
     X[t] = node_coord_1d((int)(2 * i + t), 2 * res[0] + 1, lo[0], hi[0]);
@@ -197,6 +216,13 @@ UWG_HD void q2_element_verts(const int* res, const double* lo, const double* hi,
     v[a * DIM + 1] = Y[(a / 3) % 3];
     if (DIM == 3) v[a * DIM + 2] = Z[a / 9];
   }
+}
+
+template <int DIM>
+UWG_HD void q2_element_verts(const int* res, const double* lo, const double* hi, long e, double* v) {
+  int ijk[3];
+  element_ijk<DIM>(res, e, ijk);
+  q2_element_verts_ijk<DIM>(res, lo, hi, ijk, v);
 }
 
 // Shape functions with the factorisation and evaluation order of Biquadratic_EvalBasis
@@ -352,9 +378,11 @@ UWG_HD bool num_approx(double var, double val) {
 // Mesh_RegularAlgorithms_SearchElements, LU/StgDomain/Mesh/src/Mesh_RegularAlgorithms.c:174-206
 // sep = (max-min)/nel as computed once in Mesh_RegularAlgorithms_Update (:153-154).
 // The final clamp only guards memory; the reference has no element past nel-1.
+// The element's (i, j, k) instead of its index: search_element_ijk.  out >= 0 here, so the
+// fractional part modf would return is out - trunc(out), exactly.
 template <int DIM>
-UWG_HD bool search_element(const int* res, const double* lo, const double* hi, const double* x, long* el) {
-  long ind[3] = {0, 0, 0};
+UWG_HD bool search_element_ijk(const int* res, const double* lo, const double* hi, const double* x, int* ind) {
+  ind[0] = ind[1] = ind[2] = 0;
   for (int d = 0; d < DIM; d++) {
     double sep = (hi[d] - lo[d]) / res[d];
     if (num_approx(x[d] - hi[d], 0.0)) {
@@ -362,13 +390,22 @@ UWG_HD bool search_element(const int* res, const double* lo, const double* hi, c
     } else if (x[d] < lo[d] || x[d] > hi[d]) {
       return false;
     } else {
-      double out = (x[d] - lo[d]) / sep, integer;
-      double frac = modf(out, &integer);
-      ind[d] = (long)(unsigned)integer;
-      if (ind[d] > 0 && num_approx(frac, 0.0)) ind[d]--;
-      if (ind[d] > res[d] - 1) ind[d] = res[d] - 1;
+      const double out = (x[d] - lo[d]) / sep;
+      const double integer = trunc(out);
+      const double frac = out - integer;
+      long id = (long)(unsigned)integer;
+      if (id > 0 && num_approx(frac, 0.0)) id--;
+      if (id > res[d] - 1) id = res[d] - 1;
+      ind[d] = (int)id;
     }
   }
+  return true;
+}
+
+template <int DIM>
+UWG_HD bool search_element(const int* res, const double* lo, const double* hi, const double* x, long* el) {
+  int ind[3];
+  if (!search_element_ijk<DIM>(res, lo, hi, x, ind)) return false;
   *el = ind[0] + (long)res[0] * (ind[1] + (long)res[1] * ind[2]);
   return true;
 }

@@ -84,7 +84,8 @@ struct uwg_swarm {
   int* slot_cell = nullptr;   // [cap]
   bool has_mat = false;
   bool has_integration = false;
-  long state = 0;             // bumps whenever particles move
+  long state = 0;             // bumps whenever particles or their materials change (assembly cache)
+  long geom_state = 0;        // bumps whenever particles move (integration-point cache)
   long integration_state = -1;
   RelocWork* reloc = nullptr;
   long reloc_P = -1;
@@ -504,6 +505,7 @@ static void populate_tables(uwg_swarm* s, int ppc) {
   UWG_HIP(hipMemcpy(s->owner, ids.data(), sizeof(int) * P, hipMemcpyHostToDevice));
   UWG_HIP(hipMemset(s->mat, 0, sizeof(int) * P));
   s->state++;
+  s->geom_state++;
 }
 
 extern "C" int uwg_swarm_add_particles(uwg_swarm* s, long n, const double* coords) {
@@ -521,6 +523,7 @@ extern "C" int uwg_swarm_add_particles(uwg_swarm* s, long n, const double* coord
   ensure_reloc(s);
   long outside = relocate(g, s->P, s->coords, s->owner, s->cell_ptr, s->cell_pid, s->reloc, 0);
   s->state++;
+  s->geom_state++;
   if (outside > 0) {
     set_error("particles outside the mesh were added");
     return UWG_ERR_DOMAIN;
@@ -546,6 +549,7 @@ extern "C" int uwg_swarm_set_coords(uwg_swarm* s, const double* in, long* outsid
   long o = relocate(s->mesh->g, s->P, s->coords, s->owner, s->cell_ptr, s->cell_pid, s->reloc, 0);
   if (outside) *outside = o;
   s->state++;
+  s->geom_state++;
   return o > 0 ? UWG_ERR_DOMAIN : 0;
   API_END
 }
@@ -576,6 +580,7 @@ extern "C" int uwg_swarm_update_owners(uwg_swarm* s, long* outside) {
   long o = relocate(s->mesh->g, s->P, s->coords, s->owner, s->cell_ptr, s->cell_pid, s->reloc, 0);
   if (outside) *outside = o;
   s->state++;
+  s->geom_state++;
   if (o > 0) {
     set_error("Particles appear to have left the domain, but swarm flag `particleEscape` is False.");
     return UWG_ERR_DOMAIN;
@@ -689,7 +694,7 @@ extern "C" int uwg_swarm_integration_points(uwg_swarm* s, int use_dvc, int dvc_r
   }
   UWG_HIP(hipDeviceSynchronize());
   s->has_integration = true;
-  s->integration_state = s->state;
+  s->integration_state = s->geom_state;
   return 0;
   API_END
 }
@@ -708,7 +713,7 @@ extern "C" int uwg_swarm_set_integration(uwg_swarm* s, const double* xi_slot, co
   UWG_HIP(hipMemcpy(s->xi_slot, xi_slot, sizeof(double) * s->P * dim, hipMemcpyHostToDevice));
   UWG_HIP(hipMemcpy(s->w_slot, w_slot, sizeof(double) * s->P, hipMemcpyHostToDevice));
   s->has_integration = true;
-  s->integration_state = s->state;
+  s->integration_state = s->geom_state;
   return 0;
   API_END
 }
@@ -727,6 +732,7 @@ extern "C" int uwg_advect_rk2(uwg_swarm* s, uwg_field* vel, double dt, int updat
   UWG_HIP(hipMemcpy(h, cnt, sizeof(int) * 2, hipMemcpyDeviceToHost));
   dfree(cnt);
   s->state++;
+  s->geom_state++;
   if (h[0] > 0) {
     set_error("Unable to determine valid velocity for " + std::to_string(h[0]) +
               " particle(s): coordinate outside the global domain");
@@ -1915,7 +1921,7 @@ extern "C" int uwg_stokes_assemble(uwg_stokes* st, uwg_swarm* s) {
   if (st->mode == 0 && !s->has_mat && st->nmat > 1) return arg_error("swarm has no material index");
   if (st->mode == 1 && st->np_vals != s->P) return arg_error("particle values do not match the swarm size");
   API_BEGIN
-  if (!s->has_integration || s->integration_state != s->state) {
+  if (!s->has_integration || s->integration_state != s->geom_state) {
     int rc = uwg_swarm_integration_points(s, 1, 15);
     if (rc) return rc;
   }
@@ -3206,6 +3212,7 @@ extern "C" int uwg_swarm_population_control(uwg_swarm* s, double delete_threshol
   s->P = P;
   for (int v = 0; v < 8; v++) s->var_rows[v] = 0;  // swarm variables must be uploaded again
   s->state++;
+  s->geom_state++;
   s->has_integration = false;
   return 0;
   API_END
