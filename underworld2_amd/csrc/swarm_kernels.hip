@@ -421,8 +421,10 @@ __global__ __launch_bounds__(64) void k_dvc_wave(int num_rt, long E, const int* 
                                                  double* __restrict__ xi_slot, double* __restrict__ w_slot,
                                                  short* __restrict__ gbound, int cap, int* __restrict__ flag) {
   const int num = NUM > 0 ? NUM : num_rt;  // compile-time grid size for the default resolution
-  __shared__ unsigned char own[DVCW_MAXNC];
-  __shared__ unsigned int mark[DVCW_MAXNC / 32];
+  constexpr int NC = NUM > 0 ? (DIM == 3 ? NUM * NUM * NUM : NUM * NUM) : DVCW_MAXNC;  // claim grid cells
+  constexpr int NCW = (NC + 31) / 32;                                                 // mark words
+  __shared__ unsigned char own[NC];
+  __shared__ unsigned int mark[NCW];
   __shared__ double ppos[DVCW_MAXNP * DIM];
   __shared__ int nbound[DVCW_MAXNP];
   __shared__ unsigned char which[DVCW_MAXNP];
@@ -442,20 +444,19 @@ __global__ __launch_bounds__(64) void k_dvc_wave(int num_rt, long E, const int* 
     double Z = -1.0 - dz / 2.0;
     for (int k = 0; k < nz; k++) { Z = Z + dz; cen[2][k] = Z; }
   }
-  for (int q = lane; q < DVCW_MAXNC / 32; q += 64) mark[q] = 0u;
+  for (int q = lane; q < NCW; q += 64) mark[q] = 0u;
   // chains: buffer h (0/1) of particle p at gb + (2 p + h) * cap
   short* gb = gbound + (long)blockIdx.x * 2 * DVCW_MAXNP * cap;
-  // neighbour k of cell c in the order S, N, E, W, U, D; -2 outside the grid
+  // neighbour k of cell c in the order S, N, E, W, U, D; -2 outside the grid.  Selects, not a
+  // switch: k differs between lanes, and a branch per case costs scalar exec-mask work per lane
+  // group
   auto neighbour = [&](int c, int k) -> int {
     const int i = c % nx, j = (c / nx) % ny, kk = c / nxy;
-    switch (k) {
-      case 0: return j > 0 ? c - nx : -2;
-      case 1: return j < ny - 1 ? c + nx : -2;
-      case 2: return i < nx - 1 ? c + 1 : -2;
-      case 3: return i > 0 ? c - 1 : -2;
-      case 4: return kk < nz - 1 ? c + nxy : -2;
-      default: return kk > 0 ? c - nxy : -2;
-    }
+    const int off = k == 0 ? -nx : (k == 1 ? nx : (k == 2 ? 1 : (k == 3 ? -1 : (k == 4 ? nxy : -nxy))));
+    const bool ok = k == 0 ? j > 0
+                           : (k == 1 ? j < ny - 1
+                                     : (k == 2 ? i < nx - 1 : (k == 3 ? i > 0 : (k == 4 ? kk < nz - 1 : kk > 0))));
+    return ok ? c + off : -2;
   };
   for (long e = blockIdx.x; e < E; e += gridDim.x) {
     const int s0 = cell_ptr[e];
@@ -536,23 +537,18 @@ __global__ __launch_bounds__(64) void k_dvc_wave(int num_rt, long E, const int* 
         int ncl = 0;
         for (int base = 0; base < nb; base += 64) {
           const int q = base + lane;
-          bool take = false;
-          int c0 = 0;
-          if (q < nb) {
-            c0 = b[q];
-            const int o = own[c0];
-            if (o == 0xFF) {
-              take = true;
-            } else if (o != pi) {
-              const int ci = c0 % nx, cj = (c0 / nx) % ny, ck = c0 / nxy;
-              const double x0 = cen[0][ci], y0 = cen[1][cj], z0 = cen[2][ck];
-              const double* p1 = ppos + o * DIM;
-              const double* p2 = ppos + pi * DIM;
-              double r = (p1[0] + p2[0] - x0 - x0) * (p1[0] - p2[0]) + (p1[1] + p2[1] - y0 - y0) * (p1[1] - p2[1]);
-              if (DIM == 3) r = r + (p1[2] + p2[2] - z0 - z0) * (p1[2] - p2[2]);
-              take = r > 0.0;
-            }
-          }
+          // every lane evaluates the claim rule (lanes past the chain on its last entry, unowned
+          // cells against the particle itself) and the predicate selects the result
+          const int c0 = b[q < nb ? q : nb - 1];
+          const int o = own[c0];
+          const int oo = o == 0xFF ? pi : o;
+          const int ci = c0 % nx, cj = (c0 / nx) % ny, ck = c0 / nxy;
+          const double x0 = cen[0][ci], y0 = cen[1][cj], z0 = cen[2][ck];
+          const double* p1 = ppos + oo * DIM;
+          const double* p2 = ppos + pi * DIM;
+          double r = (p1[0] + p2[0] - x0 - x0) * (p1[0] - p2[0]) + (p1[1] + p2[1] - y0 - y0) * (p1[1] - p2[1]);
+          if (DIM == 3) r = r + (p1[2] + p2[2] - z0 - z0) * (p1[2] - p2[2]);
+          const bool take = q < nb && (o == 0xFF || (o != pi && r > 0.0));
           const unsigned long long m = __ballot(take);
           const int rank = __popcll(m & lt);
           if (take) {
@@ -574,24 +570,63 @@ __global__ __launch_bounds__(64) void k_dvc_wave(int num_rt, long E, const int* 
     }
     if (overflow) {
       // leave LDS marks clean for the next element
-      for (int q = lane; q < DVCW_MAXNC / 32; q += 64) mark[q] = 0u;
+      for (int q = lane; q < NCW; q += 64) mark[q] = 0u;
       if (lane == 0) flag[e] = 1;
       __syncthreads();
       continue;
     }
-    // centroids and weights, grid-index order (_DVCWeights_GetCentroids3D :198-226)
+    // centroids and weights, grid-index order (_DVCWeights_GetCentroids3D :198-226).  A
+    // particle's cells lie inside their bounding box, so its lane scans only the box, in grid
+    // order: the same cells in the same order as a scan of the whole grid, hence the same sums.
+    // The boxes come from one pass over the grid rows (a lane per row, one atomic min/max per
+    // run of equal owners); they live where the coordinates were (ppos is not read again for
+    // this element).
+    int* bb = reinterpret_cast<int*>(ppos);  // [2 DIM][DVCW_MAXNP]: lower i, j, k, then upper
+    for (int p = lane; p < np; p += 64) {
+      for (int d = 0; d < DIM; d++) {
+        bb[d * DVCW_MAXNP + p] = 1 << 20;
+        bb[(DIM + d) * DVCW_MAXNP + p] = -1;
+      }
+    }
+    __syncthreads();
+    for (int row = lane; row < ny * nz; row += 64) {
+      const int j = row % ny, k = row / ny;
+      int cur = 0xFF, i0 = 0;
+      for (int i = 0; i <= nx; i++) {
+        const int o = i < nx ? (int)own[i + nx * row] : -1;
+        if (o != cur) {
+          if (cur != 0xFF) {
+            atomicMin(&bb[0 * DVCW_MAXNP + cur], i0);
+            atomicMax(&bb[DIM * DVCW_MAXNP + cur], i - 1);
+            atomicMin(&bb[1 * DVCW_MAXNP + cur], j);
+            atomicMax(&bb[(DIM + 1) * DVCW_MAXNP + cur], j);
+            if (DIM == 3) {
+              atomicMin(&bb[2 * DVCW_MAXNP + cur], k);
+              atomicMax(&bb[(DIM + 2) * DVCW_MAXNP + cur], k);
+            }
+          }
+          cur = o;
+          i0 = i;
+        }
+      }
+    }
+    __syncthreads();
     const double da = DIM == 3 ? dx * dy * dz : dx * dy;
     for (int p = lane; p < np; p += 64) {
       double sx = 0.0, sy = 0.0, sz = 0.0;
       int cnt = 0;
-      for (int c = 0; c < ncell; c++) {
-        if (own[c] != p) continue;
-        const int ci = c % nx, cj = (c / nx) % ny, ck = c / nxy;
-        sx += cen[0][ci];
-        sy += cen[1][cj];
-        if (DIM == 3) sz += cen[2][ck];
-        cnt++;
-      }
+      const int i0 = bb[0 * DVCW_MAXNP + p], i1 = bb[DIM * DVCW_MAXNP + p];
+      const int j0 = bb[1 * DVCW_MAXNP + p], j1 = bb[(DIM + 1) * DVCW_MAXNP + p];
+      const int k0 = DIM == 3 ? bb[2 * DVCW_MAXNP + p] : 0, k1 = DIM == 3 ? bb[(DIM + 2) * DVCW_MAXNP + p] : 0;
+      for (int ck = k0; ck <= k1; ck++)
+        for (int cj = j0; cj <= j1; cj++)
+          for (int ci = i0; ci <= i1; ci++) {
+            if (own[ci + nx * cj + nxy * ck] != p) continue;
+            sx += cen[0][ci];
+            sy += cen[1][cj];
+            if (DIM == 3) sz += cen[2][ck];
+            cnt++;
+          }
       double* out = xi_slot + (long)(s0 + p) * DIM;
       w_slot[s0 + p] = cnt * da;
       if (cnt != 0) {
@@ -629,6 +664,7 @@ void launch_dvc_wave(int dim, int num, long E, const int* cell_ptr, double* xi_s
 }
 
 long dvc_wave_bound_shorts(int cap) { return 2L * DVCW_MAXNP * cap; }
+
 
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap) {
   int ncell = num * num * (dim == 3 ? num : 1);
