@@ -134,10 +134,17 @@ def reference_solution(P, rtol=1e-11, restart=200, max_its=6000):
     s.set_restart(restart)
     s.options.scr.ksp_max_it = max_its
     s.set_learned(model)
+    import underworld2_amd._lib as L
+    # no fallback for the reference: the stencil keeps converging in the slow tail, where the
+    # restarts' true-residual steps can look like a stall to the monitor
+    L.check(L.lib().uwg_stokes_set_nn_monitor(P["stokes"]._h, 1 << 30, 0.01), "monitor")
     t0 = time.perf_counter()
-    with warnings.catch_warnings():
-        warnings.simplefilter("ignore")
-        s.solve()
+    try:
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            s.solve()
+    finally:
+        L.check(L.lib().uwg_stokes_set_nn_monitor(P["stokes"]._h, 20, 0.01), "monitor")
     t = time.perf_counter() - t0
     st = s.get_stats()
     rel = float(np.hypot(st.res_momentum, st.res_continuity) / st.norm_f)
