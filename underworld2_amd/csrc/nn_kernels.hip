@@ -1069,65 +1069,10 @@ static void snn_ensure(SchurNet* m, int B, long N, bool train) {
   m->ncap = N;
 }
 
-// 3x3x3 convolution for the stencil network's grids: one node per thread, COB output channels
-// per block row.  The node's 27 neighbour offsets (and which of them lie inside the grid) are
-// formed once; each input channel's plane is then a wave-uniform base, so a tap is one load at a
-// 32-bit offset from it and COB multiply-adds with wave-uniform (scalar) weights.  Same
-// accumulation order as k_conv_fwd_blk.  Requires 4 * N < 2^32.
-template <int COB>
-__global__ __launch_bounds__(256) void k_conv3_fwd_tap(int B, int Ci, int Co, Grid3 g, const float* __restrict__ in,
-                                                       const float* __restrict__ w, const float* __restrict__ bias,
-                                                       float* __restrict__ out, int relu) {
-  const long N = g.N();
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int co0 = blockIdx.y * COB;
-  if (t >= (long)B * N) return;
-  const long p = t % N;
-  const int b = (int)(t / N);
-  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
-  uint32_t boff[27];
-  uint32_t inside = 0u;
-#pragma unroll
-  for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-      for (int dx = -1; dx <= 1; dx++) {
-        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-        const bool ok = k + dz >= 0 && k + dz < g.nz && j + dy >= 0 && j + dy < g.ny && i + dx >= 0 && i + dx < g.nx;
-        const long q = ok ? p + dx + (long)g.nx * (dy + (long)g.ny * dz) : p;
-        boff[tap] = (uint32_t)q * 4u;
-        inside |= (ok ? 1u : 0u) << tap;
-      }
-  float acc[COB];
-#pragma unroll
-  for (int c = 0; c < COB; c++) acc[c] = bias[co0 + c];
-  const long cs = (long)Ci * 27;
-  for (int ci = 0; ci < Ci; ci++) {
-    const char* plane = reinterpret_cast<const char*>(in + ((long)b * Ci + ci) * N);
-    const float* wc = w + (long)co0 * cs + ci * 27;
-#pragma unroll
-    for (int tap = 0; tap < 27; tap++) {
-      const float raw = *reinterpret_cast<const float*>(plane + boff[tap]);
-      const float v = ((inside >> tap) & 1u) ? raw : 0.0f;
-#pragma unroll
-      for (int c = 0; c < COB; c++) acc[c] += wc[c * cs + tap] * v;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < COB; c++) {
-    const float a = acc[c];
-    out[((long)b * Co + co0 + c) * N + p] = relu ? fmaxf(a, 0.0f) : a;
-  }
-}
-
 static void conv3_fwd(int B, int ci, int co, Grid3 g, const float* in, const float* w, const float* bias, float* out,
                       bool relu, hipStream_t s) {
   const long nb = (long)B * g.N();
-  if (co % 16 == 0 && 4L * g.N() < (1L << 32))
-    hipLaunchKernelGGL(k_conv3_fwd_tap<16>, dim3(nbk(nb, 256), co / 16), 256, 0, s, B, ci, co, g, in, w, bias, out,
-                       relu ? 1 : 0);
-  else if (co % 16 == 0)
+  if (co % 16 == 0)
     hipLaunchKernelGGL(k_conv_fwd_blk<16>, dim3(nbk(nb, 256), co / 16), 256, 0, s, B, ci, co, g, in, w, bias, out,
                        relu ? 1 : 0);
   else if (co % 8 == 0)

@@ -665,7 +665,6 @@ void launch_dvc_wave(int dim, int num, long E, const int* cell_ptr, double* xi_s
 
 long dvc_wave_bound_shorts(int cap) { return 2L * DVCW_MAXNP * cap; }
 
-
 long dvc_scratch_bytes(int dim, int num, int max_np, int cap) {
   int ncell = num * num * (dim == 3 ? num : 1);
   return dvc_layout(ncell, max_np, cap).bytes;
