@@ -614,8 +614,8 @@ static int run_dvc(uwg_swarm* s, int dvc_res) {
     UWG_HIP(hipGetDevice(&dev));
     UWG_HIP(hipGetDeviceProperties(&prop, dev));
     const int cap = std::min(512, ncell);
-    // 16 workgroups per CU: measured faster than filling the CU's LDS (19 fit; 559 against 684 ms
-    // per pass at 128^3)
+    // 16 workgroups per CU (19 fit in LDS; measured per pass at 128^3: 12 -> 659 ms, 16 -> 559 ms,
+    // 19 -> 684 ms)
     const int nblk = (int)std::min<long>(E, (long)prop.multiProcessorCount * 16);
     short* gb = dalloc<short>(dvc_wave_bound_shorts(cap) * nblk);
     int* flag = dalloc<int>(E);
