@@ -138,6 +138,7 @@ def reference_solution(P, rtol=1e-11, restart=200, max_its=6000):
     # no fallback for the reference: the stencil keeps converging in the slow tail, where the
     # restarts' true-residual steps can look like a stall to the monitor
     L.check(L.lib().uwg_stokes_set_nn_monitor(P["stokes"]._h, 1 << 30, 0.01), "monitor")
+    L.check(L.lib().uwg_stokes_set_nn_restart_rise(P["stokes"]._h, 0.0), "monitor")
     t0 = time.perf_counter()
     try:
         with warnings.catch_warnings():
@@ -145,6 +146,7 @@ def reference_solution(P, rtol=1e-11, restart=200, max_its=6000):
             s.solve()
     finally:
         L.check(L.lib().uwg_stokes_set_nn_monitor(P["stokes"]._h, 20, 0.01), "monitor")
+        L.check(L.lib().uwg_stokes_set_nn_restart_rise(P["stokes"]._h, 1.25), "monitor")
     t = time.perf_counter() - t0
     st = s.get_stats()
     rel = float(np.hypot(st.res_momentum, st.res_continuity) / st.norm_f)

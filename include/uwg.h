@@ -201,6 +201,11 @@ int uwg_stokes_set_nn_mode(uwg_stokes *st, int mode);
  * min_drop (fraction) over every `window` iterations, else the plain preconditioner takes over
  * (defaults 20 and 0.01) */
 int uwg_stokes_set_nn_monitor(uwg_stokes *st, int window, double min_drop);
+/* second fallback test (no counterpart in the reference): the plain preconditioner also takes
+ * over when the FP64 outer residual after a Krylov restart exceeds `factor` times the previous
+ * cycle's final estimate -- the error left is then Arnoldi-basis rounding, which the diagonal
+ * Schur block removes faster than the learned one (default 1.25; 0 disables the test) */
+int uwg_stokes_set_nn_restart_rise(uwg_stokes *st, double factor);
 /* diagnostic, no counterpart in the reference: with rtol > 0 the coupled FP64 solve (method 1,
  * precision 0) applies K^-1 in its block preconditioner by an inner FGMRES + V-cycle solve to
  * rtol instead of one V-cycle, so its outer count is the floor set by the Schur-block choice

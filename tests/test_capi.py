@@ -37,6 +37,18 @@ def test_binding_covers_header():
     _lib.lib()
 
 
+def test_fallback_monitor_setters_reject_bad_arguments():
+    """The learned-preconditioner monitor setters check their arguments before they touch the
+    handle, so the checks run here without a GPU (a null handle)."""
+    from underworld2_amd import _lib
+    L = _lib.lib()
+    for factor in (0.5, -1.0, float("nan"), float("inf")):
+        assert L.uwg_stokes_set_nn_restart_rise(None, factor) != 0
+        assert "factor" in L.uwg_last_error().decode()
+    assert L.uwg_stokes_set_nn_monitor(None, 0, 0.01) != 0
+    assert L.uwg_stokes_set_nn_monitor(None, 20, 1.0) != 0
+
+
 def test_no_oracle_in_product():
     """The product package never imports or loads the oracle."""
     pkg = os.path.join(ROOT, "underworld2_amd")

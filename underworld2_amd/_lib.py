@@ -103,6 +103,7 @@ SIGNATURES = {
     "uwg_stokes_load_nn": [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int],
     "uwg_stokes_set_nn_mode": [_vp, ctypes.c_int],
     "uwg_stokes_set_nn_monitor": [_vp, ctypes.c_int, ctypes.c_double],
+    "uwg_stokes_set_nn_restart_rise": [_vp, ctypes.c_double],
     "uwg_stokes_set_velocity_block_solve": [_vp, ctypes.c_double],
     "uwg_stokes_capture": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "uwg_stokes_get_captured": [_vp, _fp, _ip],

@@ -208,6 +208,8 @@ struct uwg_stokes {
   bool nn_disabled = false;      // set when the FP64 residual shows the model is not helping
   int nn_window = 20;            // fallback monitor: over nn_window iterations the residual must
   double nn_min_drop = 0.01;     // fall by at least this fraction while the network is in use
+  double nn_rise = 1.25;         // and, across a restart, may not rise above this factor of the
+                                 // previous cycle's final estimate (0: no such test)
   double exact_vel_rtol = 0.0;   // > 0: the coupled FP64 preconditioner's velocity block is an
                                  // inner solve to this rtol instead of one V-cycle (diagnostic)
   int nn_mode = 0;               // 0: replace the coarse cycle, 1: add to it
@@ -2189,16 +2191,29 @@ static int solve_block(uwg_stokes* st, const uwg_solve_opts& o, uwg_solve_stats*
   // (FGMRES admits a preconditioner that changes between iterations)
   // hist[k] is the relative residual after iteration k (hist[0] = 1, the start), so the test
   // compares the residual with the one nn_window iterations earlier
+  //
+  // second test: within a cycle the Arnoldi estimate never rises, so a rise from one iteration
+  // to the next is the first estimate after a restart exceeding the last one before it.  With
+  // the FP32 Arnoldi basis, long cycles under the learned block accumulate basis rounding that
+  // the estimate does not see (128^3 sinker, restart 70: 1.28e-5 estimated, 2.09e-5 after the
+  // restart; 96^3 convection: rises of at most 1.01x).  A rise above nn_rise means the error
+  // left is that rounding rather than smooth error, which the diagonal Schur block removes in a
+  // few iterations (sinker: 89 iterations instead of 172 at restart 70) and the learned
+  // stencil, fitted to smooth error, removes slowly.
   std::vector<double> hist{1.0};
   std::function<void(int, double)> mon = [&](int k, double rel) {
     hist.push_back(rel);
     const int wdw = st->nn_window;
     const bool learned_on = st->nn || st->snn;
-    if (o.learned && learned_on && !st->nn_disabled && k >= wdw && k < (int)hist.size() &&
-        rel > (1.0 - st->nn_min_drop) * hist[k - wdw]) {
+    if (!o.learned || !learned_on || st->nn_disabled || k < 1 || k >= (int)hist.size()) return;
+    const bool stalled = k >= wdw && rel > (1.0 - st->nn_min_drop) * hist[k - wdw];
+    const bool rose = st->nn_rise > 0.0 && rel > st->nn_rise * hist[k - 1];
+    if (stalled || rose) {
       st->nn_disabled = true;
       S->fallback_used = 1;
-      if (g_verbose) fprintf(stderr, " learned correction disabled at iteration %d (rel %.3e)\n", k, rel);
+      if (g_verbose)
+        fprintf(stderr, " learned correction disabled at iteration %d (rel %.3e, %s)\n", k, rel,
+                rose ? "rise after restart" : "stalled");
     }
   };
   st->nn_disabled = !o.learned;
@@ -2711,6 +2726,12 @@ extern "C" int uwg_stokes_set_nn_monitor(uwg_stokes* st, int window, double min_
   if (window < 1 || !(min_drop >= 0.0 && min_drop < 1.0)) return arg_error("window >= 1 and 0 <= min_drop < 1");
   st->nn_window = window;
   st->nn_min_drop = min_drop;
+  return 0;
+}
+
+extern "C" int uwg_stokes_set_nn_restart_rise(uwg_stokes* st, double factor) {
+  if (!(factor == 0.0 || (factor >= 1.0 && factor < 1e300))) return arg_error("factor must be 0 or >= 1");
+  st->nn_rise = factor;
   return 0;
 }
 
