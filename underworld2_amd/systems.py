@@ -410,8 +410,8 @@ class StokesSolver:
     def set_restart(self, restart):
         """FGMRES restart length (engine-specific): the outer Schur iteration's (default 30, the
         reference's PETSc default) or the coupled iteration's (default 60; 70 with the learned
-        Schur stencil, whose solves then finish with the diagonal Schur block once the residual
-        after a restart rises above the previous cycle's estimate)."""
+        Schur stencil in mixed precision, whose FP32 Arnoldi basis makes the residual rise across
+        the first restart, after which the diagonal Schur block finishes the solve)."""
         if not isinstance(restart, int) or not 2 <= restart <= 200:
             raise ValueError("restart must be an int in [2, 200]")
         self._restart = restart
@@ -449,10 +449,12 @@ class StokesSolver:
         elif self._method == 0:
             o.restart = 30
         else:
-            # coupled: 60; with the learned Schur stencil 70, long enough for the FP32-basis
-            # rounding to show as a rise after the first restart (128^3 sinker, rtol 1e-6:
-            # 89 iterations against 163 at 60, 97 at 80)
-            o.restart = 70 if (self._learned and self._learned_stencil) else 60
+            # coupled: 60; with the learned Schur stencil in mixed precision 70, long enough
+            # for the FP32-basis rounding to show as a rise after the first restart (128^3
+            # sinker, rtol 1e-6: 89 iterations against 163 at 60, 97 at 80).  In FP64 the
+            # basis shows no such rise, and 60 keeps tight-tolerance solves on their accuracy.
+            stencil = self._learned and self._learned_stencil
+            o.restart = 70 if (stencil and self._precision == 1) else 60
         max_it = getattr(self.options.scr, "ksp_max_it", None)  # PETSc's option name; default 10,000
         if max_it is not None:
             o.max_outer_its = int(max_it)
