@@ -71,31 +71,46 @@ __global__ void k_conv_fwd(int B, int Ci, int Co, Grid3 g, const float* __restri
 }
 
 // Channel-blocked forward: each thread produces COB output channels of one node, so every
-// input value fetched serves COB FMAs; the weight index depends only on the block's channel
-// group and the loop counters (wave-uniform), so weights come through scalar loads.
+// input value fetched serves COB FMAs.  The block's weights for one input channel (27 taps x
+// COB output channels) are staged in LDS and read as broadcasts (every lane the same address,
+// 16 bytes per read).  Read straight from global memory they were wave-uniform scalar loads,
+// one per weight (weights of consecutive output channels are Ci*27 apart), and the 432 loaded
+// values per input channel overflowed the scalar registers: about 4,400 instructions per
+// channel for 432 FMAs.  The FMA order (input channel, then taps) is unchanged, so the sums are
+// the same bit for bit.
 template <int COB>
 __global__ __launch_bounds__(256) void k_conv_fwd_blk(int B, int Ci, int Co, Grid3 g, const float* __restrict__ in,
                                                       const float* __restrict__ w, const float* __restrict__ bias,
                                                       float* __restrict__ out, int relu) {
+  __shared__ __align__(16) float ws[27 * COB];  // [tap][c]
   const long N = g.N();
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int co0 = blockIdx.y * COB;
-  if (t >= (long)B * N) return;
-  const long p = t % N;
-  const int b = (int)(t / N);
+  // threads past the last node still stage weights and reach every barrier
+  const bool active = t < (long)B * N;
+  const long p = active ? t % N : 0;
+  const int b = active ? (int)(t / N) : 0;
   const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((long)g.nx * g.ny));
   float acc[COB];
 #pragma unroll
   for (int c = 0; c < COB; c++) acc[c] = bias[co0 + c];
   const long cs = (long)Ci * 27;
   for (int ci = 0; ci < Ci; ci++) {
+    __syncthreads();  // the previous channel's weights are no longer read
+    for (int q = threadIdx.x; q < 27 * COB; q += blockDim.x) {
+      const int c = q % COB, tap = q / COB;
+      ws[q] = w[(long)(co0 + c) * cs + ci * 27 + tap];
+    }
+    __syncthreads();
+    if (!active) continue;
     const float* src = in + ((long)b * Ci + ci) * N;
-    const float* wc = w + (long)co0 * cs + ci * 27;
-#pragma unroll
+    // dz and dy stay loops: fully unrolled, the compiler hoists all 27 taps' weight reads and
+    // the kernel needs 496 registers per lane (one wave per SIMD)
+#pragma unroll 1
     for (int dz = -1; dz <= 1; dz++) {
       const int kk = k + dz;
       const bool okz = kk >= 0 && kk < g.nz;
-#pragma unroll
+#pragma unroll 1
       for (int dy = -1; dy <= 1; dy++) {
         const int jj = j + dy;
         const bool oky = okz && jj >= 0 && jj < g.ny;
@@ -105,11 +120,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd_blk(int B, int Ci, int Co, Gri
           const float v = (oky && ii >= 0 && ii < g.nx) ? src[ii + (long)g.nx * (jj + (long)g.ny * kk)] : 0.0f;
           const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
 #pragma unroll
-          for (int c = 0; c < COB; c++) acc[c] += wc[c * cs + tap] * v;
+          for (int c = 0; c < COB; c++) acc[c] += ws[tap * COB + c] * v;
         }
       }
     }
   }
+  if (!active) return;
 #pragma unroll
   for (int c = 0; c < COB; c++) {
     const float a = acc[c];
