@@ -54,3 +54,48 @@ def test_make_sample_scaling_and_layout():
     assert np.allclose(t.T.reshape(-1) * rms, x, rtol=1e-6)
     assert np.array_equal(xi[3], feat)
     assert abs(np.sqrt(np.mean(xi[:3].astype(np.float64) ** 2)) - 1.0) < 1e-6
+
+
+def test_restart_defaults_follow_method_precision_and_model():
+    """FGMRES restart chosen by the solver front end (host logic, no GPU): 30 for the Schur
+    method, 60 for the coupled one, 70 with the learned Schur stencil in mixed precision only
+    (the FP32 Arnoldi basis's restart rise hands the tail to approxS there), 60 with the stencil
+    in FP64 or with the coarse correction; an explicit restart always wins."""
+    from underworld2_amd import systems
+
+    class _Stencil(learned.LearnedSchurStencil):
+        def __init__(self):
+            pass
+
+        def attach(self, stokes):  # the handle is not needed for the options
+            pass
+
+    class _Coarse(learned.LearnedCoarseCorrection):
+        def __init__(self):
+            pass
+
+        def attach(self, stokes, level_res=None):
+            pass
+
+    def restart(method, precision, model=None, explicit=None):
+        s = systems.StokesSolver(systems.Stokes.__new__(systems.Stokes))
+        s.set_method(method)
+        s.set_precision(precision)
+        s.set_learned(model)
+        if explicit is not None:
+            s.set_restart(explicit)
+        return s._opts().restart
+
+    assert restart("schur", "fp64") == 30
+    assert restart("block", "fp64") == 60
+    assert restart("block", "mixed") == 60
+    assert restart("block", "mixed", _Stencil()) == 70
+    assert restart("block", "fp64", _Stencil()) == 60
+    assert restart("block", "mixed", _Coarse()) == 60
+    assert restart("block", "mixed", _Stencil(), explicit=40) == 40
+    s = systems.StokesSolver(systems.Stokes.__new__(systems.Stokes))
+    s.set_method("block")
+    s.set_precision("mixed")
+    s.set_learned(_Stencil())
+    s.set_learned(None)  # detaching returns to the plain default
+    assert s._opts().restart == 60
