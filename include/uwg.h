@@ -202,9 +202,10 @@ int uwg_stokes_set_nn_mode(uwg_stokes *st, int mode);
  * (defaults 20 and 0.01) */
 int uwg_stokes_set_nn_monitor(uwg_stokes *st, int window, double min_drop);
 /* second fallback test (no counterpart in the reference): the plain preconditioner also takes
- * over when the FP64 outer residual after a Krylov restart exceeds `factor` times the previous
- * cycle's final estimate -- the error left is then Arnoldi-basis rounding, which the diagonal
- * Schur block removes faster than the learned one (default 1.25; 0 disables the test) */
+ * over when the first residual estimate after a Krylov restart (one iteration from the
+ * recomputed FP64 residual) exceeds `factor` times the previous cycle's final estimate -- the
+ * error left is then Arnoldi-basis rounding, which the diagonal Schur block removes faster
+ * than the learned one (default 1.25; 0 disables the test) */
 int uwg_stokes_set_nn_restart_rise(uwg_stokes *st, double factor);
 /* diagnostic, no counterpart in the reference: with rtol > 0 the coupled FP64 solve (method 1,
  * precision 0) applies K^-1 in its block preconditioner by an inner FGMRES + V-cycle solve to
