@@ -1264,7 +1264,6 @@ static void capture_vector(uwg_stokes* st, long n, const T* v) {
 template <typename T>
 static void vcycle(uwg_stokes* st, int l, const T* b, T* x) {
   Level& A = st->L[l];
-  const long n = A.n();
   if (l == (int)st->L.size() - 1) {
     coarse_solve<T>(st, A, b, x);
     return;
